@@ -1,0 +1,253 @@
+"""Benchmark: the single drop-in shear_fdtd solve on one MI355X.
+
+One bench step is one complete solve of the 256^3 workload with every input
+already resident in HBM: the Helmholtz-Hodge projection of the push (CG to
+1e-8, float64 reductions), 2000 fused time steps (float32 storage, increment
+form), and the four strain invariants every 20 steps into a device-resident
+ring. ``value`` is whole-solve throughput in Gvoxel-steps/s (256^3 * 2000 /
+seconds per solve, projection and strain included).
+
+The JSON line also carries the roofline of the dominant kernel (the
+force-free viscous time step, timed with HIP events on the stream it runs on)
+and a CPU baseline (the NumPy oracle in float64 on a bounded sample of the
+same loop, one core).
+
+    python bench.py [--steps K] [--warmup W] [--n 256] [--nt 2000] [--no-cpu] [--fp64]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "Gvoxel-steps/s & %HBM roofline; batched solves/s + p50/p99 at 1e-5 rel err"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--nt", type=int, default=2000)
+    ap.add_argument("--dtype", default="float32", choices=["float32", "float64"])
+    ap.add_argument("--fp64", action="store_true", help="also time one float64 solve")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--kernel-steps", type=int, default=200)
+    ap.add_argument("--kernel", default="auto")
+    ap.add_argument("--tile-y", type=int, default=0)
+    ap.add_argument("--x-chunk", type=int, default=0)
+    return ap.parse_args()
+
+
+def traffic_from_profile():
+    """HBM bytes per step-kernel launch from the committed PMC summary, if present."""
+    path = os.path.join(ROOT, "profiles", "fdtd_step_256_fp32_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("hbm_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+class DropIn:
+    """Resident inputs + state of one drop-in solve (the bench step)."""
+
+    def __init__(self, w, dtype, kernel, tile_y, x_chunk):
+        import torch
+
+        from shearwave_amd import engine as E
+
+        self.w = w
+        self.E = E
+        self.solve = E.Solve(w["shape"], (w["h"],) * 3, w["dt"], w["rho"], w["mu"], w["eta"], dtype=dtype,
+                             form="increment", kernel=kernel, tile_y=tile_y, x_chunk=x_chunk)
+        self.bz = E.to_device(w["bz"], dtype, w["shape"])
+        self.n_inv = w["n_steps"] // w["strain_stride"]
+        self.inv = torch.empty((self.n_inv, 4) + tuple(w["shape"]), dtype=E.torch_dtype(dtype), device=E.device())
+        self.env = w["env"]
+        self.last_cg = None
+
+    def run(self):
+        s = self.solve
+        s.step_count = 0
+        s.set_initial(None, None)
+        info, _ = s.set_force(None, None, self.bz, self.w["tol"], self.w["max_iters"])
+        self.last_cg = info
+        s.run(self.w["n_steps"], self.env, inv_stride=self.w["strain_stride"], inv_out=self.inv)
+
+
+def time_step_kernel(solve, n_launch):
+    """Average duration of the force-free step kernel, HIP events on the launch stream."""
+    import torch
+
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    solve.run(10)  # warm
+    start.record()
+    solve.run(n_launch)
+    stop.record()
+    stop.synchronize()
+    return start.elapsed_time(stop) / 1e3 / n_launch
+
+
+def cpu_baseline(w, n_steps):
+    """The float64 oracle on `n_steps` force-free steps of the same 256^3 loop (one core)."""
+    from oracle import solver as O
+
+    shape = w["shape"]
+    h = w["h"]
+    rho = w["rho"]
+    m = w["mu"] / rho
+    e = w["eta"] / rho
+    dt = w["dt"]
+    rng = np.random.default_rng(0)
+    mask = O.interior_mask(shape, np.float64)
+    u_prev = 1e-9 * rng.standard_normal(shape + (3,)) * mask[..., None]
+    u = 1e-9 * rng.standard_normal(shape + (3,)) * mask[..., None]
+
+    def one():
+        nonlocal u_prev, u
+        vel = (u - u_prev) / dt * mask[..., None]
+        nxt = np.empty_like(u)
+        for c in range(3):
+            acc = m * O.laplacian_center(u[..., c], h, h, h) + e * O.laplacian_center(vel[..., c], h, h, h)
+            nxt[..., c] = 2.0 * u[..., c] - u_prev[..., c] + dt * dt * acc
+        nxt *= mask[..., None]
+        u_prev, u = u, nxt
+
+    one()  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(n_steps):
+        one()
+    t = time.perf_counter() - t0
+    vox = int(np.prod(shape))
+    return {
+        "value": vox * n_steps / t / 1e9,
+        "unit": "Gvoxel-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{n_steps} force-free viscous time steps of the {shape[0]}^3 loop in float64 (NumPy oracle "
+                  f"restating solver.py:194-234) after 1 warm-up step; {t:.1f} s; projection not included",
+        "seconds": t,
+        "numpy": np.__version__,
+        "host": platform.processor() or platform.machine(),
+        "os_cpu_count": os.cpu_count(),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported by this build "
+              "(a multi-GPU layout is described in DESIGN.md, not implemented)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from shearwave_amd import workloads as W
+
+    w = W.dropin_256(n=args.n, n_steps=args.nt)
+    item = 4 if args.dtype == "float32" else 8
+    vox = int(np.prod(w["shape"]))
+    bench = DropIn(w, args.dtype, args.kernel, args.tile_y, args.x_chunk)
+    for _ in range(args.warmup):
+        bench.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        bench.run()
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    per_solve = t / args.steps
+    value = vox * w["n_steps"] / per_solve / 1e9
+
+    # projection alone and the loop alone (one extra solve, phase split)
+    s = bench.solve
+    torch.cuda.synchronize()
+    tp0 = time.perf_counter()
+    s.set_initial(None, None)
+    s.set_force(None, None, bench.bz, w["tol"], w["max_iters"])
+    torch.cuda.synchronize()
+    t_proj = time.perf_counter() - tp0
+
+    kt = time_step_kernel(s, args.kernel_steps)
+    alg_bytes = W.step_bytes(item, "increment", viscous=True, forced=False) * vox
+    achieved = alg_bytes / kt / 1e9
+    traffic = traffic_from_profile() if (args.n == 256 and args.dtype == "float32") else None
+
+    rec = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gvoxel-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(per_solve * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.dtype == "float32" else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"single drop-in shear_fdtd solve, {args.n}^3 grid, h=0.25 mm, 8 seeded inclusions, "
+                        f"per-voxel rho and eta, Gaussian push, nT={w['n_steps']}, CG tol 1e-8, strain invariants "
+                        f"every 20 steps, {args.dtype} storage (increment form), inputs resident in HBM",
+            "grid": list(w["shape"]),
+            "n_steps": w["n_steps"],
+            "dt": w["dt"],
+            "dt_cfl": w["dt_cfl"],
+            "seed": 2,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "kernel": "step_march_kernel (force-free, viscous)" if args.kernel != "direct" else "step_direct_kernel",
+            "alg_bytes_per_voxel": W.step_bytes(item, "increment", viscous=True, forced=False),
+            "avg_launch_us": round(kt * 1e6, 2),
+        },
+        "phases": {
+            "projection_s": round(t_proj, 4),
+            "cg_iters": bench.last_cg["iters"],
+            "cg_relres": bench.last_cg["relres"],
+            "loop_gvox_steps_per_s": round(vox / kt / 1e9, 2),
+        },
+    }
+    if args.fp64:
+        b64 = DropIn(w, "float64", args.kernel, 0, 0)
+        b64.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b64.run()
+        torch.cuda.synchronize()
+        t64 = time.perf_counter() - t0
+        kt64 = time_step_kernel(b64.solve, args.kernel_steps // 2)
+        ab64 = W.step_bytes(8, "increment", viscous=True, forced=False) * vox
+        rec["fp64"] = {"value": round(vox * w["n_steps"] / t64 / 1e9, 3), "ms_per_solve": round(t64 * 1e3, 2),
+                       "step_kernel_GBs": round(ab64 / kt64 / 1e9, 1), "frac": round(ab64 / kt64 / 1e9 / HBM_PEAK_GBS, 4)}
+        del b64
+    if not args.no_cpu:
+        rec["cpu_baseline"] = cpu_baseline(w, args.cpu_steps)
+    else:
+        rec["cpu_baseline"] = None
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

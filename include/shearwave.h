@@ -1,0 +1,209 @@
+/*
+ * shearwave.h - C ABI of the MI355X (gfx950) shear_fdtd hot path.
+ *
+ * Every entry point is extern "C", takes plain device pointers (caller-owned,
+ * SoA: one contiguous C-order (nx, ny, nz) array per component, z fastest),
+ * sizes and an explicit hipStream_t passed as void*, and returns an int status
+ * (SW_OK = 0). On failure, sw_last_error() returns a thread-local message.
+ * No C++ exception crosses the boundary. The library allocates no device
+ * memory: scratch space is passed in as a workspace.
+ *
+ * The reference (pinton-lab/shearwave) is pure Python; each declaration names
+ * the Python function it replaces as file:line in src/shearwave/.
+ */
+#ifndef SHEARWAVE_H
+#define SHEARWAVE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SW_ABI_VERSION 1
+
+/* status codes */
+#define SW_OK 0
+#define SW_ERR_ARG 1          /* invalid argument (shape, pointer, option) */
+#define SW_ERR_HIP 2          /* HIP runtime error */
+#define SW_ERR_WORKSPACE 3    /* workspace too small */
+#define SW_ERR_UNSUPPORTED 4  /* option combination not supported */
+
+/* storage types */
+#define SW_F32 0
+#define SW_F64 1
+
+/* time-loop state forms */
+#define SW_FORM_INCREMENT 0 /* stores u_n and d_n = u_n - u_{n-1} */
+#define SW_FORM_LEAPFROG 1  /* stores u_n and u_{n-1}, as solver.py:148-150, 234 */
+
+/* time-step kernel choice */
+#define SW_KERNEL_AUTO 0
+#define SW_KERNEL_DIRECT 1  /* one voxel per thread, neighbours through the caches */
+#define SW_KERNEL_MARCH 2   /* x-marching LDS tiles with register queues */
+
+/* CG stop reasons */
+#define SW_CG_RUNNING 0
+#define SW_CG_CONVERGED 1   /* sqrt(rs/rs0) < tol            (solver.py:1044) */
+#define SW_CG_DENOM 2       /* p.Ap non-finite or <= 0         (solver.py:1036) */
+#define SW_CG_NONFINITE 3   /* r.r non-finite                  (solver.py:1042) */
+#define SW_CG_MAX_ITERS 4   /* loop exhausted                  (solver.py:1033) */
+#define SW_CG_EARLY_EXIT 5  /* max|div f| < 1e-12, no solve    (solver.py:672)  */
+
+/* Last error message of the calling thread ("" when none). */
+const char* sw_last_error(void);
+
+/* SW_ABI_VERSION of the loaded library. */
+int sw_abi_version(void);
+
+/* Number of compute units and the gfx architecture name of the current device. */
+int sw_device_info(int* cu_count, char* arch, int arch_len);
+
+/* ---------------------------------------------------------------- stencils */
+
+/* laplacian_center (solver.py:686-695): compact 7-point Laplacian, 0 on the
+ * boundary layer. dtype SW_F32 or SW_F64; f and out have n = nx*ny*nz elements. */
+int sw_laplacian_center(int dtype, const void* f, void* out, int64_t nx, int64_t ny, int64_t nz,
+                        double dx, double dy, double dz, void* stream);
+
+/* divergence_center (solver.py:708-717): 2h central divergence, 0 on the boundary layer. */
+int sw_divergence_center(int dtype, const void* fx, const void* fy, const void* fz, void* out,
+                         int64_t nx, int64_t ny, int64_t nz, double dx, double dy, double dz, void* stream);
+
+/* gradient_center (solver.py:720-729): 2h central gradient, 0 on the boundary layer. */
+int sw_gradient_center(int dtype, const void* phi, void* gx, void* gy, void* gz, int64_t nx, int64_t ny,
+                       int64_t nz, double dx, double dy, double dz, void* stream);
+
+/* ------------------------------------------------------------- projection */
+
+typedef struct sw_cg_info {
+    int64_t iters;   /* number of phi updates performed */
+    double relres;   /* last sqrt(r.r / rs0) */
+    double max_div;  /* max |div_c(mask f)| of the input */
+    int32_t reason;  /* SW_CG_* */
+    int32_t pad;
+} sw_cg_info;
+
+/* Bytes of device workspace sw_project_vector_field needs for an nx*ny*nz grid. */
+int64_t sw_project_workspace_bytes(int64_t nx, int64_t ny, int64_t nz);
+
+/* project_vector_field (solver.py:644-683) with poisson_cg (solver.py:1010-1050).
+ * Overwrites fx, fy, fz (dtype storage) with mask*(mask*f - grad_c phi), phi
+ * solving -div_c grad_c phi = -div_c(mask*f) by CG on the interior. The CG
+ * vectors, dot products and residual norm are float64 in both storage types.
+ * If scale_out is non-NULL (three arrays of dtype), it also receives
+ * scale * f_S * inv_rho (inv_rho: dtype array, or NULL for 1). `info` is a
+ * host pointer, filled after the call synchronises `stream`. */
+int sw_project_vector_field(int dtype, void* fx, void* fy, void* fz, int64_t nx, int64_t ny, int64_t nz,
+                            double dx, double dy, double dz, double tol, int64_t max_iters, void* workspace,
+                            int64_t workspace_bytes, void* const* scale_out, const void* inv_rho, double scale,
+                            sw_cg_info* info, void* stream);
+
+/* ---------------------------------------------------------------- time loop */
+
+/* Material coefficients (solver.py:124-145, 1072-1079): from float64 rho, mu
+ * and eta arrays (eta may be NULL) writes mu/rho, eta/rho and 1/rho in dtype. */
+int sw_prepare_coefficients(int dtype, const double* rho, const double* mu, const double* eta, void* mu_over_rho,
+                            void* eta_over_rho, void* inv_rho, int64_t n, void* stream);
+
+typedef struct sw_fdtd_state {
+    int32_t dtype;  /* SW_F32 / SW_F64 */
+    int32_t form;   /* SW_FORM_* */
+    int64_t nx, ny, nz;
+    double dx, dy, dz, dt;
+    double vel_damp;
+    const void* mu_over_rho;  /* (nx,ny,nz) */
+    const void* eta_over_rho; /* (nx,ny,nz) or NULL when inviscid */
+    const void* force[3];     /* dt^2 * f_S / rho per component, or NULL */
+    void* u[3];               /* u_n                                 (in/out, rotated) */
+    void* b[3];               /* d_n (increment) or u_{n-1} (leapfrog)  (in/out, rotated) */
+    void* u_next[3];          /* scratch for u_{n+1}                 (rotated) */
+    void* b_next[3];          /* scratch for d_{n+1} (increment form only; rotated) */
+    int32_t kernel;           /* SW_KERNEL_* */
+    int32_t tile_y;           /* march kernel: y rows per tile (4 or 8; 0 = auto) */
+    int32_t x_chunk;          /* march kernel: x planes per tile (0 = auto) */
+    int32_t pad;
+} sw_fdtd_state;
+
+/* Optional per-step outputs of sw_fdtd_run. Step indices are global:
+ * step0 is the index of the first step of this call, so a solve run in
+ * several calls (host callback, reprojection) numbers its outputs as one. */
+typedef struct sw_fdtd_outputs {
+    int64_t step0;
+    /* every snap_stride global steps (0 = never): u as AoS (nx,ny,nz,3) into
+     * snap_out + k*3n, k = (step+1)/snap_stride - 1, as shear_fdtd_jax's
+     * snapshot_stride (solver.py:469-507) */
+    int64_t snap_stride;
+    void* snap_out;
+    /* every inv_stride global steps (0 = never): the four strain invariants of
+     * u (strain.py:16-135) as SoA (4,nx,ny,nz) into inv_out + k*4n */
+    int64_t inv_stride;
+    void* inv_out;
+    /* after every step: u_z at n_traces flat voxel indices (device int64
+     * array) into trace_out[k*trace_ld + step], as shear_fdtd_jax's
+     * trace_indices / return_traces (solver.py:396-413, 444-447) */
+    const int64_t* trace_idx;
+    int64_t n_traces;
+    void* trace_out;
+    int64_t trace_ld;
+} sw_fdtd_outputs;
+
+/* The time loop of shear_fdtd (solver.py:194-239) for n_steps steps with
+ * separable forcing f(x,t) = f_S(x) env[it] (solver.py:189-197). env is a host
+ * array of n_steps doubles for these steps (NULL = all zero). Rotates the
+ * state's buffers in place: after the call u/b hold the state after the last
+ * step. `out` may be NULL. */
+int sw_fdtd_run(sw_fdtd_state* st, int64_t n_steps, const double* env, const sw_fdtd_outputs* out, void* stream);
+
+/* Initial state (solver.py:147-171): u = mask*u0 (u0 AoS (nx,ny,nz,3) or NULL
+ * for zero); u_prev = mask*(u - dt*v0) when v0 (AoS) is given, else 0. Writes
+ * u and, per form, b = u - u_prev (increment) or b = u_prev (leapfrog). */
+int sw_import_state(int dtype, int form, const void* u0, const void* v0, double dt, void* const* u, void* const* b,
+                    int64_t nx, int64_t ny, int64_t nz, void* stream);
+
+/* Returned fields (solver.py:241-243): u as AoS and v = (u - u_prev)/dt as
+ * AoS, i.e. b*inv_dt (increment) or (u - b)*inv_dt (leapfrog). Either output may be NULL. */
+int sw_export_state(int dtype, int form, const void* const* u, const void* const* b, double inv_dt, void* u_aos,
+                    void* v_aos, int64_t n, void* stream);
+
+/* out[c] = alpha*a[c] + beta*b[c] for the three components of n elements
+ * (state conversions around reprojection, solver.py:236-239). */
+int sw_combine3(int dtype, const void* const* a, const void* const* b, double alpha, double beta, void* const* out,
+                int64_t n, void* stream);
+
+/* out[k] = f[idx[k]] for k < count (device int64 indices). */
+int sw_gather_traces(int dtype, const void* f, const int64_t* idx, int64_t count, void* out, int64_t out_stride,
+                     void* stream);
+
+/* ------------------------------------------------------------------ strain */
+
+/* compute_strain_tensor (strain.py:16-70): Voigt [exx,eyy,ezz,gyz,gxz,gxy],
+ * engineering shear, written AoS (nx,ny,nz,6). */
+int sw_strain_tensor(int dtype, const void* ux, const void* uy, const void* uz, void* eps, int64_t nx, int64_t ny,
+                     int64_t nz, double dx, double dy, double dz, void* stream);
+
+/* compute_strain_invariants (strain.py:73-135) on `count` Voigt 6-vectors (AoS):
+ * volumetric, deviatoric, max_shear (dtype) and von_mises, which is float64 for
+ * both dtypes because the reference multiplies by np.sqrt(2/3), a float64
+ * scalar, and NumPy 2 promotes the float32 result (strain.py:123). */
+int sw_strain_invariants(int dtype, const void* eps, int64_t count, void* vol, void* dev, void* max_shear,
+                         void* von_mises, void* stream);
+
+/* Fused compute_strain_invariants(compute_strain_tensor(u)): the four
+ * invariants straight from the displacement, without the tensor in memory. */
+int sw_strain_invariants_from_u(int dtype, const void* ux, const void* uy, const void* uz, void* vol, void* dev,
+                                void* max_shear, void* von_mises, int64_t nx, int64_t ny, int64_t nz, double dx,
+                                double dy, double dz, void* stream);
+
+/* ------------------------------------------------------------------ layout */
+
+/* (count, 3) AoS <-> three SoA arrays of `count` elements (solver.py:148 keeps
+ * vectors as (nx,ny,nz,3); the kernels use SoA). */
+int sw_aos3_to_soa(int dtype, const void* aos, void* x, void* y, void* z, int64_t count, void* stream);
+int sw_soa_to_aos3(int dtype, const void* x, const void* y, const void* z, void* aos, int64_t count, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SHEARWAVE_H */

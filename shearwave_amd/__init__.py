@@ -1,0 +1,49 @@
+"""shearwave_amd: shearwave's shear_fdtd hot path, MI355X-native.
+
+The viscoelastic shear FDTD time loop, its Helmholtz-Hodge force projection
+and the strain post-processing of pinton-lab/shearwave, re-implemented as
+hand-written gfx950 HIP kernels behind a C ABI (``include/shearwave.h``) and a
+Python drop-in with the reference's API. Build the library with
+``python -m shearwave_amd.build``.
+"""
+
+from ._abi import NativeLibraryMissing, SwError, lib
+from .api import (
+    compute_strain_invariants,
+    compute_strain_tensor,
+    divergence_center,
+    gradient_center,
+    hann1d,
+    laplacian_center,
+    project_body_force_to_shear,
+    project_vector_field,
+    shear_fdtd,
+    shear_fdtd_jax,
+)
+from .engine import Solve
+
+__version__ = "0.1.0"
+
+
+def require_native():
+    """Load the HIP library; raise NativeLibraryMissing if it is not built."""
+    return lib()
+
+
+__all__ = [
+    "NativeLibraryMissing",
+    "Solve",
+    "SwError",
+    "__version__",
+    "compute_strain_invariants",
+    "compute_strain_tensor",
+    "divergence_center",
+    "gradient_center",
+    "hann1d",
+    "laplacian_center",
+    "project_body_force_to_shear",
+    "project_vector_field",
+    "require_native",
+    "shear_fdtd",
+    "shear_fdtd_jax",
+]

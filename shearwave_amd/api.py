@@ -1,0 +1,396 @@
+"""Drop-in API of shearwave's hot path on MI355X.
+
+Same names, arguments, option keys, return values and error messages as the
+reference's ``src/shearwave/solver.py`` and ``src/shearwave/strain.py``
+(cited per function). The arithmetic runs in the gfx950 HIP library; inputs
+are NumPy arrays (or torch device tensors) and outputs are NumPy arrays in
+the reference's dtypes, or device tensors when asked for.
+
+Build-only options (namespaced ``sw_``):
+  * ``sw_dtype``: storage type, ``"float32"`` (default, as the reference) or ``"float64"``;
+  * ``sw_form``: ``"increment"`` (default) or ``"leapfrog"`` state storage;
+  * ``sw_kernel``: ``"auto"``, ``"march"`` or ``"direct"`` time-step kernel;
+  * ``sw_tile_y``, ``sw_x_chunk``: march-kernel tile shape (0 = automatic);
+  * ``sw_strain_stride``: with ``shear_fdtd(..., return_strain=True)``, the
+    four strain invariants every that many steps, computed in-loop.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import engine as E
+
+
+def _is_dev(x):
+    return isinstance(x, torch.Tensor)
+
+
+def _np_out(t, dtype=None):
+    a = t.detach().cpu().numpy()
+    return a if dtype is None else a.astype(dtype, copy=False)
+
+
+def _prepare_material(field, nx, ny, nz):
+    """Scalar or (nx, ny, nz) array check (solver.py:1072-1079); returns a float64 source."""
+    if np.isscalar(field):
+        return float(field)
+    if _is_dev(field):
+        if tuple(field.shape) != (nx, ny, nz):
+            raise ValueError("Material field has incompatible shape.")
+        return field
+    arr = np.asarray(field)
+    if arr.shape != (nx, ny, nz):
+        raise ValueError("Material field has incompatible shape.")
+    return arr
+
+
+def _check_body(comp, nx, ny, nz, n_steps):
+    """Shape check of _prepare_body_component (solver.py:1060-1069), without the repeat."""
+    if comp is None:
+        return None
+    shape = tuple(comp.shape)
+    if len(shape) == 3 and shape == (nx, ny, nz):
+        return comp
+    if len(shape) == 4 and shape == (nx, ny, nz, n_steps):
+        return comp
+    raise ValueError("Body-force component has incompatible shape.")
+
+
+def _trace_flat(trace_indices, shape):
+    if not isinstance(trace_indices, (tuple, list)) or len(trace_indices) != 3:
+        raise ValueError("trace_indices must be a 3-tuple: (ix, iy, iz).")
+    ix, iy, iz = (np.asarray(a, dtype=np.int64) for a in trace_indices)
+    if not (ix.shape == iy.shape == iz.shape):
+        raise ValueError("trace index arrays must share the same shape.")
+    if ix.ndim != 1:
+        raise ValueError("trace index arrays must be 1D.")
+    idx = []
+    for a, n in zip((ix, iy, iz), shape):
+        a = np.where(a < 0, a + n, a)
+        if np.any((a < 0) | (a >= n)):
+            raise IndexError("trace index out of bounds")
+        idx.append(a)
+    return (idx[0] * shape[1] + idx[1]) * shape[2] + idx[2]
+
+
+def shear_fdtd(  # noqa: PLR0912, PLR0915
+    bx,
+    by,
+    bz,
+    rho,
+    mu,
+    dX,
+    dY,
+    dZ,
+    dT,
+    n_steps,
+    opts=None,
+    callback=None,
+    *,
+    trace_indices=None,
+    return_traces=False,
+    return_device=False,
+    snapshot_stride=None,
+    return_strain=False,
+):
+    """3-D collocated shear FDTD with Kelvin-Voigt viscosity (solver.py:27-243) on the GPU.
+
+    Also takes the keyword arguments of shear_fdtd_jax (solver.py:246-566):
+    ``trace_indices`` / ``return_traces`` (u_z after every step at the given
+    voxels, shape (n_traces, n_steps)), ``snapshot_stride`` (u after every
+    stride steps, shape (n_steps//stride, nx, ny, nz, 3)) and
+    ``return_device`` (torch tensors left in HBM instead of NumPy arrays).
+    Returns (u, v), (u, v, snaps) or (u, v, traces); with ``return_strain`` a
+    dict of in-loop strain invariants (every ``opts['sw_strain_stride']``
+    steps, each (n_k, nx, ny, nz)) is appended.
+    """
+    ref = bx if bx is not None else (by if by is not None else bz)
+    if ref is None:
+        raise ValueError("At least one body-force component must be provided.")
+    if return_traces and trace_indices is None:
+        raise ValueError("trace_indices must be provided when return_traces=True.")
+    if snapshot_stride is not None:
+        if not isinstance(snapshot_stride, (int, np.integer)) or int(snapshot_stride) <= 0:
+            raise ValueError("snapshot_stride must be a positive integer.")
+        snapshot_stride = int(snapshot_stride)
+        if n_steps % snapshot_stride != 0:
+            raise ValueError(f"n_steps ({n_steps}) must be a multiple of snapshot_stride ({snapshot_stride}).")
+        if return_traces:
+            raise ValueError("return_traces and snapshot_stride are not both supported in v1.")
+
+    opts = opts or {}
+    vel_damp = opts.get("vel_damp", 0.0)
+    poisson_tol = opts.get("poisson_tol", 1e-5)
+    poisson_max_iters = opts.get("poisson_max_iters", 200)
+    reproject_every = int(opts.get("reproject_every", 0) or 0)
+    if reproject_every < 0:
+        raise ValueError("reproject_every must be a non-negative integer.")
+    verbose = opts.get("verbose", False)
+    progress_every = opts.get("progress_every", max(1, n_steps // 10))
+    dtype = E.dtype_name(opts.get("sw_dtype", "float32"))
+    np_dtype = np.float32 if dtype == "float32" else np.float64
+    form = opts.get("sw_form", "increment")
+    strain_stride = int(opts.get("sw_strain_stride", 0) or 0)
+    if return_strain and strain_stride <= 0:
+        raise ValueError("return_strain needs opts['sw_strain_stride'] > 0.")
+
+    nx, ny, nz = (int(s) for s in ref.shape[:3])
+    shape = (nx, ny, nz)
+    env_t = opts.get("env_t")
+    if env_t is not None:
+        env_t = np.asarray(env_t, dtype=np_dtype).ravel()
+        if env_t.shape[0] != n_steps:
+            raise ValueError(f"env_t length ({env_t.shape[0]}) must equal n_steps ({n_steps}).")
+        comps = [None if b is None else b for b in (bx, by, bz)]
+        per_step = False
+    else:
+        comps = [_check_body(b, nx, ny, nz, n_steps) for b in (bx, by, bz)]
+        per_step = any(c is not None and len(c.shape) == 4 for c in comps)
+
+    rho_f = _prepare_material(rho, nx, ny, nz)
+    mu_f = _prepare_material(mu, nx, ny, nz)
+    eta = opts.get("eta", 0.0)
+    if np.isscalar(eta):
+        eta = None if float(eta) == 0.0 else float(eta)
+    elif eta is not None:
+        eta = _prepare_material(eta, nx, ny, nz)
+
+    initial_u = opts.get("initial_u")
+    initial_v = opts.get("initial_v")
+    for name, a in (("initial_u", initial_u), ("initial_v", initial_v)):
+        if a is not None and tuple(a.shape) != shape + (3,):
+            raise ValueError(f"{name} must match shape (nx,ny,nz,3)")
+
+    trace_flat = None
+    if trace_indices is not None:
+        trace_flat = _trace_flat(trace_indices, shape)
+
+    solve = E.Solve(shape, (dX, dY, dZ), float(dT), rho_f, mu_f, eta, dtype=dtype, form=form,
+                    vel_damp=float(vel_damp), kernel=opts.get("sw_kernel", "auto"),
+                    tile_y=int(opts.get("sw_tile_y", 0)), x_chunk=int(opts.get("sw_x_chunk", 0)))
+    if initial_u is not None or initial_v is not None:
+        solve.set_initial(initial_u, initial_v)
+
+    dev = E.device()
+    tdt = E.torch_dtype(dtype)
+    snaps = None
+    if snapshot_stride:
+        snaps = torch.empty((n_steps // snapshot_stride,) + shape + (3,), dtype=tdt, device=dev)
+    inv = None
+    if return_strain:
+        inv = torch.empty((n_steps // strain_stride, 4) + shape, dtype=tdt, device=dev)
+    tr_idx = tr_out = None
+    if return_traces:
+        tr_idx = torch.from_numpy(trace_flat).to(dev)
+        tr_out = torch.zeros((trace_flat.shape[0], n_steps), dtype=tdt, device=dev)
+    outs = dict(snap_stride=snapshot_stride or 0, snap_out=snaps, inv_stride=strain_stride if return_strain else 0,
+                inv_out=inv, trace_idx=tr_idx, trace_out=tr_out)
+
+    if per_step:
+        env = np.ones(n_steps)
+    elif env_t is not None:
+        env = env_t.astype(np.float64)
+    else:
+        env = np.ones(n_steps)  # 3-D forces without env_t: the same projected force every step
+
+    if not per_step:
+        if verbose and env_t is not None:
+            print("Pre-computing Helmholtz projection (separable force)...")
+        solve.set_force(*comps, poisson_tol, poisson_max_iters)
+
+    if callback is None and not reproject_every and not per_step and not verbose:
+        solve.run(n_steps, env, **outs)
+    else:
+        it = 0
+        while it < n_steps:
+            if per_step:
+                k = 1
+                sl = [None if c is None else (c[..., it] if len(c.shape) == 4 else c) for c in comps]
+                solve.set_force(*sl, poisson_tol, poisson_max_iters)
+            elif callback is not None or verbose:
+                k = 1
+            else:  # reprojection only: run up to the next reprojection
+                k = reproject_every - (it % reproject_every)
+                k = min(k, n_steps - it)
+            solve.run(k, env[it:it + k], **outs)
+            it += k
+            if callback is not None:
+                callback(it - 1, _np_out(solve.u_aos()))
+            if verbose and it % progress_every == 0:
+                print(f"FDTD it={it}/{n_steps}")
+            if reproject_every and it % reproject_every == 0:
+                solve.reproject(poisson_tol, poisson_max_iters)
+
+    u_aos, v_aos = solve.export()
+    v_dtype = np.float64 if (dtype == "float32" and isinstance(dT, np.float64)) else None
+    if return_device:
+        res = [u_aos, v_aos if v_dtype is None else v_aos.to(torch.float64)]
+        if snaps is not None:
+            res.append(snaps)
+        if return_traces:
+            res.append(tr_out)
+    else:
+        res = [_np_out(u_aos), _np_out(v_aos, v_dtype)]
+        if snaps is not None:
+            res.append(_np_out(snaps))
+        if return_traces:
+            res.append(_np_out(tr_out))
+    if return_strain:
+        keys = ("volumetric", "deviatoric", "max_shear", "von_mises")
+        res.append({k: (inv[:, j] if return_device else _np_out(inv[:, j])) for j, k in enumerate(keys)})
+    return tuple(res)
+
+
+def shear_fdtd_jax(bx, by, bz, rho, mu, dX, dY, dZ, dT, n_steps, opts=None, callback=None, trace_indices=None,
+                   return_traces=False, return_device=False, snapshot_stride=None):
+    """shear_fdtd_jax's contract (solver.py:246-566), served by the same gfx950 engine.
+
+    Keeps its restrictions: no callback, no reproject_every; ``opts['jit']`` is accepted and ignored.
+    """
+    if callback is not None:
+        raise ValueError("callback is not supported in shear_fdtd_jax.")
+    if (opts or {}).get("reproject_every"):
+        raise NotImplementedError("reproject_every is only implemented in the NumPy solver (shear_fdtd).")
+    return shear_fdtd(bx, by, bz, rho, mu, dX, dY, dZ, dT, n_steps, opts, None, trace_indices=trace_indices,
+                      return_traces=return_traces, return_device=return_device, snapshot_stride=snapshot_stride)
+
+
+# ------------------------------------------------------------------ projection
+
+
+def project_vector_field(fx, fy, fz, dX, dY, dZ, tol, max_iters, *, dtype="float32", return_device=False, info=None):
+    """Divergence-free projection (solver.py:644-683) on the GPU.
+
+    Returns float32 arrays like the reference (``dtype="float64"`` keeps
+    float64). The CG runs in float64 on the device; ``info`` (a dict) receives
+    iters, relres, max_div and the stop reason.
+    """
+    shape = tuple(int(s) for s in fx.shape)
+    dev_in = _is_dev(fx)
+    f = torch.empty((3,) + shape, dtype=E.torch_dtype(dtype), device=E.device())
+    for c, comp in enumerate((fx, fy, fz)):
+        f[c].copy_(E.to_device(comp, dtype, shape))
+    rep = E.project_inplace(f, (dX, dY, dZ), tol, max_iters)
+    if info is not None:
+        info.update(rep)
+    if return_device or dev_in:
+        return f[0], f[1], f[2]
+    return tuple(_np_out(f[c]) for c in range(3))
+
+
+def project_body_force_to_shear(bz, dX, dY, dZ, tol=5e-5, max_iters=400, **kw):
+    """``project_vector_field(0, 0, bz)`` (solver.py:628-641)."""
+    z = np.zeros(tuple(bz.shape), dtype=np.float32)
+    return project_vector_field(z, z, bz, dX, dY, dZ, tol, max_iters, **kw)
+
+
+# -------------------------------------------------------------------- stencils
+
+
+def _field_dtype(*arrays):
+    dt = np.result_type(*[a.detach().cpu().numpy().dtype if _is_dev(a) else np.asarray(a).dtype for a in arrays])
+    return "float64" if dt == np.float64 else ("float32" if dt == np.float32 else "float64")
+
+
+def laplacian_center(field, dX, dY, dZ):
+    """Compact 7-point Laplacian, 0 on the boundary layer (solver.py:686-695)."""
+    from ._abi import check, lib
+
+    dt = _field_dtype(field)
+    shape = tuple(int(s) for s in field.shape)
+    f = E.to_device(field, dt, shape)
+    out = torch.empty_like(f)
+    check(lib().sw_laplacian_center(E.sw_code(dt), f.data_ptr(), out.data_ptr(), *shape, dX, dY, dZ, E.stream_ptr()))
+    return out if _is_dev(field) else _np_out(out)
+
+
+def divergence_center(fx, fy, fz, dX, dY, dZ):
+    """2h central divergence, 0 on the boundary layer (solver.py:708-717)."""
+    from ._abi import check, lib
+
+    dt = _field_dtype(fx, fy, fz)
+    shape = tuple(int(s) for s in fx.shape)
+    f = [E.to_device(a, dt, shape) for a in (fx, fy, fz)]
+    out = torch.empty_like(f[0])
+    check(lib().sw_divergence_center(E.sw_code(dt), *(a.data_ptr() for a in f), out.data_ptr(), *shape, dX, dY, dZ,
+                                     E.stream_ptr()))
+    return out if _is_dev(fx) else _np_out(out)
+
+
+def gradient_center(phi, dX, dY, dZ):
+    """2h central gradient, 0 on the boundary layer (solver.py:720-729)."""
+    from ._abi import check, lib
+
+    dt = _field_dtype(phi)
+    shape = tuple(int(s) for s in phi.shape)
+    p = E.to_device(phi, dt, shape)
+    g = torch.empty((3,) + shape, dtype=p.dtype, device=p.device)
+    check(lib().sw_gradient_center(E.sw_code(dt), p.data_ptr(), g[0].data_ptr(), g[1].data_ptr(), g[2].data_ptr(),
+                                   *shape, dX, dY, dZ, E.stream_ptr()))
+    if _is_dev(phi):
+        return g[0], g[1], g[2]
+    return tuple(_np_out(g[c]) for c in range(3))
+
+
+def hann1d(n):
+    """Hann window of length n as float32 (solver.py:1053-1057); a host utility."""
+    if n <= 1:
+        return np.ones((n,), dtype=np.float32)
+    return np.hanning(n).astype(np.float32)
+
+
+# ---------------------------------------------------------------------- strain
+
+
+def _validate_strain_inputs(ux, uy, uz, dX, dY, dZ):
+    """strain.py:138-159."""
+    if ux.ndim != 3:
+        raise ValueError(f"Displacement arrays must be 3-D, got ndim={ux.ndim}")
+    if tuple(ux.shape) != tuple(uy.shape) or tuple(ux.shape) != tuple(uz.shape):
+        raise ValueError(
+            "All displacement components must share the same shape: "
+            f"ux={tuple(ux.shape)}, uy={tuple(uy.shape)}, uz={tuple(uz.shape)}"
+        )
+    for name, val in (("dX", dX), ("dY", dY), ("dZ", dZ)):
+        if val <= 0:
+            raise ValueError(f"Grid spacing {name} must be positive, got {val}")
+
+
+def compute_strain_tensor(ux, uy, uz, dX, dY, dZ):
+    """Voigt strain [exx, eyy, ezz, gyz, gxz, gxy], engineering shear (strain.py:16-70)."""
+    from ._abi import check, lib
+
+    _validate_strain_inputs(ux, uy, uz, dX, dY, dZ)
+    dt = _field_dtype(ux, uy, uz)
+    shape = tuple(int(s) for s in ux.shape)
+    u = [E.to_device(a, dt, shape) for a in (ux, uy, uz)]
+    eps = torch.empty(shape + (6,), dtype=u[0].dtype, device=u[0].device)
+    check(lib().sw_strain_tensor(E.sw_code(dt), *(a.data_ptr() for a in u), eps.data_ptr(), *shape, dX, dY, dZ,
+                                 E.stream_ptr()))
+    return eps if _is_dev(ux) else _np_out(eps)
+
+
+def compute_strain_invariants(epsilon):
+    """Volumetric, deviatoric, max-shear and von Mises strain (strain.py:73-135).
+
+    von_mises is float64 for float32 input, as in the reference (NumPy 2
+    promotes through the float64 scalar np.sqrt(2/3)).
+    """
+    from ._abi import check, lib
+
+    if epsilon.shape[-1] != 6:
+        raise ValueError(f"Expected last dimension == 6 (Voigt notation), got {epsilon.shape[-1]}")
+    dt = _field_dtype(epsilon)
+    lead = tuple(int(s) for s in epsilon.shape[:-1])
+    e = E.to_device(epsilon, dt, lead + (6,))
+    n = int(np.prod(lead)) if lead else 1
+    outs = [torch.empty(lead, dtype=e.dtype, device=e.device) for _ in range(3)]
+    vm = torch.empty(lead, dtype=torch.float64, device=e.device)
+    check(lib().sw_strain_invariants(E.sw_code(dt), e.data_ptr(), n, *(o.data_ptr() for o in outs), vm.data_ptr(),
+                                     E.stream_ptr()))
+    res = {"volumetric": outs[0], "deviatoric": outs[1], "max_shear": outs[2], "von_mises": vm}
+    if _is_dev(epsilon):
+        return res
+    return {k: _np_out(v) for k, v in res.items()}

@@ -1,0 +1,169 @@
+// Shared helpers of the shearwave gfx950 kernels: error state, launch checks,
+// voxel indexing and deterministic wave64 reductions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "shearwave.h"
+
+namespace sw {
+
+void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+void clear_error();
+
+// Number of compute units of the current device (cached per device).
+int device_cu_count();
+
+}  // namespace sw
+
+#define SW_TRY_HIP(expr)                                                                             \
+    do {                                                                                             \
+        hipError_t sw_e_ = (expr);                                                                   \
+        if (sw_e_ != hipSuccess) {                                                                   \
+            sw::set_error("%s: %s (%s:%d)", #expr, hipGetErrorString(sw_e_), __FILE__, __LINE__);    \
+            return SW_ERR_HIP;                                                                       \
+        }                                                                                            \
+    } while (0)
+
+#define SW_LAUNCH_CHECK() SW_TRY_HIP(hipGetLastError())
+
+#define SW_REQUIRE(cond, code, ...)  \
+    do {                             \
+        if (!(cond)) {               \
+            sw::set_error(__VA_ARGS__); \
+            return (code);           \
+        }                            \
+    } while (0)
+
+namespace sw {
+
+constexpr int kWave = 64;
+
+// Grid of (nx, ny, nz) voxels, C order, z fastest.
+struct Grid {
+    int nx, ny, nz;
+    __host__ __device__ int64_t plane() const { return (int64_t)ny * nz; }
+    __host__ __device__ int64_t size() const { return (int64_t)nx * ny * nz; }
+    __host__ __device__ int64_t at(int x, int y, int z) const { return ((int64_t)x * ny + y) * nz + z; }
+    __host__ __device__ bool interior(int x, int y, int z) const {
+        return x > 0 && y > 0 && z > 0 && x < nx - 1 && y < ny - 1 && z < nz - 1;
+    }
+};
+
+inline int check_grid(int64_t nx, int64_t ny, int64_t nz) {
+    SW_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1, SW_ERR_ARG, "grid (%lld,%lld,%lld) must be positive", (long long)nx,
+               (long long)ny, (long long)nz);
+    SW_REQUIRE(nx < (1 << 20) && ny < (1 << 20) && nz < (1 << 20), SW_ERR_ARG, "grid axis too large");
+    return SW_OK;
+}
+
+// Row-structured launch used by the pointwise and small-stencil kernels:
+// block (64, 4); blockIdx.x walks groups of 4 (x, y) rows; lanes walk z.
+constexpr int kRowsPerBlock = 4;
+
+inline dim3 row_block() { return dim3(kWave, kRowsPerBlock, 1); }
+
+inline unsigned row_grid(const Grid& g, int64_t max_blocks) {
+    int64_t rows = (int64_t)g.nx * g.ny;
+    int64_t groups = (rows + kRowsPerBlock - 1) / kRowsPerBlock;
+    if (max_blocks > 0 && groups > max_blocks) groups = max_blocks;
+    return (unsigned)(groups < 1 ? 1 : groups);
+}
+
+// Deterministic wave64 sum (fixed butterfly order).
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+// Block sum over all threads of a block of up to 1024 threads; every thread gets the result.
+// `red` must hold 16 doubles of shared memory. Deterministic for a fixed block shape.
+__device__ inline double block_sum(double v, double* red) {
+    const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int nthreads = blockDim.x * blockDim.y * blockDim.z;
+    const int nwaves = (nthreads + kWave - 1) / kWave;
+    v = wave_sum(v);
+    __syncthreads();
+    if ((tid & (kWave - 1)) == 0) red[tid / kWave] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int w = 0; w < nwaves; ++w) s += red[w];
+    return s;
+}
+
+__device__ inline double block_max(double v, double* red) {
+    const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int nthreads = blockDim.x * blockDim.y * blockDim.z;
+    const int nwaves = (nthreads + kWave - 1) / kWave;
+    v = wave_max(v);
+    __syncthreads();
+    if ((tid & (kWave - 1)) == 0) red[tid / kWave] = v;
+    __syncthreads();
+    double s = red[0];
+    for (int w = 1; w < nwaves; ++w) s = fmax(s, red[w]);
+    return s;
+}
+
+// Visit every voxel of g once: rows (x, y) by block-row, z by lane.
+#define SW_FOR_EACH_VOXEL(g, ...)                                                       \
+    do {                                                                                \
+        const int64_t rows_ = (int64_t)(g).nx * (g).ny;                                 \
+        for (int64_t r0_ = (int64_t)blockIdx.x * kRowsPerBlock; r0_ < rows_;            \
+             r0_ += (int64_t)gridDim.x * kRowsPerBlock) {                               \
+            const int64_t row_ = r0_ + threadIdx.y;                                     \
+            if (row_ >= rows_) continue;                                                \
+            const int x = (int)(row_ / (g).ny);                                         \
+            const int y = (int)(row_ - (int64_t)x * (g).ny);                            \
+            const int64_t base_ = row_ * (g).nz;                                        \
+            for (int z = threadIdx.x; z < (g).nz; z += kWave) {                         \
+                const int64_t i = base_ + z;                                            \
+                __VA_ARGS__                                                             \
+            }                                                                           \
+        }                                                                               \
+    } while (0)
+
+inline unsigned flat_grid(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    const int64_t cap = (int64_t)device_cu_count() * 16;
+    if (b > cap) b = cap;
+    return (unsigned)(b < 1 ? 1 : b);
+}
+
+inline int check_dtype(int dtype) {
+    SW_REQUIRE(dtype == SW_F32 || dtype == SW_F64, SW_ERR_ARG, "dtype %d is not SW_F32 or SW_F64", dtype);
+    return SW_OK;
+}
+
+template <typename T>
+struct DType;
+template <>
+struct DType<float> {
+    static constexpr int code = SW_F32;
+};
+template <>
+struct DType<double> {
+    static constexpr int code = SW_F64;
+};
+
+}  // namespace sw
+
+#define SW_DISPATCH(dtype, T, ...)       \
+    do {                                 \
+        if ((dtype) == SW_F32) {         \
+            using T = float;             \
+            __VA_ARGS__;                 \
+        } else {                         \
+            using T = double;            \
+            __VA_ARGS__;                 \
+        }                                \
+    } while (0)
+

@@ -1,0 +1,285 @@
+"""Device-resident solve state and the host calls around the C ABI.
+
+:class:`Solve` owns one solve's fields in HBM (SoA, C order, z fastest) as
+torch tensors and drives the gfx950 kernels through ``_abi``. PyTorch only
+allocates device memory and provides the stream; all arithmetic on the fields
+happens in the HIP library. Without a visible GPU or without the built
+library, everything here raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _abi
+from ._abi import check, lib, ptrs
+
+_DTYPES = {
+    "float32": (torch.float32, _abi.SW_F32, np.float32),
+    "float64": (torch.float64, _abi.SW_F64, np.float64),
+}
+_KERNELS = {"auto": _abi.SW_KERNEL_AUTO, "direct": _abi.SW_KERNEL_DIRECT, "march": _abi.SW_KERNEL_MARCH}
+_FORMS = {"increment": _abi.SW_FORM_INCREMENT, "leapfrog": _abi.SW_FORM_LEAPFROG}
+
+
+def dtype_name(dtype) -> str:
+    if isinstance(dtype, str):
+        name = dtype
+    elif isinstance(dtype, torch.dtype):
+        name = str(dtype).replace("torch.", "")
+    else:
+        name = np.dtype(dtype).name
+    if name not in _DTYPES:
+        raise ValueError(f"storage dtype must be float32 or float64, got {dtype!r}")
+    return name
+
+
+def torch_dtype(name):
+    return _DTYPES[dtype_name(name)][0]
+
+
+def sw_code(name):
+    return _DTYPES[dtype_name(name)][1]
+
+
+def device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("shearwave_amd runs on a ROCm GPU; torch.cuda.is_available() is False")
+    lib()  # fail loudly if the HIP library is missing
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream_ptr():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def to_device(x, dtype, shape=None) -> torch.Tensor:
+    """Contiguous device tensor of `dtype` from a NumPy array, scalar or tensor."""
+    dev = device()
+    tdt = torch_dtype(dtype)
+    if isinstance(x, torch.Tensor):
+        t = x.to(device=dev, dtype=tdt).contiguous()
+    elif np.isscalar(x):
+        if shape is None:
+            raise ValueError("scalar needs a shape")
+        t = torch.full(tuple(shape), float(x), dtype=tdt, device=dev)
+    else:
+        a = np.ascontiguousarray(np.asarray(x), dtype=_DTYPES[dtype_name(dtype)][2])
+        t = torch.from_numpy(a).to(dev)
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"expected shape {tuple(shape)}, got {tuple(t.shape)}")
+    return t
+
+
+_WORKSPACES: dict = {}
+
+
+def project_workspace(shape) -> torch.Tensor:
+    """Cached CG workspace for a grid shape (one per device and shape)."""
+    key = (torch.cuda.current_device(), tuple(shape))
+    ws = _WORKSPACES.get(key)
+    if ws is None:
+        nbytes = int(lib().sw_project_workspace_bytes(*shape))
+        if nbytes <= 0:
+            raise ValueError(f"bad grid {shape}")
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=device())
+        _WORKSPACES[key] = ws
+    return ws
+
+
+def project_inplace(f, spacing, tol, max_iters, *, scale_out=None, inv_rho=None, scale=1.0) -> dict:
+    """Helmholtz-Hodge projection of f (3, nx, ny, nz) in place on the device.
+
+    Optionally writes scale * f_S * inv_rho into scale_out (3, nx, ny, nz).
+    Returns the CG report: iters, relres, max_div, reason.
+    """
+    shape = tuple(f.shape[1:])
+    ws = project_workspace(shape)
+    info = _abi.CgInfo()
+    so = ptrs(*scale_out) if scale_out is not None else None
+    check(
+        lib().sw_project_vector_field(
+            sw_code(f.dtype), f[0].data_ptr(), f[1].data_ptr(), f[2].data_ptr(), *shape, *spacing, float(tol),
+            int(max_iters), ws.data_ptr(), ws.numel(), so, None if inv_rho is None else inv_rho.data_ptr(),
+            float(scale), C.byref(info), stream_ptr(),
+        )
+    )
+    return {
+        "iters": int(info.iters),
+        "relres": float(info.relres),
+        "max_div": float(info.max_div),
+        "reason": _abi.CG_REASONS.get(int(info.reason), str(info.reason)),
+    }
+
+
+class Solve:
+    """One shear_fdtd solve resident in HBM.
+
+    Fields: mu/rho, eta/rho (None when inviscid), 1/rho, the state (u, b) and
+    its next-step buffers, and the loop's force dT^2 f_S / rho. The state form
+    is ``increment`` (u, d = u - u_prev) by default: in float32 storage it
+    keeps the velocity to float32 precision (the reference's (u, u_prev) form
+    loses it to cancellation); ``leapfrog`` stores (u, u_prev).
+    """
+
+    def __init__(self, shape, spacing, dt, rho, mu, eta=None, *, dtype="float32", form="increment",
+                 vel_damp=0.0, kernel="auto", tile_y=0, x_chunk=0):
+        self.shape = tuple(int(s) for s in shape)
+        self.spacing = tuple(float(h) for h in spacing)
+        self.dt = float(dt)
+        self.dtype = dtype_name(dtype)
+        self.code = sw_code(self.dtype)
+        self.form = form
+        self.n = int(np.prod(self.shape))
+        tdt = torch_dtype(self.dtype)
+        dev = device()
+        st = stream_ptr()
+        rho64 = to_device(rho, "float64", self.shape)
+        mu64 = to_device(mu, "float64", self.shape)
+        eta64 = None if eta is None else to_device(eta, "float64", self.shape)
+        self.m = torch.empty(self.shape, dtype=tdt, device=dev)
+        self.e = None if eta is None else torch.empty(self.shape, dtype=tdt, device=dev)
+        self.inv_rho = torch.empty(self.shape, dtype=tdt, device=dev)
+        check(lib().sw_prepare_coefficients(
+            self.code, rho64.data_ptr(), mu64.data_ptr(), None if eta64 is None else eta64.data_ptr(),
+            self.m.data_ptr(), None if self.e is None else self.e.data_ptr(), self.inv_rho.data_ptr(), self.n, st))
+        vec = (3,) + self.shape
+        self._bufs = [torch.zeros(vec, dtype=tdt, device=dev) for _ in range(4 if form == "increment" else 3)]
+        self._by_ptr = {}
+        for t in self._bufs:
+            for c in range(3):
+                self._by_ptr[t[c].data_ptr()] = t[c]
+        self.force = None
+        self.step_count = 0
+        s = _abi.FdtdState()
+        s.dtype = self.code
+        s.form = _FORMS[form]
+        s.nx, s.ny, s.nz = self.shape
+        s.dx, s.dy, s.dz = self.spacing
+        s.dt = self.dt
+        s.vel_damp = float(vel_damp)
+        s.mu_over_rho = self.m.data_ptr()
+        s.eta_over_rho = None if self.e is None else self.e.data_ptr()
+        for c in range(3):
+            s.force[c] = None
+            s.u[c] = self._bufs[0][c].data_ptr()
+            s.b[c] = self._bufs[1][c].data_ptr()
+            s.u_next[c] = self._bufs[2][c].data_ptr()
+            s.b_next[c] = self._bufs[3][c].data_ptr() if form == "increment" else None
+        s.kernel = _KERNELS[kernel]
+        s.tile_y = int(tile_y)
+        s.x_chunk = int(x_chunk)
+        self.state = s
+
+    # -------------------------------------------------------------- inputs
+    def set_force(self, fx, fy, fz, tol, max_iters):
+        """Project the force pattern (solver.py:189-192) and keep dT^2 f_S / rho for the loop."""
+        f = torch.empty((3,) + self.shape, dtype=torch_dtype(self.dtype), device=device())
+        for c, comp in enumerate((fx, fy, fz)):
+            if comp is None:
+                f[c].zero_()
+            else:
+                f[c].copy_(to_device(comp, self.dtype, self.shape))
+        if self.force is None:
+            self.force = torch.empty_like(f)
+            for c in range(3):
+                self.state.force[c] = self.force[c].data_ptr()
+        info = project_inplace(f, self.spacing, tol, max_iters, scale_out=self.force, inv_rho=self.inv_rho,
+                               scale=self.dt * self.dt)
+        return info, f
+
+    def set_initial(self, u0=None, v0=None):
+        """Initial state from AoS (nx, ny, nz, 3) arrays (solver.py:157-171)."""
+        aos = self.shape + (3,)
+        u0d = None if u0 is None else to_device(u0, self.dtype, aos)
+        v0d = None if v0 is None else to_device(v0, self.dtype, aos)
+        u = [self.state.u[c] for c in range(3)]
+        b = [self.state.b[c] for c in range(3)]
+        check(lib().sw_import_state(
+            self.code, self.state.form, None if u0d is None else u0d.data_ptr(),
+            None if v0d is None else v0d.data_ptr(), self.dt, (C.c_void_p * 3)(*u), (C.c_void_p * 3)(*b),
+            *self.shape, stream_ptr()))
+
+    # ----------------------------------------------------------------- loop
+    def run(self, n_steps, env=None, *, snap_stride=0, snap_out=None, inv_stride=0, inv_out=None,
+            trace_idx=None, trace_out=None):
+        """Advance n_steps steps; env (host, length n_steps) scales the force per step."""
+        n_steps = int(n_steps)
+        if n_steps <= 0:
+            return
+        env_arr = None
+        if env is not None:
+            env_np = np.ascontiguousarray(np.asarray(env, dtype=np.float64).ravel())
+            if env_np.shape[0] != n_steps:
+                raise ValueError("env length must equal n_steps")
+            env_arr = env_np.ctypes.data_as(C.POINTER(C.c_double))
+        out = _abi.FdtdOutputs()
+        out.step0 = self.step_count
+        out.snap_stride = int(snap_stride)
+        out.snap_out = None if snap_out is None else snap_out.data_ptr()
+        out.inv_stride = int(inv_stride)
+        out.inv_out = None if inv_out is None else inv_out.data_ptr()
+        if trace_idx is not None and trace_idx.numel() > 0:
+            out.trace_idx = trace_idx.data_ptr()
+            out.n_traces = trace_idx.numel()
+            out.trace_out = trace_out.data_ptr()
+            out.trace_ld = trace_out.shape[1]
+        check(lib().sw_fdtd_run(C.byref(self.state), n_steps, env_arr, C.byref(out), stream_ptr()))
+        self.step_count += n_steps
+
+    # --------------------------------------------------------------- access
+    def _vec(self, ptr_array):
+        return [self._by_ptr[ptr_array[c]] for c in range(3)]
+
+    def u_components(self):
+        return self._vec(self.state.u)
+
+    def b_components(self):
+        return self._vec(self.state.b)
+
+    def export(self, u_aos=None, v_aos=None):
+        """u and v = (u - u_prev)/dT as AoS device tensors (solver.py:241-243)."""
+        aos = self.shape + (3,)
+        tdt = torch_dtype(self.dtype)
+        if u_aos is None:
+            u_aos = torch.empty(aos, dtype=tdt, device=device())
+        if v_aos is None:
+            v_aos = torch.empty(aos, dtype=tdt, device=device())
+        check(lib().sw_export_state(
+            self.code, self.state.form, (C.c_void_p * 3)(*self.state.u), (C.c_void_p * 3)(*self.state.b),
+            1.0 / self.dt, u_aos.data_ptr(), v_aos.data_ptr(), self.n, stream_ptr()))
+        return u_aos, v_aos
+
+    def u_aos(self, out=None):
+        aos = self.shape + (3,)
+        if out is None:
+            out = torch.empty(aos, dtype=torch_dtype(self.dtype), device=device())
+        u = self.state.u
+        check(lib().sw_soa_to_aos3(self.code, u[0], u[1], u[2], out.data_ptr(), self.n, stream_ptr()))
+        return out
+
+    def reproject(self, tol, max_iters):
+        """Project u and u_prev to divergence-free (solver.py:236-239)."""
+        u = self.u_components()
+        b = self.b_components()
+        uvec = torch.stack(u)  # copies: the projection works in place
+        if self.form == "increment":
+            prev = torch.empty_like(uvec)
+            # u_prev = u - d
+            check(lib().sw_combine3(self.code, ptrs(*u), ptrs(*b), 1.0, -1.0, ptrs(*prev), self.n, stream_ptr()))
+        else:
+            prev = torch.stack(b)
+        infos = [project_inplace(uvec, self.spacing, tol, max_iters), project_inplace(prev, self.spacing, tol,
+                                                                                        max_iters)]
+        for c in range(3):
+            u[c].copy_(uvec[c])
+        if self.form == "increment":
+            # d = u - u_prev
+            check(lib().sw_combine3(self.code, ptrs(*u), ptrs(*prev), 1.0, -1.0, ptrs(*b), self.n, stream_ptr()))
+        else:
+            for c in range(3):
+                b[c].copy_(prev[c])
+        return infos

@@ -1,0 +1,97 @@
+"""Seeded synthetic inputs of the benchmark workloads (NumPy, host side).
+
+* ``plain_64``: the reference's CPU-runnable case: 64^3, h = 0.5 mm,
+  homogeneous rho = 1000, mu = 2.5 kPa, eta = 1 Pa s, a Gaussian axial push
+  (sigma = 2 mm) at the centre for the first 100 us, nT = 500, CG tol 1e-10,
+  snapshot stride 10.
+* ``dropin_256``: the single drop-in solve: 256^3, h = 0.25 mm,
+  rho = 1000 (1 + U(-0.01, 0.01)), mu = 3 kPa with 8 seeded spherical
+  inclusions (radius U(2, 6) mm, mu U(6, 20) kPa, later ones overwrite),
+  eta U(0.5, 3) Pa s per voxel, Gaussian push (sigma = 1.5 mm) along z at the
+  centre for t < 100 us, nT = 2000, CG tol 1e-8, invariants every 20 steps.
+
+The time step is min(0.3 h / (c_max sqrt 3), 0.5 rho_min h^2 / (6 eta_max)):
+the CFL-0.3 step alone violates the explicit viscous limit rho h^2 / (6 eta)
+and diverges; the factor-0.5 viscous cap is the rule of the reference's
+example driver (examples/sparse_transcranial_shear_3d.py:1258-1265).
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+def stable_dt(h, rho_min, mu_max, eta_max, cfl=0.3):
+    dt_cfl = cfl * h / (math.sqrt(mu_max / rho_min) * math.sqrt(3.0))
+    dt_visc = 0.5 * rho_min * h * h / (6.0 * eta_max) if eta_max > 0 else math.inf
+    return min(dt_cfl, dt_visc), dt_cfl
+
+
+def gaussian_push(shape, h, sigma, amplitude=300.0, centre=None):
+    c = [(n - 1) / 2 * h for n in shape] if centre is None else centre
+    ax = [np.arange(n) * h - cc for n, cc in zip(shape, c)]
+    r2 = ax[0][:, None, None] ** 2 + ax[1][None, :, None] ** 2 + ax[2][None, None, :] ** 2
+    return amplitude * np.exp(-r2 / (2 * sigma * sigma))
+
+
+def inclusion_map(shape, h, rng, n_incl, background, r_range=(2e-3, 6e-3), mu_range=(6e3, 20e3)):
+    mu = np.full(shape, float(background))
+    ext = [n * h for n in shape]
+    ax = [np.arange(n) * h for n in shape]
+    for _ in range(n_incl):
+        r = rng.uniform(*r_range)
+        c = [rng.uniform(r, e - r) for e in ext]
+        val = rng.uniform(*mu_range)
+        # only the bounding box of the sphere is touched
+        lo = [max(0, int((cc - r) / h)) for cc in c]
+        hi = [min(n, int((cc + r) / h) + 2) for cc, n in zip(c, shape)]
+        sub = (
+            (ax[0][lo[0]:hi[0], None, None] - c[0]) ** 2
+            + (ax[1][None, lo[1]:hi[1], None] - c[1]) ** 2
+            + (ax[2][None, None, lo[2]:hi[2]] - c[2]) ** 2
+        ) < r * r
+        mu[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]][sub] = val
+    return mu
+
+
+def plain_64(n=64):
+    h = 0.5e-3
+    rho, mu, eta = 1000.0, 2500.0, 1.0
+    dt, dt_cfl = stable_dt(h, rho, mu, eta)
+    n_steps = 500
+    bz = gaussian_push((n, n, n), h, 2e-3)
+    env = (np.arange(n_steps) * dt < 100e-6).astype(np.float64)
+    return dict(shape=(n, n, n), h=h, rho=rho, mu=mu, eta=eta, dt=dt, dt_cfl=dt_cfl, n_steps=n_steps, bx=None,
+                by=None, bz=bz, env=env, tol=1e-10, max_iters=5000, snapshot_stride=10)
+
+
+def dropin_256(n=256, seed=2, n_steps=2000):
+    rng = np.random.default_rng(seed)
+    shape = (n, n, n)
+    h = 0.25e-3
+    rho = 1000.0 * (1.0 + rng.uniform(-0.01, 0.01, shape))
+    mu = inclusion_map(shape, h, rng, 8, 3000.0)
+    eta = rng.uniform(0.5, 3.0, shape)
+    dt, dt_cfl = stable_dt(h, float(rho.min()), float(mu.max()), float(eta.max()))
+    bz = gaussian_push(shape, h, 1.5e-3)
+    env = (np.arange(n_steps) * dt < 100e-6).astype(np.float64)
+    return dict(shape=shape, h=h, rho=rho, mu=mu, eta=eta, dt=dt, dt_cfl=dt_cfl, n_steps=n_steps, bx=None, by=None,
+                bz=bz, env=env, tol=1e-8, max_iters=20000, strain_stride=20)
+
+
+# algorithmic HBM bytes per voxel of each kernel (read + write, once each)
+def step_bytes(itemsize, form="increment", viscous=True, forced=False):
+    """Time step: u, b read (stencil, counted once), u' (and d') written, mu/rho (and eta/rho) read."""
+    vec = 3 * itemsize
+    b = 2 * vec + vec  # u, b read; u' written
+    if form == "increment":
+        b += vec  # d' written
+    b += itemsize * (2 if viscous else 1)
+    if forced:
+        b += vec
+    return b
+
+
+CG_BYTES_PER_ITER = 8 * 8  # dir: r, p_old read, p written; step: phi, r, p read, phi, r written (float64)
