@@ -17,6 +17,8 @@
 // are deterministic) and applies the stopping rules. The host only polls the
 // stop flag once per chunk of iterations.
 #include <cmath>
+#include <cstdlib>
+#include <type_traits>
 
 #include "sw_common.h"
 
@@ -230,6 +232,259 @@ __global__ void __launch_bounds__(256) cg_step_kernel(double* __restrict__ phi, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// x-marching CG kernels. A (BY x 64) tile of (y, z) columns marches along x;
+// p at planes x-2 .. x+3 of the column sits in a register queue, and plane x
+// (with a two-voxel halo in y and z) is staged in LDS, so r, p_old, p, phi are
+// each read from HBM about once per iteration. The queue's roles rotate over
+// an unrolled loop of six planes, so no prefetched register is moved.
+
+constexpr int kCgBY = 8;   // tile rows in y (block of 512 threads)
+constexpr int kCgLZ = 68;  // 64 + 2 + 2
+
+struct CgTile {
+    int y0, z0, x0, x1, y, z;
+    bool inyz;
+    int64_t yz;
+    bool halo_thread, halo_in;
+    int ly, lz;
+    int64_t hyz;
+};
+
+__device__ inline CgTile cg_tile(const Grid& g, int xchunk, int nty, int ntz) {
+    CgTile t;
+    const int tz = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tz;
+    const unsigned nb = gridDim.x;
+    unsigned b = blockIdx.x;
+    if ((nb & 7u) == 0u) b = (b & 7u) * (nb >> 3) + (b >> 3);  // XCD-aware: neighbours share an L2
+    const int tyi = (int)(b % (unsigned)nty);
+    const unsigned rest = b / (unsigned)nty;
+    const int tzi = (int)(rest % (unsigned)ntz);
+    const int txi = (int)(rest / (unsigned)ntz);
+    t.y0 = tyi * kCgBY;
+    t.z0 = tzi * 64;
+    t.x0 = txi * xchunk;
+    t.x1 = min(t.x0 + xchunk, g.nx);
+    t.y = t.y0 + ty;
+    t.z = t.z0 + tz;
+    t.inyz = t.y < g.ny && t.z < g.nz;
+    t.yz = t.inyz ? (int64_t)t.y * g.nz + t.z : 0;
+    // halo: rows y0-2, y0-1, y0+BY, y0+BY+1 (4 x 64 points), columns z0-2, z0-1, z0+64, z0+65 (4 x BY)
+    int hy = -1, hz = -1;
+    t.ly = t.lz = 0;
+    if (tid < 256) {
+        const int rr = tid >> 6;
+        hy = rr < 2 ? t.y0 - 2 + rr : t.y0 + kCgBY + rr - 2;
+        t.ly = rr < 2 ? rr : kCgBY + rr;
+        hz = t.z0 + (tid & 63);
+        t.lz = (tid & 63) + 2;
+    } else if (tid < 256 + 4 * kCgBY) {
+        const int k = tid - 256, cc = k / kCgBY, yy = k % kCgBY;
+        hy = t.y0 + yy;
+        t.ly = yy + 2;
+        hz = cc < 2 ? t.z0 - 2 + cc : t.z0 + 64 + cc - 2;
+        t.lz = cc < 2 ? cc : 64 + cc;
+    }
+    t.halo_thread = tid < 256 + 4 * kCgBY;
+    t.halo_in = t.halo_thread && hy >= 0 && hy < g.ny && hz >= 0 && hz < g.nz;
+    t.hyz = t.halo_in ? (int64_t)hy * g.nz + hz : 0;
+    return t;
+}
+
+// A p at an interior voxel: x terms from the queue (a = x-2, c = x, e = x+2),
+// y and z terms from the LDS plane; rows next to the wall drop G on the
+// boundary layer (solver.py:1024-1026 with gradient_center zero there).
+__device__ inline double cg_apply(const Grid& g, int x, int y, int z, double a, double c, double e,
+                                  const double (*L)[kCgLZ], int r, int q, double qx, double qy, double qz) {
+    const double tx = ((x + 1 < g.nx - 1) ? (e - c) : 0.0) - ((x - 1 > 0) ? (c - a) : 0.0);
+    const double ty = ((y + 1 < g.ny - 1) ? (L[r + 2][q] - c) : 0.0) - ((y - 1 > 0) ? (c - L[r - 2][q]) : 0.0);
+    const double tz = ((z + 1 < g.nz - 1) ? (L[r][q + 2] - c) : 0.0) - ((z - 1 > 0) ? (c - L[r][q - 2]) : 0.0);
+    return -(tx * qx + ty * qy + tz * qz);
+}
+
+// p = r + beta p_old (one fma, so every recomputation is bit-identical), A p, p.Ap.
+__global__ void __launch_bounds__(64 * kCgBY) cg_dir_march_kernel(const double* __restrict__ r,
+                                                                  const double* __restrict__ p_old,
+                                                                  double* __restrict__ p, Grid g, double qx,
+                                                                  double qy, double qz, int xchunk, int nty, int ntz,
+                                                                  double* part_sum, CgState* st) {
+    if (st->stop) return;
+    __shared__ double lds[2][kCgBY + 4][kCgLZ];
+    __shared__ double red[16];
+    const double beta = st->beta;
+    const CgTile t = cg_tile(g, xchunk, nty, ntz);
+    const int64_t P = g.plane();
+    const int r0 = threadIdx.y + 2, q0 = threadIdx.x + 2;
+    // voxel 0 is on the boundary layer, where r and p_old are zero
+    auto pn = [&](int64_t o, bool ok) {
+        const int64_t k = ok ? o : 0;
+        return fma(beta, p_old[k], r[k]);
+    };
+    auto center = [&](int xx) { return pn((int64_t)xx * P + t.yz, t.inyz && xx >= 0 && xx < g.nx); };
+    auto halo = [&](int xx) {
+        return t.halo_thread ? pn((int64_t)xx * P + t.hyz, t.halo_in && xx >= 0 && xx < g.nx) : 0.0;
+    };
+    double s = 0.0;
+    auto plane = [&](auto buf, double a, double b, double c, double d, double e, double& f, double hc, double& hn,
+                     int x) {
+        constexpr int B = decltype(buf)::value;
+        (void)b;
+        (void)d;
+        lds[B][r0][q0] = c;
+        if (t.halo_thread) lds[B][t.ly][t.lz] = hc;
+        f = center(x + 3);
+        hn = halo(x + 1);
+        __syncthreads();
+        if (!t.inyz) return;
+        const bool in = g.interior(x, t.y, t.z);
+        if (in) s += c * cg_apply(g, x, t.y, t.z, a, c, e, lds[B], r0, q0, qx, qy, qz);
+        p[(int64_t)x * P + t.yz] = in ? c : 0.0;
+    };
+    using B0 = std::integral_constant<int, 0>;
+    using B1 = std::integral_constant<int, 1>;
+    const int x0 = t.x0, x1 = t.x1;
+    double v0 = center(x0 - 2), v1 = center(x0 - 1), v2 = center(x0), v3 = center(x0 + 1), v4 = center(x0 + 2), v5;
+    double h0 = halo(x0), h1;
+    for (int x = x0; x < x1; x += 6) {
+        plane(B0{}, v0, v1, v2, v3, v4, v5, h0, h1, x);
+        if (x + 1 >= x1) break;
+        plane(B1{}, v1, v2, v3, v4, v5, v0, h1, h0, x + 1);
+        if (x + 2 >= x1) break;
+        plane(B0{}, v2, v3, v4, v5, v0, v1, h0, h1, x + 2);
+        if (x + 3 >= x1) break;
+        plane(B1{}, v3, v4, v5, v0, v1, v2, h1, h0, x + 3);
+        if (x + 4 >= x1) break;
+        plane(B0{}, v4, v5, v0, v1, v2, v3, h0, h1, x + 4);
+        if (x + 5 >= x1) break;
+        plane(B1{}, v5, v0, v1, v2, v3, v4, h1, h0, x + 5);
+    }
+    s = block_sum(s, red);
+    if (publish_partial(part_sum, s, nullptr, 0.0, st)) {
+        double denom, unused;
+        reduce_partials(part_sum, nullptr, red, &denom, &unused);
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+            st->counter = 0u;
+            if (!isfinite(denom) || denom <= 0.0) {  // solver.py:1036-1037
+                st->stop = 1;
+                st->reason = SW_CG_DENOM;
+            } else {
+                st->alpha = st->rs_old / denom;
+            }
+        }
+    }
+}
+
+// phi += alpha p, r -= alpha A p (A p recomputed from p), r.r and the stopping rules.
+__global__ void __launch_bounds__(64 * kCgBY) cg_step_march_kernel(double* __restrict__ phi, double* __restrict__ r,
+                                                                   const double* __restrict__ p, Grid g, double qx,
+                                                                   double qy, double qz, int xchunk, int nty, int ntz,
+                                                                   double* part_sum, CgState* st) {
+    if (st->stop) return;
+    __shared__ double lds[2][kCgBY + 4][kCgLZ];
+    __shared__ double red[16];
+    const double alpha = st->alpha;
+    const CgTile t = cg_tile(g, xchunk, nty, ntz);
+    const int64_t P = g.plane();
+    const int r0 = threadIdx.y + 2, q0 = threadIdx.x + 2;
+    auto ld = [&](const double* f, int64_t o, bool ok) { return f[ok ? o : 0]; };
+    auto center = [&](int xx) { return ld(p, (int64_t)xx * P + t.yz, t.inyz && xx >= 0 && xx < g.nx); };
+    auto halo = [&](int xx) {
+        return t.halo_thread ? ld(p, (int64_t)xx * P + t.hyz, t.halo_in && xx >= 0 && xx < g.nx) : 0.0;
+    };
+    auto ldr = [&](int xx) { return ld(r, (int64_t)xx * P + t.yz, t.inyz && xx >= 0 && xx < g.nx); };
+    auto ldphi = [&](int xx) { return ld(phi, (int64_t)xx * P + t.yz, t.inyz && xx >= 0 && xx < g.nx); };
+    double s = 0.0;
+    auto plane = [&](auto buf, double a, double b, double c, double d, double e, double& f, double hc, double& hn,
+                     double rc, double& rn, double fc, double& fn, int x) {
+        constexpr int B = decltype(buf)::value;
+        (void)b;
+        (void)d;
+        lds[B][r0][q0] = c;
+        if (t.halo_thread) lds[B][t.ly][t.lz] = hc;
+        f = center(x + 3);
+        hn = halo(x + 1);
+        rn = ldr(x + 1);
+        fn = ldphi(x + 1);
+        __syncthreads();
+        if (!t.inyz || !g.interior(x, t.y, t.z)) return;
+        const double ap = cg_apply(g, x, t.y, t.z, a, c, e, lds[B], r0, q0, qx, qy, qz);
+        const int64_t o = (int64_t)x * P + t.yz;
+        phi[o] = fc + alpha * c;
+        const double rv = rc - alpha * ap;
+        r[o] = rv;
+        s += rv * rv;
+    };
+    using B0 = std::integral_constant<int, 0>;
+    using B1 = std::integral_constant<int, 1>;
+    const int x0 = t.x0, x1 = t.x1;
+    double v0 = center(x0 - 2), v1 = center(x0 - 1), v2 = center(x0), v3 = center(x0 + 1), v4 = center(x0 + 2), v5;
+    double h0 = halo(x0), h1;
+    double ra = ldr(x0), rb, fa = ldphi(x0), fb;
+    for (int x = x0; x < x1; x += 6) {
+        plane(B0{}, v0, v1, v2, v3, v4, v5, h0, h1, ra, rb, fa, fb, x);
+        if (x + 1 >= x1) break;
+        plane(B1{}, v1, v2, v3, v4, v5, v0, h1, h0, rb, ra, fb, fa, x + 1);
+        if (x + 2 >= x1) break;
+        plane(B0{}, v2, v3, v4, v5, v0, v1, h0, h1, ra, rb, fa, fb, x + 2);
+        if (x + 3 >= x1) break;
+        plane(B1{}, v3, v4, v5, v0, v1, v2, h1, h0, rb, ra, fb, fa, x + 3);
+        if (x + 4 >= x1) break;
+        plane(B0{}, v4, v5, v0, v1, v2, v3, h0, h1, ra, rb, fa, fb, x + 4);
+        if (x + 5 >= x1) break;
+        plane(B1{}, v5, v0, v1, v2, v3, v4, h1, h0, rb, ra, fb, fa, x + 5);
+    }
+    s = block_sum(s, red);
+    if (publish_partial(part_sum, s, nullptr, 0.0, st)) {
+        double rs_new, unused;
+        reduce_partials(part_sum, nullptr, red, &rs_new, &unused);
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+            const double eps = 2.220446049250313e-16;
+            st->counter = 0u;
+            st->iters += 1;
+            if (!isfinite(rs_new)) {  // solver.py:1042-1043
+                st->stop = 1;
+                st->reason = SW_CG_NONFINITE;
+            } else {
+                const double rel = sqrt(rs_new / st->rs0);
+                st->relres = rel;
+                if (rel < st->tol) {  // solver.py:1044-1045
+                    st->stop = 1;
+                    st->reason = SW_CG_CONVERGED;
+                } else {
+                    st->beta = rs_new / fmax(st->rs_old, eps);
+                    st->rs_old = rs_new;
+                }
+            }
+        }
+    }
+}
+
+struct CgShape {
+    int xchunk, nty, ntz, ntx;
+    unsigned blocks() const { return (unsigned)nty * ntz * ntx; }
+};
+
+CgShape cg_shape(const Grid& g) {
+    CgShape c;
+    c.nty = (g.ny + kCgBY - 1) / kCgBY;
+    c.ntz = (g.nz + 63) / 64;
+    const int64_t cols = (int64_t)c.nty * c.ntz;
+    // about four 512-thread blocks per CU in one wave, chunks of at least 12 planes
+    const int64_t target = (int64_t)device_cu_count() * 4;
+    int64_t ntx = (target + cols - 1) / cols;
+    if (ntx < 1) ntx = 1;
+    int64_t xc = (g.nx + ntx - 1) / ntx;
+    if (xc < 12) xc = 12;
+    c.xchunk = (int)xc;
+    c.ntx = (g.nx + c.xchunk - 1) / c.xchunk;
+    return c;
+}
+
+// Upper bound on the march grid's block count (chunks never shorter than 12 planes).
+int64_t cg_max_blocks(int64_t nx, int64_t ny, int64_t nz) {
+    return ((ny + kCgBY - 1) / kCgBY) * ((nz + 63) / 64) * ((nx + 11) / 12);
+}
+
 // f_S = mask (mask f - grad_c phi); optionally out = scale * f_S * inv_rho.
 template <typename T>
 __global__ void __launch_bounds__(256) grad_sub_kernel(T* __restrict__ fx, T* __restrict__ fy, T* __restrict__ fz,
@@ -265,9 +520,19 @@ struct Workspace {
 
 int64_t align256(int64_t b) { return (b + 255) & ~(int64_t)255; }
 
-int64_t workspace_bytes(int64_t n) { return 4 * align256(n * 8) + 2 * align256(kMaxRedBlocks * 8) + 256; }
+int64_t partial_capacity(int64_t nx, int64_t ny, int64_t nz) {
+    const int64_t m = cg_max_blocks(nx, ny, nz);
+    return m > kMaxRedBlocks ? m : kMaxRedBlocks;
+}
 
-Workspace carve(void* base, int64_t n) {
+int64_t workspace_bytes(int64_t nx, int64_t ny, int64_t nz) {
+    const int64_t n = nx * ny * nz;
+    return 4 * align256(n * 8) + 2 * align256(partial_capacity(nx, ny, nz) * 8) + 256;
+}
+
+Workspace carve(void* base, int64_t nx, int64_t ny, int64_t nz) {
+    const int64_t n = nx * ny * nz;
+    const int64_t pc = partial_capacity(nx, ny, nz);
     char* p = (char*)base;
     Workspace w;
     w.phi = (double*)p;
@@ -279,9 +544,9 @@ Workspace carve(void* base, int64_t n) {
     w.p1 = (double*)p;
     p += align256(n * 8);
     w.part_sum = (double*)p;
-    p += align256(kMaxRedBlocks * 8);
+    p += align256(pc * 8);
     w.part_max = (double*)p;
-    p += align256(kMaxRedBlocks * 8);
+    p += align256(pc * 8);
     w.st = (CgState*)p;
     return w;
 }
@@ -301,7 +566,7 @@ extern "C" {
 
 int64_t sw_project_workspace_bytes(int64_t nx, int64_t ny, int64_t nz) {
     if (nx < 1 || ny < 1 || nz < 1) return -1;
-    return workspace_bytes(nx * ny * nz);
+    return workspace_bytes(nx, ny, nz);
 }
 
 int sw_project_vector_field(int dtype, void* fx, void* fy, void* fz, int64_t nx, int64_t ny, int64_t nz, double dx,
@@ -314,17 +579,21 @@ int sw_project_vector_field(int dtype, void* fx, void* fy, void* fz, int64_t nx,
     SW_REQUIRE(fx && fy && fz, SW_ERR_ARG, "null force component");
     SW_REQUIRE(max_iters >= 0, SW_ERR_ARG, "max_iters must be >= 0");
     SW_REQUIRE(dx > 0 && dy > 0 && dz > 0, SW_ERR_ARG, "grid spacing must be positive");
-    const int64_t n = nx * ny * nz;
-    SW_REQUIRE(workspace && workspace_bytes_given >= workspace_bytes(n), SW_ERR_WORKSPACE,
+    SW_REQUIRE(workspace && workspace_bytes_given >= workspace_bytes(nx, ny, nz), SW_ERR_WORKSPACE,
                "workspace of %lld bytes is smaller than the %lld needed", (long long)workspace_bytes_given,
-               (long long)workspace_bytes(n));
+               (long long)workspace_bytes(nx, ny, nz));
     SW_REQUIRE(((uintptr_t)workspace & 255) == 0, SW_ERR_ARG, "workspace must be 256-byte aligned");
     if (scale_out) SW_REQUIRE(scale_out[0] && scale_out[1] && scale_out[2], SW_ERR_ARG, "null scale_out component");
 
     hipStream_t s = (hipStream_t)stream;
     Grid g{(int)nx, (int)ny, (int)nz};
-    Workspace w = carve(workspace, n);
+    Workspace w = carve(workspace, nx, ny, nz);
     const unsigned nb = red_grid(g);
+    const CgShape cs = cg_shape(g);
+    // the march kernels need at least one 64-wide z tile and 8 rows; tiny grids use the direct kernels
+    const char* env_k = getenv("SW_CG_KERNEL");
+    bool march = nz >= 32 && ny >= 8;
+    if (env_k && env_k[0] == 'd') march = false;
     const dim3 blk = row_block();
     const double i2hx = 1.0 / (2.0 * dx), i2hy = 1.0 / (2.0 * dy), i2hz = 1.0 / (2.0 * dz);
     const double qx = 1.0 / (4.0 * dx * dx), qy = 1.0 / (4.0 * dy * dy), qz = 1.0 / (4.0 * dz * dz);
@@ -346,10 +615,17 @@ int sw_project_vector_field(int dtype, void* fx, void* fy, void* fz, int64_t nx,
     while (!hs.stop && launched < max_iters) {
         const int64_t k = (max_iters - launched < chunk) ? (max_iters - launched) : chunk;
         for (int64_t j = 0; j < k; ++j) {
-            hipLaunchKernelGGL(cg_dir_kernel, dim3(nb), blk, 0, s, w.r, p_old, p_new, g, qx, qy, qz, w.part_sum,
-                               w.st);
-            hipLaunchKernelGGL(cg_step_kernel, dim3(nb), blk, 0, s, w.phi, w.r, p_new, g, qx, qy, qz, w.part_sum,
-                               w.st);
+            if (march) {
+                hipLaunchKernelGGL(cg_dir_march_kernel, dim3(cs.blocks()), dim3(64, kCgBY), 0, s, w.r, p_old, p_new,
+                                   g, qx, qy, qz, cs.xchunk, cs.nty, cs.ntz, w.part_sum, w.st);
+                hipLaunchKernelGGL(cg_step_march_kernel, dim3(cs.blocks()), dim3(64, kCgBY), 0, s, w.phi, w.r, p_new,
+                                   g, qx, qy, qz, cs.xchunk, cs.nty, cs.ntz, w.part_sum, w.st);
+            } else {
+                hipLaunchKernelGGL(cg_dir_kernel, dim3(nb), blk, 0, s, w.r, p_old, p_new, g, qx, qy, qz, w.part_sum,
+                                   w.st);
+                hipLaunchKernelGGL(cg_step_kernel, dim3(nb), blk, 0, s, w.phi, w.r, p_new, g, qx, qy, qz, w.part_sum,
+                                   w.st);
+            }
             double* t = p_old;
             p_old = p_new;
             p_new = t;

@@ -335,3 +335,19 @@ def test_push_stays_solenoidal_on_gpu(sw):
         opts = {"vel_damp": 0.02 if damped else 0.0, "eta": 0.5 if damped else 0.0, "env_t": env}
         sw.shear_fdtd(bx, None, None, rho, mu, d, d, d, dt, n_steps, opts, callback=cb)
         assert ratios and max(ratios) < 1e-4, ratios
+
+
+@pytest.mark.parametrize("variant", ["march", "direct"])
+def test_projection_cg_kernels_agree(sw, variant, monkeypatch):
+    """Both CG kernel families (x-marching LDS tiles, and one voxel per thread) at a pinned count."""
+    if variant == "direct":
+        monkeypatch.setenv("SW_CG_KERNEL", "direct")
+    rng = np.random.default_rng(9)
+    shape = (30, 21, 70)  # partial y and z tiles
+    f = [rng.standard_normal(shape) * taper(shape) for _ in range(3)]
+    info = {}
+    out = sw.project_vector_field(*f, *H, 1e-9, 5000, dtype="float64", info=info)
+    assert info["reason"] == "converged"
+    want = O.project_vector_field(*f, *H, 0.0, info["iters"], dtype=np.float64)
+    for a, b in zip(out, want):
+        assert rel(a, b) < 1e-10
