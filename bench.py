@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--kernel", default="auto")
     ap.add_argument("--tile-y", type=int, default=0)
     ap.add_argument("--x-chunk", type=int, default=0)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batching comparison")
+    ap.add_argument("--batch-solves", type=int, default=48)
     return ap.parse_args()
 
 
@@ -149,6 +151,73 @@ def cpu_baseline(w, n_steps):
     }
 
 
+def bench_batch(n_solves=48, check=8):
+    """Config 3: 48 uneven solves, batched vs the plain per-solve drop-in (host arrays in and out)."""
+    import torch
+
+    import shearwave_amd as sw
+    from shearwave_amd import workloads as W
+    from shearwave_amd.autotune import default_tuner
+    from shearwave_amd.batch import Batch
+
+    specs = W.batch_48(n_solves=n_solves)
+    vox_steps = sum(sp.voxels * sp.n_steps for sp in specs)
+    tuner = default_tuner()
+    for sp in specs:
+        tuner.xchunk(sp.shape)  # tune every grid class before timing
+
+    def plain(sp, dtype="float32"):
+        opts = {"env_t": sp.env, "eta": sp.eta, "vel_damp": sp.vel_damp, "poisson_tol": sp.tol,
+                "poisson_max_iters": sp.max_iters, "sw_dtype": dtype}
+        return sw.shear_fdtd(None, None, sp.force[2], sp.rho, sp.mu, *sp.h, sp.dt, sp.n_steps, opts)
+
+    plain(specs[0])  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain_out = [plain(sp) for sp in specs]
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b = Batch(specs, tuner=tuner)
+    t_setup = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    info = b.project()
+    t_proj = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    b.run()
+    t_run = time.perf_counter() - t1
+    batch_out = [b.result(k) for k in range(len(specs))]
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+
+    errs = []
+    for k in range(min(check, len(specs))):
+        u64, v64 = plain(specs[k], "float64")
+        ub, vb = batch_out[k]
+        errs.append(max(float(np.linalg.norm(ub - u64) / np.linalg.norm(u64)),
+                        float(np.linalg.norm(vb - v64) / np.linalg.norm(v64))))
+    agree = max(float(np.linalg.norm(pb[0] - pp[0]) / np.linalg.norm(pp[0])) for pb, pp in zip(batch_out, plain_out))
+    del b
+    torch.cuda.empty_cache()
+    return {
+        "workload": f"{len(specs)} solves, grids U{{64..192}}^2 x U{{64..256}}, nT U{{500..2000}}, float32, seed 3",
+        "voxel_steps": vox_steps,
+        "batched_solves_per_s": round(len(specs) / t_batch, 3),
+        "plain_dropin_solves_per_s": round(len(specs) / t_plain, 3),
+        "speedup": round(t_plain / t_batch, 3),
+        "batched_s": round(t_batch, 3),
+        "plain_s": round(t_plain, 3),
+        "batched_phases_s": {"upload_setup": round(t_setup, 3), "projection": round(t_proj, 3), "loop": round(t_run, 3)},
+        "batched_loop_gvox_steps_per_s": round(vox_steps / t_run / 1e9, 2),
+        "cg_iters_max": max(i["iters"] for i in info),
+        "max_rel_err_vs_fp64": max(errs) if errs else None,
+        "checked_solves": len(errs),
+        "max_rel_diff_batched_vs_plain": agree,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -242,6 +311,10 @@ def main():
         rec["fp64"] = {"value": round(vox * w["n_steps"] / t64 / 1e9, 3), "ms_per_solve": round(t64 * 1e3, 2),
                        "step_kernel_GBs": round(ab64 / kt64 / 1e9, 1), "frac": round(ab64 / kt64 / 1e9 / HBM_PEAK_GBS, 4)}
         del b64
+    if not args.no_batch:
+        del bench
+        torch.cuda.empty_cache()
+        rec["batch"] = bench_batch(args.batch_solves)
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(w, args.cpu_steps)
     else:

@@ -175,6 +175,91 @@ int sw_combine3(int dtype, const void* const* a, const void* const* b, double al
 int sw_gather_traces(int dtype, const void* f, const int64_t* idx, int64_t count, void* out, int64_t out_stride,
                      void* stream);
 
+/* ---------------------------------------------------------------- batching */
+
+/* One solve of a batch (float32 storage, increment form), as device memory.
+ * A batch advances all its solves with ONE persistent launch per time step:
+ * work items are (solve, z tile, y tile, x chunk) quadruples of int32, and a
+ * device-side global step counter gives each solve its local step
+ * local = gstep - start (active while 0 <= local < n_steps; its state is
+ * read from parity local&1 and written to the other parity). No reference
+ * counterpart: the reference runs one solve per call (solver.py:27). */
+typedef struct sw_batch_solve {
+    int32_t nx, ny, nz;
+    int32_t xchunk;                  /* x planes per work item */
+    float cx, cy, cz;                /* dt^2 / h^2 */
+    float vx, vy, vz;                /* dt / h^2 */
+    float damp;                      /* 1 - vel_damp */
+    float pad0;
+    int64_t start;                   /* global step of this solve's first step */
+    int64_t n_steps;
+    const float* mu_over_rho;
+    const float* eta_over_rho;       /* NULL when inviscid */
+    const float* force[3];           /* dt^2 f_S / rho, or NULL */
+    const double* env;               /* device, n_steps values, or NULL */
+    float* u[2][3];                  /* u by parity */
+    float* d[2][3];                  /* d = u - u_prev by parity */
+} sw_batch_solve;
+
+/* Advance the batch by n_steps global steps: per step one persistent launch
+ * over all `items` (tile_y = 4 or 8 rows; blocks = 0 picks one resident wave)
+ * and one increment of *gstep (device int64). */
+int sw_batch_step(const sw_batch_solve* solves, const int32_t* items, int64_t n_items, int64_t* gstep, int32_t tile_y,
+                  int32_t blocks, int64_t n_steps, void* stream);
+
+/* The same n_steps captured once into a hipGraph (no memset nodes), to be
+ * replayed with sw_graph_launch; the graph keeps the pointers it was built with. */
+typedef struct sw_graph sw_graph;
+int sw_batch_graph_create(const sw_batch_solve* solves, const int32_t* items, int64_t n_items, int64_t* gstep,
+                          int32_t tile_y, int32_t blocks, int64_t n_steps, sw_graph** graph);
+int sw_graph_launch(sw_graph* graph, void* stream);
+int sw_graph_destroy(sw_graph* graph);
+
+/* One solve's projection in a batched projection (project_vector_field,
+ * solver.py:644-683, for many solves at once), as device memory. */
+typedef struct sw_batch_cg {
+    int32_t nx, ny, nz;
+    int32_t xchunk;                  /* x planes per CG work item */
+    double dx, dy, dz, tol;
+    int64_t max_iters;
+    float* f[3];                     /* force in, f_S out (in place) */
+    float* out[3];                   /* scale * f_S * inv_rho, or NULL */
+    const float* inv_rho;            /* or NULL for 1 */
+    double scale;
+    double *phi, *r, *p0, *p1;       /* float64 CG vectors, nx*ny*nz each */
+    double* part;                    /* 2 * n_items float64 partial slots */
+    int32_t item0, n_items;          /* this solve's contiguous range in the CG item list */
+} sw_batch_cg;
+
+/* Bytes of device CG state per solve. */
+int64_t sw_cg_state_bytes(void);
+
+/* Project every solve of the batch; CG items are (solve, z tile, y tile,
+ * x chunk) quadruples over 8-row tiles, grouped by solve. `states` is device
+ * memory of n_solves * sw_cg_state_bytes() bytes. Runs until every solve has
+ * stopped; info (host, n_solves entries) receives each solve's report. */
+int sw_batch_project(const sw_batch_cg* solves, int32_t n_solves, const int32_t* items, int64_t n_items,
+                     void* states, sw_cg_info* info, void* stream);
+
+/* -------------------------------------------------------- differentiation */
+
+/* One step of the adjoint of sw_fdtd_run's update (increment form), for
+ * gradients with respect to mu/rho and eta/rho. No reference counterpart: the
+ * reference has no derivative code. Given the adjoint (ub_next, db_next) of
+ * the state after the step and the forward state (u, d) before it, writes the
+ * adjoint (ub, db) of the state before the step and accumulates
+ * grad_m += dJ/d(mu/rho), grad_e += dJ/d(eta/rho) (float64 arrays; grad_e and
+ * eta_over_rho NULL when inviscid). */
+int sw_adjoint_step(int dtype, const void* const* u, const void* const* d, const void* const* ub_next,
+                    const void* const* db_next, void* const* ub, void* const* db, const void* mu_over_rho,
+                    const void* eta_over_rho, double* grad_m, double* grad_e, int64_t nx, int64_t ny, int64_t nz,
+                    double dx, double dy, double dz, double dt, double vel_damp, void* stream);
+
+/* f[idx[k]] += grad[k*stride] for k < count (unique device int64 indices): the
+ * misfit gradient of sensor traces entering the adjoint of u_z. */
+int sw_inject_traces(int dtype, void* f, const int64_t* idx, int64_t count, const void* grad, int64_t stride,
+                     void* stream);
+
 /* ------------------------------------------------------------------ strain */
 
 /* compute_strain_tensor (strain.py:16-70): Voigt [exx,eyy,ezz,gyz,gxz,gxy],

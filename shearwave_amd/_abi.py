@@ -82,6 +82,56 @@ class FdtdOutputs(C.Structure):
     ]
 
 
+class BatchSolve(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32),
+        ("ny", C.c_int32),
+        ("nz", C.c_int32),
+        ("xchunk", C.c_int32),
+        ("cx", C.c_float),
+        ("cy", C.c_float),
+        ("cz", C.c_float),
+        ("vx", C.c_float),
+        ("vy", C.c_float),
+        ("vz", C.c_float),
+        ("damp", C.c_float),
+        ("pad0", C.c_float),
+        ("start", _i64),
+        ("n_steps", _i64),
+        ("mu_over_rho", _p),
+        ("eta_over_rho", _p),
+        ("force", _p * 3),
+        ("env", _p),
+        ("u", (_p * 3) * 2),
+        ("d", (_p * 3) * 2),
+    ]
+
+
+class BatchCg(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32),
+        ("ny", C.c_int32),
+        ("nz", C.c_int32),
+        ("xchunk", C.c_int32),
+        ("dx", _d),
+        ("dy", _d),
+        ("dz", _d),
+        ("tol", _d),
+        ("max_iters", _i64),
+        ("f", _p * 3),
+        ("out", _p * 3),
+        ("inv_rho", _p),
+        ("scale", _d),
+        ("phi", _p),
+        ("r", _p),
+        ("p0", _p),
+        ("p1", _p),
+        ("part", _p),
+        ("item0", C.c_int32),
+        ("n_items", C.c_int32),
+    ]
+
+
 # name -> (restype, argtypes); one entry per declaration of include/shearwave.h
 SIGNATURES = {
     "sw_last_error": (C.c_char_p, []),
@@ -107,6 +157,15 @@ SIGNATURES = {
     "sw_import_state": (_i32, [_i32, _i32, _p, _p, _d, C.POINTER(_p), C.POINTER(_p), _i64, _i64, _i64, _p]),
     "sw_combine3": (_i32, [_i32, C.POINTER(_p), C.POINTER(_p), _d, _d, C.POINTER(_p), _i64, _p]),
     "sw_gather_traces": (_i32, [_i32, _p, _p, _i64, _p, _i64, _p]),
+    "sw_batch_step": (_i32, [_p, _p, _i64, _p, _i32, _i32, _i64, _p]),
+    "sw_batch_graph_create": (_i32, [_p, _p, _i64, _p, _i32, _i32, _i64, C.POINTER(_p)]),
+    "sw_graph_launch": (_i32, [_p, _p]),
+    "sw_graph_destroy": (_i32, [_p]),
+    "sw_cg_state_bytes": (_i64, []),
+    "sw_batch_project": (_i32, [_p, _i32, _p, _i64, _p, C.POINTER(CgInfo), _p]),
+    "sw_adjoint_step": (_i32, [_i32, C.POINTER(_p), C.POINTER(_p), C.POINTER(_p), C.POINTER(_p), C.POINTER(_p),
+                               C.POINTER(_p), _p, _p, _p, _p, _i64, _i64, _i64, _d, _d, _d, _d, _d, _p]),
+    "sw_inject_traces": (_i32, [_i32, _p, _p, _i64, _p, _i64, _p]),
 }
 
 _lock = threading.Lock()

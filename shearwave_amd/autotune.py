@@ -1,0 +1,131 @@
+"""Autotuner for the batched time-step launch shape.
+
+The batched step kernel marches x-chunks of a (8 x 64) tile of (y, z)
+columns; the chunk depth trades halo re-reads (two planes per chunk) against
+the number of work items available to balance the persistent grid. The tuner
+times the step kernel on a synthetic solve of each grid class with every
+candidate chunk depth, keeps the fastest, and caches the choice per device
+architecture in a JSON file (``$SW_AUTOTUNE_CACHE`` or
+``~/.cache/shearwave_amd/autotune.json``).
+"""
+
+from __future__ import annotations
+
+import json
+import math
+import os
+import threading
+
+CANDIDATES = (16, 24, 32, 48, 64, 96, 128)
+
+
+def grid_class(shape):
+    """Bucket a grid by its x length (the march) and its y/z tile counts."""
+    nx, ny, nz = (int(s) for s in shape)
+    return (1 << max(0, math.ceil(math.log2(max(nx, 1)))), math.ceil(ny / 32), math.ceil(nz / 64))
+
+
+def _cache_path():
+    p = os.environ.get("SW_AUTOTUNE_CACHE")
+    if p:
+        return p
+    return os.path.join(os.path.expanduser("~"), ".cache", "shearwave_amd", "autotune.json")
+
+
+class Tuner:
+    """Per-class x-chunk choice, measured on first use and cached."""
+
+    def __init__(self, path=None, timer=None, arch=None):
+        self.path = path or _cache_path()
+        self._timer = timer
+        self._arch = arch
+        self._lock = threading.Lock()
+        self._table = {}
+        try:
+            with open(self.path) as f:
+                self._table = json.load(f)
+        except (OSError, ValueError):
+            self._table = {}
+
+    @property
+    def arch(self):
+        if self._arch is None:
+            import ctypes as C
+
+            from ._abi import lib
+
+            buf = C.create_string_buffer(64)
+            cu = C.c_int()
+            lib().sw_device_info(C.byref(cu), buf, 64)
+            self._arch = f"{buf.value.decode()}:{cu.value}"
+        return self._arch
+
+    def key(self, shape):
+        return f"{self.arch}|{'x'.join(str(v) for v in grid_class(shape))}"
+
+    def xchunk(self, shape):
+        k = self.key(shape)
+        with self._lock:
+            if k not in self._table:
+                self._table[k] = self._tune(shape)
+                self._save()
+            return min(int(self._table[k]["xchunk"]), int(shape[0]))
+
+    def _save(self):
+        try:
+            os.makedirs(os.path.dirname(self.path), exist_ok=True)
+            tmp = self.path + ".tmp"
+            with open(tmp, "w") as f:
+                json.dump(self._table, f, indent=1, sort_keys=True)
+            os.replace(tmp, self.path)
+        except OSError:
+            pass  # a read-only home keeps the table in memory
+
+    def _tune(self, shape):
+        timer = self._timer or _time_step_kernel
+        cls = grid_class(shape)
+        rep = (min(cls[0], 512), cls[1] * 32, cls[2] * 64)
+        times = {}
+        for xc in CANDIDATES:
+            if xc > rep[0]:
+                continue
+            times[xc] = float(timer(rep, xc))
+        best = min(times, key=times.get)
+        return {"xchunk": best, "shape": list(rep), "us": {str(k): round(v * 1e6, 2) for k, v in times.items()}}
+
+
+def _time_step_kernel(shape, xchunk, n_steps=20):
+    """Average time of one batched step on one synthetic solve of `shape` (HIP events)."""
+    import numpy as np
+    import torch
+
+    from .batch import Batch, SolveSpec
+
+    rng = np.random.default_rng(0)
+    h = (0.5e-3,) * 3
+    spec = SolveSpec(shape=shape, h=h, dt=1e-6, rho=1000.0, mu=3000.0, eta=rng.uniform(0.5, 3, shape),
+                     force=(None, None, None), env=np.zeros(n_steps + 5), n_steps=n_steps + 5, xchunk=xchunk)
+    b = Batch([spec])
+    desc = b._desc_array(b.solves)
+    from .batch import step_items
+
+    items = step_items(b.solves)
+    items_t = torch.from_numpy(items.reshape(-1).copy()).to(desc.device)
+    b._advance(desc, items_t, items.shape[0], 5)
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record()
+    b._advance(desc, items_t, items.shape[0], n_steps)
+    stop.record()
+    stop.synchronize()
+    return start.elapsed_time(stop) / 1e3 / n_steps
+
+
+_DEFAULT = None
+
+
+def default_tuner():
+    global _DEFAULT
+    if _DEFAULT is None:
+        _DEFAULT = Tuner()
+    return _DEFAULT

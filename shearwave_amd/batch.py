@@ -1,0 +1,270 @@
+"""Batched solves: many shear_fdtd problems of uneven sizes in one persistent launch per step.
+
+:class:`Batch` keeps every solve's fields resident in HBM (float32 storage,
+increment form), projects all their forces with one batched CG
+(``sw_batch_project``), and advances all active solves with one persistent
+kernel launch per time step (``sw_batch_step``), optionally replayed from a
+captured hipGraph (``sw_batch_graph_create``). Work items are (solve, z tile,
+y tile, x chunk) quadruples; when solves finish, the item list is rebuilt so
+finished solves cost nothing. A solve's arithmetic is the single-solve tile
+code with per-solve reductions in a fixed order, so its result does not
+depend on the other solves in the batch.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _abi
+from . import engine as E
+from ._abi import check, lib
+
+STEP_BY = 8  # tile rows of the batched time step (512-thread blocks)
+CG_BY = 8  # tile rows of the batched CG (fixed by the kernels)
+
+
+@dataclass
+class SolveSpec:
+    """One solve: grid, spacing, time step, materials, force pattern and envelope."""
+
+    shape: tuple
+    h: tuple
+    dt: float
+    rho: object
+    mu: object
+    eta: object = None
+    force: tuple = (None, None, None)
+    env: np.ndarray = None
+    n_steps: int = 0
+    vel_damp: float = 0.0
+    tol: float = 1e-8
+    max_iters: int = 20000
+    xchunk: int = 0  # 0: from the autotuner / default
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def voxels(self):
+        return int(np.prod(self.shape))
+
+
+class _Solve:
+    """Device buffers of one solve in a batch."""
+
+    def __init__(self, spec: SolveSpec):
+        dev = E.device()
+        self.spec = spec
+        shape = tuple(int(s) for s in spec.shape)
+        self.shape = shape
+        st = E.stream_ptr()
+        rho64 = E.to_device(spec.rho, "float64", shape)
+        mu64 = E.to_device(spec.mu, "float64", shape)
+        eta64 = None if spec.eta is None else E.to_device(spec.eta, "float64", shape)
+        self.m = torch.empty(shape, dtype=torch.float32, device=dev)
+        self.e = None if eta64 is None else torch.empty(shape, dtype=torch.float32, device=dev)
+        self.inv_rho = torch.empty(shape, dtype=torch.float32, device=dev)
+        check(lib().sw_prepare_coefficients(
+            _abi.SW_F32, rho64.data_ptr(), mu64.data_ptr(), None if eta64 is None else eta64.data_ptr(),
+            self.m.data_ptr(), None if self.e is None else self.e.data_ptr(), self.inv_rho.data_ptr(),
+            int(np.prod(shape)), st))
+        del rho64, mu64, eta64
+        self.u = torch.zeros((2, 3) + shape, dtype=torch.float32, device=dev)
+        self.d = torch.zeros((2, 3) + shape, dtype=torch.float32, device=dev)
+        self.f = torch.empty((3,) + shape, dtype=torch.float32, device=dev)
+        for c, comp in enumerate(spec.force):
+            if comp is None:
+                self.f[c].zero_()
+            else:
+                self.f[c].copy_(E.to_device(comp, "float32", shape))
+        self.force = torch.empty_like(self.f)
+        env = np.zeros(spec.n_steps) if spec.env is None else np.asarray(spec.env, dtype=np.float64)
+        if env.shape[0] != spec.n_steps:
+            raise ValueError("env length must equal n_steps")
+        self.env = torch.from_numpy(np.ascontiguousarray(env)).to(dev)
+        self.start = 0
+
+    def footprint(self):
+        n = int(np.prod(self.shape))
+        return n * 4 * (3 + 12 + 6) + self.spec.n_steps * 8
+
+
+def solve_bytes(spec: SolveSpec, with_cg=True) -> int:
+    """HBM footprint of one solve: fields, state, force, and (during projection) the CG vectors."""
+    n = spec.voxels
+    b = n * 4 * (3 + 12 + 6) + spec.n_steps * 8
+    if with_cg:
+        b += n * 8 * 4
+    return b
+
+
+def step_items(solves, by=STEP_BY):
+    """Work items (solve, z tile, y tile, x chunk): z fastest, each solve's items contiguous."""
+    items = []
+    for s_idx, s in enumerate(solves):
+        nx, ny, nz = s.shape
+        xc = s.xchunk
+        for tx in range(math.ceil(nx / xc)):
+            for ty in range(math.ceil(ny / by)):
+                for tz in range(math.ceil(nz / 64)):
+                    items.append((s_idx, tz, ty, tx))
+    return np.asarray(items, dtype=np.int32).reshape(-1, 4)
+
+
+def default_xchunk(shape):
+    """x planes per batched work item when the autotuner has no entry for the grid class."""
+    return int(min(shape[0], 64))
+
+
+class Batch:
+    """A set of solves advanced together on one GPU (float32 storage, increment form)."""
+
+    def __init__(self, specs, *, tile_y=STEP_BY, tuner=None, graph_steps=0):
+        self.specs = list(specs)
+        self.tile_y = int(tile_y)
+        self.graph_steps = int(graph_steps)
+        self.solves = []
+        for sp in self.specs:
+            s = _Solve(sp)
+            s.xchunk = int(sp.xchunk) if sp.xchunk else (tuner.xchunk(sp.shape) if tuner else default_xchunk(sp.shape))
+            self.solves.append(s)
+        self.gstep = torch.zeros(1, dtype=torch.int64, device=E.device())
+        self.cg_info = None
+
+    # --------------------------------------------------------- descriptors
+    def _desc_array(self, solves):
+        arr = (_abi.BatchSolve * len(solves))()
+        for k, s in enumerate(solves):
+            sp = s.spec
+            d = arr[k]
+            d.nx, d.ny, d.nz = s.shape
+            d.xchunk = s.xchunk
+            hx, hy, hz = (float(v) for v in sp.h)
+            dt = float(sp.dt)
+            d.cx, d.cy, d.cz = dt * dt / (hx * hx), dt * dt / (hy * hy), dt * dt / (hz * hz)
+            d.vx, d.vy, d.vz = dt / (hx * hx), dt / (hy * hy), dt / (hz * hz)
+            d.damp = 1.0 - float(sp.vel_damp)
+            d.start = s.start
+            d.n_steps = sp.n_steps
+            d.mu_over_rho = s.m.data_ptr()
+            d.eta_over_rho = None if s.e is None else s.e.data_ptr()
+            for c in range(3):
+                d.force[c] = s.force[c].data_ptr()
+                for par in range(2):
+                    d.u[par][c] = s.u[par, c].data_ptr()
+                    d.d[par][c] = s.d[par, c].data_ptr()
+            d.env = s.env.data_ptr()
+        raw = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8).to(E.device())
+        return raw
+
+    # ---------------------------------------------------------- projection
+    def project(self, solves=None):
+        """Project every solve's force with one batched CG; keeps dT^2 f_S / rho for the loop."""
+        solves = self.solves if solves is None else solves
+        dev = E.device()
+        cgs = (_abi.BatchCg * len(solves))()
+        items = []
+        keep = []
+        for k, s in enumerate(solves):
+            sp = s.spec
+            nx, ny, nz = s.shape
+            xc = min(nx, 48)
+            n_items0 = len(items)
+            for tx in range(math.ceil(nx / xc)):
+                for ty in range(math.ceil(ny / CG_BY)):
+                    for tz in range(math.ceil(nz / 64)):
+                        items.append((k, tz, ty, tx))
+            vec = torch.empty((4,) + s.shape, dtype=torch.float64, device=dev)
+            part = torch.empty(2 * (len(items) - n_items0), dtype=torch.float64, device=dev)
+            keep += [vec, part]
+            c = cgs[k]
+            c.nx, c.ny, c.nz = s.shape
+            c.xchunk = xc
+            c.dx, c.dy, c.dz = (float(v) for v in sp.h)
+            c.tol = float(sp.tol)
+            c.max_iters = int(sp.max_iters)
+            for j in range(3):
+                c.f[j] = s.f[j].data_ptr()
+                c.out[j] = s.force[j].data_ptr()
+            c.inv_rho = s.inv_rho.data_ptr()
+            c.scale = float(sp.dt) ** 2
+            c.phi, c.r, c.p0, c.p1 = (vec[j].data_ptr() for j in range(4))
+            c.part = part.data_ptr()
+            c.item0 = n_items0
+            c.n_items = len(items) - n_items0
+        items_t = torch.from_numpy(np.asarray(items, dtype=np.int32).reshape(-1)).to(dev)
+        desc = torch.frombuffer(bytearray(bytes(cgs)), dtype=torch.uint8).to(dev)
+        states = torch.empty(len(solves) * int(lib().sw_cg_state_bytes()), dtype=torch.uint8, device=dev)
+        info = (_abi.CgInfo * len(solves))()
+        check(lib().sw_batch_project(desc.data_ptr(), len(solves), items_t.data_ptr(), len(items), states.data_ptr(),
+                                     info, E.stream_ptr()))
+        torch.cuda.current_stream().synchronize()
+        del keep
+        self.cg_info = [{"iters": int(i.iters), "relres": float(i.relres), "max_div": float(i.max_div),
+                         "reason": _abi.CG_REASONS.get(int(i.reason), str(i.reason))} for i in info]
+        return self.cg_info
+
+    # ---------------------------------------------------------------- loop
+    def _advance(self, desc, items_t, n_items, n_steps):
+        if n_steps <= 0:
+            return
+        if self.graph_steps and n_steps >= self.graph_steps:
+            g = C.c_void_p()
+            check(lib().sw_batch_graph_create(desc.data_ptr(), items_t.data_ptr(), n_items, self.gstep.data_ptr(),
+                                              self.tile_y, 0, self.graph_steps, C.byref(g)))
+            try:
+                reps = n_steps // self.graph_steps
+                for _ in range(reps):
+                    check(lib().sw_graph_launch(g, E.stream_ptr()))
+                rest = n_steps - reps * self.graph_steps
+            finally:
+                torch.cuda.current_stream().synchronize()
+                lib().sw_graph_destroy(g)
+        else:
+            rest = n_steps
+        if rest:
+            check(lib().sw_batch_step(desc.data_ptr(), items_t.data_ptr(), n_items, self.gstep.data_ptr(), self.tile_y,
+                                      0, rest, E.stream_ptr()))
+
+    def run(self):
+        """All solves start together; the item list shrinks as solves finish."""
+        for s in self.solves:
+            s.start = 0
+        self.gstep.zero_()
+        desc = self._desc_array(self.solves)
+        ends = sorted(set(s.spec.n_steps for s in self.solves))
+        done = 0
+        for end in ends:
+            active = [k for k, s in enumerate(self.solves) if s.spec.n_steps > done]
+            items = step_items([self.solves[k] for k in active], self.tile_y)
+            # map local solve index back to the full descriptor array
+            items[:, 0] = np.asarray(active, dtype=np.int32)[items[:, 0]]
+            items_t = torch.from_numpy(items.reshape(-1).copy()).to(E.device())
+            self._advance(desc, items_t, items.shape[0], end - done)
+            done = end
+        torch.cuda.current_stream().synchronize()
+
+    # ------------------------------------------------------------- results
+    def state(self, k):
+        """(u, d) of solve k after its n_steps steps, as (3, nx, ny, nz) device tensors."""
+        s = self.solves[k]
+        par = s.spec.n_steps & 1
+        return s.u[par], s.d[par]
+
+    def result(self, k, to_host=True):
+        """(u, v) of solve k in the reference's AoS (nx, ny, nz, 3) layout."""
+        s = self.solves[k]
+        u, d = self.state(k)
+        n = int(np.prod(s.shape))
+        aos = s.shape + (3,)
+        ua = torch.empty(aos, dtype=torch.float32, device=u.device)
+        va = torch.empty(aos, dtype=torch.float32, device=u.device)
+        check(lib().sw_export_state(_abi.SW_F32, _abi.SW_FORM_INCREMENT, _abi.ptrs(u[0], u[1], u[2]),
+                                    _abi.ptrs(d[0], d[1], d[2]), 1.0 / float(s.spec.dt), ua.data_ptr(), va.data_ptr(),
+                                    n, E.stream_ptr()))
+        if to_host:
+            return ua.cpu().numpy(), va.cpu().numpy()
+        return ua, va

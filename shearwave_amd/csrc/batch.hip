@@ -1,0 +1,355 @@
+// Batched solves on gfx950: many shear_fdtd solves of uneven grid sizes and
+// material maps advance in ONE persistent launch per time step, and their
+// projections run as one batched CG.
+//
+// Work items are (solve, z tile, y tile, x chunk) quadruples. A persistent
+// grid (one wave of resident blocks) walks the item list; each XCD takes a
+// contiguous range of items, so tiles that share halos share an L2. Each item
+// runs exactly the tile code of the single-solve kernels (march_step.h,
+// march_cg.h), and every reduction of a solve goes to that solve's own slots
+// in a fixed order, so a solve's result does not depend on which other solves
+// share the batch.
+#include <algorithm>
+#include <vector>
+
+#include "march_cg.h"
+#include "march_step.h"
+#include "sw_common.h"
+
+namespace sw {
+namespace {
+
+// Contiguous item range of this block's XCD; k runs lo + j, lo + j + stride, ...
+struct ItemWalk {
+    int64_t lo, hi, first, stride;
+};
+
+__device__ inline ItemWalk item_walk(int64_t n_items) {
+    ItemWalk w;
+    const unsigned nb = gridDim.x;
+    if ((nb & 7u) == 0u) {
+        const int64_t xcd = blockIdx.x & 7u, per = nb >> 3, j = blockIdx.x >> 3;
+        w.lo = xcd * n_items / 8;
+        w.hi = (xcd + 1) * n_items / 8;
+        w.first = w.lo + j;
+        w.stride = per;
+    } else {
+        w.lo = 0;
+        w.hi = n_items;
+        w.first = blockIdx.x;
+        w.stride = nb;
+    }
+    return w;
+}
+
+template <int BY>
+__global__ void __launch_bounds__(64 * BY) batch_step_kernel(const sw_batch_solve* __restrict__ solves,
+                                                             const int4* __restrict__ items, int64_t n_items,
+                                                             const long long* __restrict__ gstep) {
+    __shared__ float lds[2][6][BY + 2][66];
+    const long long gs = *gstep;
+    const ItemWalk w = item_walk(n_items);
+    for (int64_t k = w.first; k < w.hi; k += w.stride) {
+        const int4 it = items[k];
+        const sw_batch_solve& S = solves[it.x];
+        const long long local = gs - S.start;
+        if (local < 0 || local >= S.n_steps) continue;  // uniform in the block
+        const int par = (int)(local & 1);
+        StepArgs<float> a;
+        for (int c = 0; c < 3; ++c) {
+            a.A[c] = S.u[par][c];
+            a.B[c] = S.d[par][c];
+            a.Ao[c] = S.u[par ^ 1][c];
+            a.Bo[c] = S.d[par ^ 1][c];
+            a.f[c] = S.force[c];
+        }
+        a.m = S.mu_over_rho;
+        a.e = S.eta_over_rho;
+        a.g = Grid{S.nx, S.ny, S.nz};
+        a.cx = S.cx, a.cy = S.cy, a.cz = S.cz;
+        a.vx = S.vx, a.vy = S.vy, a.vz = S.vz;
+        a.damp = S.damp;
+        a.fs = (S.env && S.force[0]) ? (float)S.env[local] : 0.f;
+        const int x0 = it.w * S.xchunk;
+        const int x1 = min(x0 + S.xchunk, S.nx);
+        const int y0 = it.z * BY, z0 = it.y * 64;
+        const bool visc = a.e != nullptr, force = a.fs != 0.f;
+        if (visc) {
+            if (force)
+                march_tile<float, true, true, true, BY>(a, x0, x1, y0, z0, lds);
+            else
+                march_tile<float, true, true, false, BY>(a, x0, x1, y0, z0, lds);
+        } else {
+            if (force)
+                march_tile<float, true, false, true, BY>(a, x0, x1, y0, z0, lds);
+            else
+                march_tile<float, true, false, false, BY>(a, x0, x1, y0, z0, lds);
+        }
+        __syncthreads();  // the next item reuses the LDS tile
+    }
+}
+
+__global__ void advance_kernel(long long* gstep) { *gstep += 1; }
+
+// ------------------------------------------------------------ batched CG
+
+__global__ void batch_cg_reset_kernel(const sw_batch_cg* solves, int n, CgState* states) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) cg_reset(states + s, solves[s].tol, (long long)solves[s].max_iters);
+}
+
+__device__ inline CgTile batch_cg_tile(const sw_batch_cg& S, int4 it) {
+    const Grid g{S.nx, S.ny, S.nz};
+    const int x0 = it.w * S.xchunk;
+    return cg_tile(g, x0, min(x0 + S.xchunk, S.nx), it.z * kCgBY, it.y * 64);
+}
+
+__global__ void __launch_bounds__(64 * kCgBY) batch_cg_init_kernel(const sw_batch_cg* __restrict__ solves, CgState* states,
+                                                                   const int4* __restrict__ items, int64_t n_items) {
+    __shared__ double red[16];
+    const ItemWalk w = item_walk(n_items);
+    for (int64_t k = w.first; k < w.hi; k += w.stride) {
+        const int4 it = items[k];
+        const sw_batch_cg& S = solves[it.x];
+        CgState* st = states + it.x;
+        const Grid g{S.nx, S.ny, S.nz};
+        const CgTile t = batch_cg_tile(S, it);
+        double s = 0.0, mx = 0.0;
+        cg_init_tile<float>(S.f[0], S.f[1], S.f[2], S.phi, S.r, S.p0, g, 0.5 / S.dx, 0.5 / S.dy, 0.5 / S.dz, t, s,
+                            mx);
+        s = block_sum(s, red);
+        mx = block_max(mx, red);
+        if (publish_slot(S.part, S.part + S.n_items, (int)(k - S.item0), s, mx, &st->counter, S.n_items)) {
+            double tot, m;
+            reduce_slots(S.part, S.part + S.n_items, S.n_items, red, &tot, &m);
+            if (threadIdx.x == 0 && threadIdx.y == 0) {
+                const int stop = st->stop, reason = st->reason;  // max_iters == 0 keeps its stop
+                cg_after_init(st, tot, m);
+                if (stop && !st->stop) {
+                    st->stop = stop;
+                    st->reason = reason;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// p ping-pongs between p0 and p1 by the solve's iteration parity.
+__global__ void __launch_bounds__(64 * kCgBY) batch_cg_dir_kernel(const sw_batch_cg* __restrict__ solves, CgState* states,
+                                                                  const int4* __restrict__ items, int64_t n_items) {
+    __shared__ CgLds lds;
+    __shared__ double red[16];
+    const ItemWalk w = item_walk(n_items);
+    for (int64_t k = w.first; k < w.hi; k += w.stride) {
+        const int4 it = items[k];
+        const sw_batch_cg& S = solves[it.x];
+        CgState* st = states + it.x;
+        if (st->stop) continue;
+        const Grid g{S.nx, S.ny, S.nz};
+        const CgTile t = batch_cg_tile(S, it);
+        const bool odd = st->iters & 1;
+        const double* p_old = odd ? S.p1 : S.p0;
+        double* p_new = odd ? S.p0 : S.p1;
+        double s = cg_dir_tile(S.r, p_old, p_new, g, st->beta, 0.25 / (S.dx * S.dx), 0.25 / (S.dy * S.dy),
+                               0.25 / (S.dz * S.dz), t, lds);
+        s = block_sum(s, red);
+        if (publish_slot(S.part, nullptr, (int)(k - S.item0), s, 0.0, &st->counter, S.n_items)) {
+            double denom, unused;
+            reduce_slots(S.part, nullptr, S.n_items, red, &denom, &unused);
+            if (threadIdx.x == 0 && threadIdx.y == 0) cg_after_dir(st, denom);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(64 * kCgBY) batch_cg_step_kernel(const sw_batch_cg* __restrict__ solves, CgState* states,
+                                                                   const int4* __restrict__ items, int64_t n_items) {
+    __shared__ CgLds lds;
+    __shared__ double red[16];
+    const ItemWalk w = item_walk(n_items);
+    for (int64_t k = w.first; k < w.hi; k += w.stride) {
+        const int4 it = items[k];
+        const sw_batch_cg& S = solves[it.x];
+        CgState* st = states + it.x;
+        if (st->stop) continue;
+        const Grid g{S.nx, S.ny, S.nz};
+        const CgTile t = batch_cg_tile(S, it);
+        const bool odd = st->iters & 1;
+        const double* p = odd ? S.p0 : S.p1;
+        double s = cg_step_tile(S.phi, S.r, p, g, st->alpha, 0.25 / (S.dx * S.dx), 0.25 / (S.dy * S.dy),
+                                0.25 / (S.dz * S.dz), t, lds);
+        s = block_sum(s, red);
+        if (publish_slot(S.part, nullptr, (int)(k - S.item0), s, 0.0, &st->counter, S.n_items)) {
+            double rs_new, unused;
+            reduce_slots(S.part, nullptr, S.n_items, red, &rs_new, &unused);
+            if (threadIdx.x == 0 && threadIdx.y == 0) cg_after_step(st, rs_new);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(64 * kCgBY) batch_cg_grad_sub_kernel(const sw_batch_cg* __restrict__ solves,
+                                                                       const int4* __restrict__ items,
+                                                                       int64_t n_items) {
+    const ItemWalk w = item_walk(n_items);
+    for (int64_t k = w.first; k < w.hi; k += w.stride) {
+        const int4 it = items[k];
+        const sw_batch_cg& S = solves[it.x];
+        const Grid g{S.nx, S.ny, S.nz};
+        const CgTile t = batch_cg_tile(S, it);
+        cg_grad_sub_tile<float>(S.f[0], S.f[1], S.f[2], S.phi, g, 0.5 / S.dx, 0.5 / S.dy, 0.5 / S.dz, S.out[0],
+                                S.out[1], S.out[2], S.inv_rho, S.scale, t);
+    }
+}
+
+template <int BY>
+unsigned persistent_blocks(int requested) {
+    if (requested > 0) return (unsigned)requested;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_step_kernel<BY>, 64 * BY, 0) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    return (unsigned)(per_cu * device_cu_count());  // a multiple of 8 on an 8-XCD device
+}
+
+int launch_batch_steps(const sw_batch_solve* solves, const int32_t* items, int64_t n_items, int64_t* gstep,
+                       int tile_y, int blocks, int64_t n_steps, hipStream_t s) {
+    if (tile_y == 4) {
+        const unsigned nb = persistent_blocks<4>(blocks);
+        for (int64_t k = 0; k < n_steps; ++k) {
+            hipLaunchKernelGGL(batch_step_kernel<4>, dim3(nb), dim3(64, 4), 0, s, solves, (const int4*)items, n_items,
+                               (const long long*)gstep);
+            hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(1), 0, s, (long long*)gstep);
+        }
+    } else {
+        const unsigned nb = persistent_blocks<8>(blocks);
+        for (int64_t k = 0; k < n_steps; ++k) {
+            hipLaunchKernelGGL(batch_step_kernel<8>, dim3(nb), dim3(64, 8), 0, s, solves, (const int4*)items, n_items,
+                               (const long long*)gstep);
+            hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(1), 0, s, (long long*)gstep);
+        }
+    }
+    SW_LAUNCH_CHECK();
+    return SW_OK;
+}
+
+}  // namespace
+}  // namespace sw
+
+struct sw_graph {
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+using namespace sw;
+
+extern "C" {
+
+int sw_batch_step(const sw_batch_solve* solves, const int32_t* items, int64_t n_items, int64_t* gstep, int32_t tile_y,
+                  int32_t blocks, int64_t n_steps, void* stream) {
+    SW_REQUIRE(solves && gstep && (items || n_items == 0), SW_ERR_ARG, "null pointer");
+    SW_REQUIRE(tile_y == 4 || tile_y == 8, SW_ERR_ARG, "tile_y %d must be 4 or 8", tile_y);
+    SW_REQUIRE(n_items >= 0 && n_steps >= 0 && blocks >= 0, SW_ERR_ARG, "negative size");
+    return launch_batch_steps(solves, items, n_items, gstep, tile_y, blocks, n_steps, (hipStream_t)stream);
+}
+
+int sw_batch_graph_create(const sw_batch_solve* solves, const int32_t* items, int64_t n_items, int64_t* gstep,
+                          int32_t tile_y, int32_t blocks, int64_t n_steps, sw_graph** graph) {
+    SW_REQUIRE(graph, SW_ERR_ARG, "null graph handle");
+    SW_REQUIRE(solves && gstep && (items || n_items == 0), SW_ERR_ARG, "null pointer");
+    SW_REQUIRE(tile_y == 4 || tile_y == 8, SW_ERR_ARG, "tile_y %d must be 4 or 8", tile_y);
+    SW_REQUIRE(n_items >= 0 && n_steps >= 1 && blocks >= 0, SW_ERR_ARG, "bad size");
+    *graph = nullptr;
+    hipStream_t cap;
+    SW_TRY_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        hipStreamDestroy(cap);
+        SW_TRY_HIP(e);
+    }
+    const int rc = launch_batch_steps(solves, items, n_items, gstep, tile_y, blocks, n_steps, cap);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(cap, &g);
+    hipStreamDestroy(cap);
+    if (rc != SW_OK) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+    }
+    SW_TRY_HIP(e);
+    hipGraphExec_t ex = nullptr;
+    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hipGraphDestroy(g);
+        SW_TRY_HIP(e);
+    }
+    *graph = new sw_graph{g, ex};
+    return SW_OK;
+}
+
+int sw_graph_launch(sw_graph* graph, void* stream) {
+    SW_REQUIRE(graph, SW_ERR_ARG, "null graph");
+    SW_TRY_HIP(hipGraphLaunch(graph->exec, (hipStream_t)stream));
+    return SW_OK;
+}
+
+int sw_graph_destroy(sw_graph* graph) {
+    if (!graph) return SW_OK;
+    hipGraphExecDestroy(graph->exec);
+    hipGraphDestroy(graph->graph);
+    delete graph;
+    return SW_OK;
+}
+
+int64_t sw_cg_state_bytes(void) { return (int64_t)sizeof(CgState); }
+
+int sw_batch_project(const sw_batch_cg* solves, int32_t n_solves, const int32_t* items, int64_t n_items,
+                     void* states, sw_cg_info* info, void* stream) {
+    SW_REQUIRE(solves && items && info && states, SW_ERR_ARG, "null pointer");
+    SW_REQUIRE(n_solves >= 1 && n_items >= 1, SW_ERR_ARG, "empty batch");
+    hipStream_t s = (hipStream_t)stream;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_cg_dir_kernel, 64 * kCgBY, 0) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    const unsigned nb = (unsigned)(per_cu * device_cu_count());
+    const dim3 blk(64, kCgBY);
+    const int4* it = (const int4*)items;
+    CgState* st = (CgState*)states;
+    std::vector<CgState> h(n_solves);
+    auto poll = [&]() -> int {
+        SW_TRY_HIP(hipMemcpyAsync(h.data(), st, sizeof(CgState) * n_solves, hipMemcpyDeviceToHost, s));
+        SW_TRY_HIP(hipStreamSynchronize(s));
+        return SW_OK;
+    };
+    hipLaunchKernelGGL(batch_cg_reset_kernel, dim3((n_solves + 63) / 64), dim3(64), 0, s, solves, n_solves, st);
+    hipLaunchKernelGGL(batch_cg_init_kernel, dim3(nb), blk, 0, s, solves, st, it, n_items);
+    SW_LAUNCH_CHECK();
+    int rc = poll();
+    int64_t chunk = 8;
+    while (rc == SW_OK) {
+        bool running = false;
+        for (int k = 0; k < n_solves; ++k) running |= (h[k].stop == 0);
+        if (!running) break;
+        for (int64_t j = 0; j < chunk; ++j) {
+            hipLaunchKernelGGL(batch_cg_dir_kernel, dim3(nb), blk, 0, s, solves, st, it, n_items);
+            hipLaunchKernelGGL(batch_cg_step_kernel, dim3(nb), blk, 0, s, solves, st, it, n_items);
+        }
+        SW_LAUNCH_CHECK();
+        rc = poll();
+        if (chunk < 64) chunk *= 2;
+    }
+    if (rc != SW_OK) return rc;
+    hipLaunchKernelGGL(batch_cg_grad_sub_kernel, dim3(nb), blk, 0, s, solves, it, n_items);
+    SW_LAUNCH_CHECK();
+    for (int k = 0; k < n_solves; ++k) {
+        info[k].iters = h[k].iters;
+        info[k].relres = h[k].relres;
+        info[k].max_div = h[k].max_div;
+        info[k].reason = h[k].reason;
+        info[k].pad = 0;
+    }
+    return SW_OK;
+}
+
+}  // extern "C"

@@ -28,26 +28,11 @@
 #include <type_traits>
 #include <utility>
 
+#include "march_step.h"
 #include "sw_common.h"
 
 namespace sw {
 namespace {
-
-template <typename T>
-struct StepArgs {
-    const T* A[3];  // u_n
-    const T* B[3];  // d_n (increment) or u_{n-1} (leapfrog)
-    T* Ao[3];       // u_{n+1}
-    T* Bo[3];       // d_{n+1} (increment form)
-    const T* m;     // mu / rho
-    const T* e;     // eta / rho
-    const T* f[3];  // dT^2 f_S / rho
-    Grid g;
-    T cx, cy, cz;  // dT^2 / h^2 per axis
-    T vx, vy, vz;  // dT / h^2 per axis
-    T damp;        // 1 - vel_damp
-    T fs;          // env[it]
-};
 
 template <typename T, bool INCR, bool VISC, bool FORCE>
 __global__ void __launch_bounds__(256) step_direct_kernel(StepArgs<T> a) {
@@ -86,186 +71,22 @@ __global__ void __launch_bounds__(256) step_direct_kernel(StepArgs<T> a) {
     });
 }
 
-// One (y, z) column's state at one x plane: u and d = u - u_prev.
-template <typename T>
-struct Col {
-    T u0, u1, u2, d0, d1, d2;
-};
-
-// Per-voxel coefficients: mu/rho, eta/rho and the force.
-template <typename T>
-struct Coef {
-    T m, e, f0, f1, f2;
-};
-
-// Loads are unconditional from a clamped offset (voxel 0 when `ok` is false).
-// Values loaded for points outside the grid only ever feed boundary voxels,
-// whose outputs are overwritten with zero, so no select follows the load and
-// the load's latency stays hidden behind the next barrier. (Voxel 0 is on the
-// boundary layer, which holds zero in every state array.)
-template <typename T, bool INCR>
-__device__ __forceinline__ Col<T> load_col(const StepArgs<T>& a, int64_t o, bool ok) {
-    const int64_t q = ok ? o : 0;
-    const T u0 = a.A[0][q], u1 = a.A[1][q], u2 = a.A[2][q];
-    const T b0 = a.B[0][q], b1 = a.B[1][q], b2 = a.B[2][q];
-    Col<T> c;
-    c.u0 = u0;
-    c.u1 = u1;
-    c.u2 = u2;
-    c.d0 = INCR ? b0 : u0 - b0;
-    c.d1 = INCR ? b1 : u1 - b1;
-    c.d2 = INCR ? b2 : u2 - b2;
-    return c;
-}
-
-template <typename T, bool VISC, bool FORCE>
-__device__ __forceinline__ Coef<T> load_coef(const StepArgs<T>& a, int64_t o, bool ok) {
-    const int64_t q = ok ? o : 0;
-    Coef<T> k;
-    k.m = a.m[q];
-    k.e = VISC ? a.e[q] : T(0);
-    k.f0 = FORCE ? a.f[0][q] : T(0);
-    k.f1 = FORCE ? a.f[1][q] : T(0);
-    k.f2 = FORCE ? a.f[2][q] : T(0);
-    return k;
-}
-
-template <typename T, bool VISC>
-__device__ __forceinline__ T comp_inc(const StepArgs<T>& a, T u0, T um, T up, T uyp, T uym, T uzp, T uzm, T d0, T dm,
-                                      T dp, T dyp, T dym, T dzp, T dzm, T m, T e, T f) {
-    const T lu = a.cx * (up + um - T(2) * u0) + a.cy * (uyp + uym - T(2) * u0) + a.cz * (uzp + uzm - T(2) * u0);
-    T inc = d0 + m * lu + f;
-    if (VISC) {
-        const T ld = a.vx * (dp + dm - T(2) * d0) + a.vy * (dyp + dym - T(2) * d0) + a.vz * (dzp + dzm - T(2) * d0);
-        inc += e * ld;
-    }
-    return inc * a.damp;
-}
-
 template <typename T, bool INCR, bool VISC, bool FORCE, int BY>
 __global__ void __launch_bounds__(64 * BY) step_march_kernel(StepArgs<T> a, int xchunk, int nty, int ntz) {
-    constexpr int BZ = 64;
-    constexpr int NL = VISC ? 6 : 3;  // planes staged in LDS: u (3) and, with viscosity, d (3)
-    constexpr int LY = BY + 2;
-    constexpr int LZ = BZ + 2;
-    __shared__ T lds[2][NL][LY][LZ];
-
-    const Grid g = a.g;
-    const int tz = threadIdx.x, ty = threadIdx.y;
-    const int tid = ty * BZ + tz;
-
-    // XCD-aware tile order: the hardware deals blocks round-robin to the 8 XCDs.
+    __shared__ T lds[2][VISC ? 6 : 3][BY + 2][66];
+    // XCD-aware tile order: the hardware deals blocks round-robin to the 8 XCDs;
+    // renumber so that consecutive tiles (which share halos) land on one XCD.
     const unsigned nb = gridDim.x;
     unsigned b = blockIdx.x;
     if ((nb & 7u) == 0u) b = (b & 7u) * (nb >> 3) + (b >> 3);
-    const int tyi = (int)(b % (unsigned)nty);
-    const unsigned rest = b / (unsigned)nty;
-    const int tzi = (int)(rest % (unsigned)ntz);
-    const int txi = (int)(rest / (unsigned)ntz);
-    const int y0 = tyi * BY, z0 = tzi * BZ, x0 = txi * xchunk;
-    const int x1 = min(x0 + xchunk, g.nx);
-    const int y = y0 + ty, z = z0 + tz;
-    const bool inyz = (y < g.ny) && (z < g.nz);
-    const int64_t P = g.plane();
-    const int64_t yz = inyz ? (int64_t)y * g.nz + z : 0;
-    const bool yz_interior = inyz && y > 0 && y < g.ny - 1 && z > 0 && z < g.nz - 1;
-
-    // halo points of the tile: two rows of 64 (y0-1, y0+BY) and two columns of BY (z0-1, z0+64)
-    int hy = -1, hz = -1, ly = 0, lz = 0;
-    if (tid < 64) {
-        hy = y0 - 1, hz = z0 + tid, ly = 0, lz = tid + 1;
-    } else if (tid < 128) {
-        hy = y0 + BY, hz = z0 + tid - 64, ly = BY + 1, lz = tid - 63;
-    } else if (tid < 128 + BY) {
-        hy = y0 + tid - 128, hz = z0 - 1, ly = tid - 127, lz = 0;
-    } else if (tid < 128 + 2 * BY) {
-        hy = y0 + tid - 128 - BY, hz = z0 + BZ, ly = tid - 127 - BY, lz = BZ + 1;
-    }
-    const bool halo_thread = tid < 128 + 2 * BY;
-    const bool halo_in = halo_thread && hy >= 0 && hy < g.ny && hz >= 0 && hz < g.nz;
-    const int64_t hyz = halo_in ? (int64_t)hy * g.nz + hz : 0;
-
-    auto center = [&](int xx) { return load_col<T, INCR>(a, (int64_t)xx * P + yz, inyz && xx >= 0 && xx < g.nx); };
-    auto halo = [&](int xx) {
-        Col<T> c{};
-        if (halo_thread) c = load_col<T, INCR>(a, (int64_t)xx * P + hyz, halo_in && xx >= 0 && xx < g.nx);
-        return c;
-    };
-    auto coef = [&](int xx) {
-        return load_coef<T, VISC, FORCE>(a, (int64_t)xx * P + yz, yz_interior && xx > 0 && xx < g.nx - 1);
-    };
-
-    // One x plane. m, c, p: planes x-1, x, x+1 of this column; n receives x+2.
-    // hc: this plane's halo point, hn receives the next one; kc / kn likewise for coefficients.
-    auto plane = [&](auto buf, const Col<T>& m, const Col<T>& c, const Col<T>& p, Col<T>& n, const Col<T>& hc,
-                     Col<T>& hn, const Coef<T>& kc, Coef<T>& kn, int x) {
-        constexpr int B = decltype(buf)::value;
-        auto& L = lds[B];
-        L[0][ty + 1][tz + 1] = c.u0;
-        L[1][ty + 1][tz + 1] = c.u1;
-        L[2][ty + 1][tz + 1] = c.u2;
-        if (VISC) {
-            L[NL - 3][ty + 1][tz + 1] = c.d0;
-            L[NL - 2][ty + 1][tz + 1] = c.d1;
-            L[NL - 1][ty + 1][tz + 1] = c.d2;
-        }
-        if (halo_thread) {
-            L[0][ly][lz] = hc.u0;
-            L[1][ly][lz] = hc.u1;
-            L[2][ly][lz] = hc.u2;
-            if (VISC) {
-                L[NL - 3][ly][lz] = hc.d0;
-                L[NL - 2][ly][lz] = hc.d1;
-                L[NL - 1][ly][lz] = hc.d2;
-            }
-        }
-        n = center(x + 2);
-        hn = halo(x + 1);
-        kn = coef(x + 1);
-        __syncthreads();
-        if (!inyz) return;
-        const bool in = yz_interior && x > 0 && x < g.nx - 1;
-        const int r = ty + 1, q = tz + 1;
-        auto nb = [&](int f, int dy, int dz) { return L[f][r + dy][q + dz]; };
-        T i0 = comp_inc<T, VISC>(a, c.u0, m.u0, p.u0, nb(0, 1, 0), nb(0, -1, 0), nb(0, 0, 1), nb(0, 0, -1), c.d0, m.d0,
-                                 p.d0, VISC ? nb(NL - 3, 1, 0) : T(0), VISC ? nb(NL - 3, -1, 0) : T(0),
-                                 VISC ? nb(NL - 3, 0, 1) : T(0), VISC ? nb(NL - 3, 0, -1) : T(0), kc.m, kc.e,
-                                 a.fs * kc.f0);
-        T i1 = comp_inc<T, VISC>(a, c.u1, m.u1, p.u1, nb(1, 1, 0), nb(1, -1, 0), nb(1, 0, 1), nb(1, 0, -1), c.d1, m.d1,
-                                 p.d1, VISC ? nb(NL - 2, 1, 0) : T(0), VISC ? nb(NL - 2, -1, 0) : T(0),
-                                 VISC ? nb(NL - 2, 0, 1) : T(0), VISC ? nb(NL - 2, 0, -1) : T(0), kc.m, kc.e,
-                                 a.fs * kc.f1);
-        T i2 = comp_inc<T, VISC>(a, c.u2, m.u2, p.u2, nb(2, 1, 0), nb(2, -1, 0), nb(2, 0, 1), nb(2, 0, -1), c.d2, m.d2,
-                                 p.d2, VISC ? nb(NL - 1, 1, 0) : T(0), VISC ? nb(NL - 1, -1, 0) : T(0),
-                                 VISC ? nb(NL - 1, 0, 1) : T(0), VISC ? nb(NL - 1, 0, -1) : T(0), kc.m, kc.e,
-                                 a.fs * kc.f2);
-        if (!in) i0 = i1 = i2 = T(0);
-        const int64_t o = (int64_t)x * P + yz;
-        a.Ao[0][o] = in ? c.u0 + i0 : T(0);
-        a.Ao[1][o] = in ? c.u1 + i1 : T(0);
-        a.Ao[2][o] = in ? c.u2 + i2 : T(0);
-        if (INCR) {
-            a.Bo[0][o] = i0;
-            a.Bo[1][o] = i1;
-            a.Bo[2][o] = i2;
-        }
-    };
-
-    using B0 = std::integral_constant<int, 0>;
-    using B1 = std::integral_constant<int, 1>;
-    Col<T> q0 = center(x0 - 1), q1 = center(x0), q2 = center(x0 + 1), q3;
-    Col<T> h0 = halo(x0), h1;
-    Coef<T> k0 = coef(x0), k1;
-    // unrolled by four so the queue's roles rotate without register moves
-    for (int x = x0; x < x1; x += 4) {
-        plane(B0{}, q0, q1, q2, q3, h0, h1, k0, k1, x);
-        if (x + 1 >= x1) break;
-        plane(B1{}, q1, q2, q3, q0, h1, h0, k1, k0, x + 1);
-        if (x + 2 >= x1) break;
-        plane(B0{}, q2, q3, q0, q1, h0, h1, k0, k1, x + 2);
-        if (x + 3 >= x1) break;
-        plane(B1{}, q3, q0, q1, q2, h1, h0, k1, k0, x + 3);
-    }
+    // z fastest, then y, then x: the tiles an XCD receives form whole z rows
+    // of one x chunk, so z and most y halo lines are fetched once into its L2
+    const int tzi = (int)(b % (unsigned)ntz);
+    const unsigned rest = b / (unsigned)ntz;
+    const int tyi = (int)(rest % (unsigned)nty);
+    const int txi = (int)(rest / (unsigned)nty);
+    const int x0 = txi * xchunk;
+    march_tile<T, INCR, VISC, FORCE, BY>(a, x0, min(x0 + xchunk, a.g.nx), tyi * BY, tzi * 64, lds);
 }
 
 struct MarchShape {
@@ -273,18 +94,20 @@ struct MarchShape {
     unsigned blocks() const { return (unsigned)nty * ntz * ntx; }
 };
 
-MarchShape march_shape(const Grid& g, int dtype, int tile_y, int x_chunk) {
+// Launch shape: tile rows `by` (4 or 8), and an x chunk that makes the grid
+// one full wave of resident blocks (occupancy of this instantiation x CUs),
+// unless the caller fixes it.
+MarchShape march_shape(const Grid& g, int by, int x_chunk, int resident_per_cu) {
     MarchShape s;
-    s.by = tile_y > 0 ? tile_y : (dtype == SW_F32 ? 8 : 4);
+    s.by = by;
     s.nty = (g.ny + s.by - 1) / s.by;
     s.ntz = (g.nz + 63) / 64;
     if (x_chunk > 0) {
         s.xchunk = x_chunk;
     } else {
-        // enough tiles for ~8 resident blocks per CU, chunks of at least 8 planes
-        const int64_t target = (int64_t)device_cu_count() * 8;
+        const int64_t target = (int64_t)device_cu_count() * (resident_per_cu > 0 ? resident_per_cu : 2);
         const int64_t cols = (int64_t)s.nty * s.ntz;
-        int64_t ntx = (target + cols - 1) / cols;
+        int64_t ntx = (target + cols / 2) / cols;
         if (ntx < 1) ntx = 1;
         int64_t xc = (g.nx + ntx - 1) / ntx;
         if (xc < 8) xc = 8;
@@ -292,6 +115,13 @@ MarchShape march_shape(const Grid& g, int dtype, int tile_y, int x_chunk) {
     }
     s.ntx = (g.nx + s.xchunk - 1) / s.xchunk;
     return s;
+}
+
+template <typename K>
+int resident_blocks(K kernel, int threads) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, threads, 0) != hipSuccess || n < 1) n = 1;
+    return n;
 }
 
 template <typename T, bool INCR, bool VISC, bool FORCE>
@@ -302,20 +132,20 @@ int launch_step(const StepArgs<T>& a, int kernel, int tile_y, int x_chunk, hipSt
         SW_LAUNCH_CHECK();
         return SW_OK;
     }
-    const MarchShape ms = march_shape(g, DType<T>::code, tile_y, x_chunk);
-    const dim3 grid(ms.blocks());
-    switch (ms.by) {
-        case 4:
-            hipLaunchKernelGGL((step_march_kernel<T, INCR, VISC, FORCE, 4>), grid, dim3(64, 4), 0, s, a, ms.xchunk,
-                               ms.nty, ms.ntz);
-            break;
-        case 8:
-            hipLaunchKernelGGL((step_march_kernel<T, INCR, VISC, FORCE, 8>), grid, dim3(64, 8), 0, s, a, ms.xchunk,
-                               ms.nty, ms.ntz);
-            break;
-        default:
-            sw::set_error("tile_y %d not supported (4 or 8)", ms.by);
-            return SW_ERR_ARG;
+    const int by = tile_y > 0 ? tile_y : (DType<T>::code == SW_F32 ? 8 : 4);
+    if (by == 4) {
+        auto k = step_march_kernel<T, INCR, VISC, FORCE, 4>;
+        static const int res = resident_blocks(k, 256);
+        const MarchShape ms = march_shape(g, 4, x_chunk, res);
+        hipLaunchKernelGGL(k, dim3(ms.blocks()), dim3(64, 4), 0, s, a, ms.xchunk, ms.nty, ms.ntz);
+    } else if (by == 8) {
+        auto k = step_march_kernel<T, INCR, VISC, FORCE, 8>;
+        static const int res = resident_blocks(k, 512);
+        const MarchShape ms = march_shape(g, 8, x_chunk, res);
+        hipLaunchKernelGGL(k, dim3(ms.blocks()), dim3(64, 8), 0, s, a, ms.xchunk, ms.nty, ms.ntz);
+    } else {
+        sw::set_error("tile_y %d not supported (4 or 8)", by);
+        return SW_ERR_ARG;
     }
     SW_LAUNCH_CHECK();
     return SW_OK;

@@ -1,0 +1,358 @@
+// Shared device code of the projection's conjugate gradients (poisson_cg,
+// solver.py:1010-1050): the device-side CG state, deterministic slot
+// reductions, and the x-marching tiles of the two fused kernels of one
+// iteration. Used by the single-solve kernels (project.hip) and the batched
+// ones (batch.hip), so a solve's arithmetic is identical in both.
+#pragma once
+
+#include <cmath>
+#include <type_traits>
+
+#include "sw_common.h"
+
+namespace sw {
+
+// Device-resident state of one CG solve. The host only reads stop/reason/iters.
+struct CgState {
+    double rs_old, rs0, alpha, beta, relres, max_div, tol, pad0;
+    long long iters;
+    int stop, reason;
+    unsigned int counter;  // slots published in the current kernel
+    int pad1;
+    long long max_iters;   // stop after this many phi updates (solver.py:1033)
+    int pad2[4];
+};
+static_assert(sizeof(CgState) <= 128, "CgState too large");
+
+__device__ inline double load_coherent(const double* p) {
+    unsigned long long v =
+        __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __longlong_as_double((long long)v);
+}
+
+// Publish a block-reduced partial into slot `slot` of `n`; returns true (in
+// every thread) in the block that published the last slot. The counter is
+// reset by that block.
+__device__ inline bool publish_slot(double* part_sum, double* part_max, int slot, double s, double m,
+                                    unsigned int* counter, unsigned int n) {
+    __shared__ int last;
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+    if (tid == 0) {
+        part_sum[slot] = s;
+        if (part_max) part_max[slot] = m;
+        __threadfence();
+        const unsigned prev = atomicAdd(counter, 1u);
+        last = (prev == n - 1);
+    }
+    __syncthreads();
+    if (last) __threadfence();
+    return last;
+}
+
+// In the last block: sum (and max) over the n slots in a fixed order.
+__device__ inline void reduce_slots(const double* part_sum, const double* part_max, int n, double* red,
+                                    double* out_sum, double* out_max) {
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+    const int nt = blockDim.x * blockDim.y;
+    double s = 0.0, m = 0.0;
+    for (int k = tid; k < n; k += nt) {
+        s += load_coherent(part_sum + k);
+        if (part_max) m = fmax(m, load_coherent(part_max + k));
+    }
+    *out_sum = block_sum(s, red);
+    if (part_max) *out_max = block_max(m, red);
+}
+
+constexpr double kEps64 = 2.220446049250313e-16;
+
+// After the divergence kernel: r0.r0 and max|div| (solver.py:672, 1028-1031).
+__device__ inline void cg_after_init(CgState* st, double rs, double maxdiv) {
+    st->rs_old = rs;
+    st->rs0 = fmax(rs, kEps64);
+    st->relres = sqrt(rs / fmax(rs, kEps64));
+    st->max_div = maxdiv;
+    st->counter = 0u;
+    if (maxdiv < 1e-12) {  // nothing to remove
+        st->stop = 1;
+        st->reason = SW_CG_EARLY_EXIT;
+        st->relres = 0.0;
+    }
+}
+
+// After p.Ap (solver.py:1034-1038).
+__device__ inline void cg_after_dir(CgState* st, double denom) {
+    st->counter = 0u;
+    if (!isfinite(denom) || denom <= 0.0) {
+        st->stop = 1;
+        st->reason = SW_CG_DENOM;
+    } else {
+        st->alpha = st->rs_old / denom;
+    }
+}
+
+// After r.r (solver.py:1041-1048).
+__device__ inline void cg_after_step(CgState* st, double rs_new) {
+    st->counter = 0u;
+    st->iters += 1;
+    if (!isfinite(rs_new)) {
+        st->stop = 1;
+        st->reason = SW_CG_NONFINITE;
+        return;
+    }
+    const double rel = sqrt(rs_new / st->rs0);
+    st->relres = rel;
+    if (rel < st->tol) {
+        st->stop = 1;
+        st->reason = SW_CG_CONVERGED;
+    } else {
+        st->beta = rs_new / fmax(st->rs_old, kEps64);
+        st->rs_old = rs_new;
+        if (st->iters >= st->max_iters) {
+            st->stop = 1;
+            st->reason = SW_CG_MAX_ITERS;
+        }
+    }
+}
+
+// Fresh state for a solve with tolerance tol and at most max_iters iterations.
+__device__ inline void cg_reset(CgState* st, double tol, long long max_iters) {
+    st->rs_old = st->rs0 = st->alpha = st->beta = st->relres = st->max_div = 0.0;
+    st->tol = tol;
+    st->iters = 0;
+    st->stop = max_iters <= 0 ? 1 : 0;
+    st->reason = max_iters <= 0 ? SW_CG_MAX_ITERS : SW_CG_RUNNING;
+    st->counter = 0u;
+    st->max_iters = max_iters;
+}
+
+// ---------------------------------------------------------------------------
+// x-marching CG tiles. A (BY x 64) tile of (y, z) columns marches along x; p
+// at planes x-2 .. x+3 of the column sits in a register queue, and plane x
+// (with a two-voxel halo in y and z, the reach of A) is staged in LDS, so r,
+// p_old, p and phi are each read from HBM about once per iteration. The
+// queue's roles rotate over a loop unrolled by six, so no prefetched register
+// is moved.
+
+constexpr int kCgBY = 8;   // tile rows in y (block of 512 threads)
+constexpr int kCgLZ = 68;  // 64 + 2 + 2
+
+using CgLds = double[2][kCgBY + 4][kCgLZ];
+
+struct CgTile {
+    int x0, x1, y, z;
+    bool inyz;
+    int64_t yz;
+    bool halo_thread, halo_in;
+    int ly, lz;
+    int64_t hyz;
+};
+
+__device__ inline CgTile cg_tile(const Grid& g, int x0, int x1, int y0, int z0) {
+    CgTile t;
+    const int tz = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tz;
+    t.x0 = x0;
+    t.x1 = x1;
+    t.y = y0 + ty;
+    t.z = z0 + tz;
+    t.inyz = t.y < g.ny && t.z < g.nz;
+    t.yz = t.inyz ? (int64_t)t.y * g.nz + t.z : 0;
+    // halo: rows y0-2, y0-1, y0+BY, y0+BY+1 (4 x 64 points), columns z0-2, z0-1, z0+64, z0+65 (4 x BY)
+    int hy = -1, hz = -1;
+    t.ly = t.lz = 0;
+    if (tid < 256) {
+        const int rr = tid >> 6;
+        hy = rr < 2 ? y0 - 2 + rr : y0 + kCgBY + rr - 2;
+        t.ly = rr < 2 ? rr : kCgBY + rr;
+        hz = z0 + (tid & 63);
+        t.lz = (tid & 63) + 2;
+    } else if (tid < 256 + 4 * kCgBY) {
+        const int k = tid - 256, cc = k / kCgBY, yy = k % kCgBY;
+        hy = y0 + yy;
+        t.ly = yy + 2;
+        hz = cc < 2 ? z0 - 2 + cc : z0 + 64 + cc - 2;
+        t.lz = cc < 2 ? cc : 64 + cc;
+    }
+    t.halo_thread = tid < 256 + 4 * kCgBY;
+    t.halo_in = t.halo_thread && hy >= 0 && hy < g.ny && hz >= 0 && hz < g.nz;
+    t.hyz = t.halo_in ? (int64_t)hy * g.nz + hz : 0;
+    return t;
+}
+
+// A p at an interior voxel: x terms from the queue (a = x-2, c = x, e = x+2),
+// y and z terms from the LDS plane; rows next to the wall drop G on the
+// boundary layer (solver.py:1024-1026 with gradient_center zero there).
+__device__ inline double cg_apply(const Grid& g, int x, int y, int z, double a, double c, double e,
+                                  const double (*L)[kCgLZ], int r, int q, double qx, double qy, double qz) {
+    const double tx = ((x + 1 < g.nx - 1) ? (e - c) : 0.0) - ((x - 1 > 0) ? (c - a) : 0.0);
+    const double ty = ((y + 1 < g.ny - 1) ? (L[r + 2][q] - c) : 0.0) - ((y - 1 > 0) ? (c - L[r - 2][q]) : 0.0);
+    const double tz = ((z + 1 < g.nz - 1) ? (L[r][q + 2] - c) : 0.0) - ((z - 1 > 0) ? (c - L[r][q - 2]) : 0.0);
+    return -(tx * qx + ty * qy + tz * qz);
+}
+
+// p = r + beta p_old (one fma, so every recomputation is bit-identical), A p.
+// Writes p over the tile; returns this thread's share of p.Ap.
+__device__ inline double cg_dir_tile(const double* __restrict__ r, const double* __restrict__ p_old,
+                                     double* __restrict__ p, const Grid& g, double beta, double qx, double qy,
+                                     double qz, const CgTile& t, CgLds& lds) {
+    const int64_t P = g.plane();
+    const int r0 = threadIdx.y + 2, q0 = threadIdx.x + 2;
+    // voxel 0 is on the boundary layer, where r and p_old are zero
+    auto pn = [&](int64_t o, bool ok) {
+        const int64_t k = ok ? o : 0;
+        return fma(beta, p_old[k], r[k]);
+    };
+    auto center = [&](int xx) { return pn((int64_t)xx * P + t.yz, t.inyz && xx >= 0 && xx < g.nx); };
+    auto halo = [&](int xx) {
+        return t.halo_thread ? pn((int64_t)xx * P + t.hyz, t.halo_in && xx >= 0 && xx < g.nx) : 0.0;
+    };
+    double s = 0.0;
+    auto plane = [&](auto buf, double a, double c, double e, double& f, double hc, double& hn, int x) {
+        constexpr int B = decltype(buf)::value;
+        lds[B][r0][q0] = c;
+        if (t.halo_thread) lds[B][t.ly][t.lz] = hc;
+        f = center(x + 3);
+        hn = halo(x + 1);
+        __syncthreads();
+        if (!t.inyz) return;
+        const bool in = g.interior(x, t.y, t.z);
+        if (in) s += c * cg_apply(g, x, t.y, t.z, a, c, e, lds[B], r0, q0, qx, qy, qz);
+        p[(int64_t)x * P + t.yz] = in ? c : 0.0;
+    };
+    using B0 = std::integral_constant<int, 0>;
+    using B1 = std::integral_constant<int, 1>;
+    const int x0 = t.x0, x1 = t.x1;
+    double v0 = center(x0 - 2), v1 = center(x0 - 1), v2 = center(x0), v3 = center(x0 + 1), v4 = center(x0 + 2), v5;
+    double h0 = halo(x0), h1;
+    for (int x = x0; x < x1; x += 6) {
+        plane(B0{}, v0, v2, v4, v5, h0, h1, x);
+        if (x + 1 >= x1) break;
+        plane(B1{}, v1, v3, v5, v0, h1, h0, x + 1);
+        if (x + 2 >= x1) break;
+        plane(B0{}, v2, v4, v0, v1, h0, h1, x + 2);
+        if (x + 3 >= x1) break;
+        plane(B1{}, v3, v5, v1, v2, h1, h0, x + 3);
+        if (x + 4 >= x1) break;
+        plane(B0{}, v4, v0, v2, v3, h0, h1, x + 4);
+        if (x + 5 >= x1) break;
+        plane(B1{}, v5, v1, v3, v4, h1, h0, x + 5);
+    }
+    return s;
+}
+
+// phi += alpha p, r -= alpha A p (A p recomputed from p) over the tile;
+// returns this thread's share of r.r.
+__device__ inline double cg_step_tile(double* __restrict__ phi, double* __restrict__ r, const double* __restrict__ p,
+                                      const Grid& g, double alpha, double qx, double qy, double qz, const CgTile& t,
+                                      CgLds& lds) {
+    const int64_t P = g.plane();
+    const int r0 = threadIdx.y + 2, q0 = threadIdx.x + 2;
+    auto ld = [&](const double* f, int64_t o, bool ok) { return f[ok ? o : 0]; };
+    auto center = [&](int xx) { return ld(p, (int64_t)xx * P + t.yz, t.inyz && xx >= 0 && xx < g.nx); };
+    auto halo = [&](int xx) {
+        return t.halo_thread ? ld(p, (int64_t)xx * P + t.hyz, t.halo_in && xx >= 0 && xx < g.nx) : 0.0;
+    };
+    auto ldr = [&](int xx) { return ld(r, (int64_t)xx * P + t.yz, t.inyz && xx >= 0 && xx < g.nx); };
+    auto ldphi = [&](int xx) { return ld(phi, (int64_t)xx * P + t.yz, t.inyz && xx >= 0 && xx < g.nx); };
+    double s = 0.0;
+    auto plane = [&](auto buf, double a, double c, double e, double& f, double hc, double& hn, double rc, double& rn,
+                     double fc, double& fn, int x) {
+        constexpr int B = decltype(buf)::value;
+        lds[B][r0][q0] = c;
+        if (t.halo_thread) lds[B][t.ly][t.lz] = hc;
+        f = center(x + 3);
+        hn = halo(x + 1);
+        rn = ldr(x + 1);
+        fn = ldphi(x + 1);
+        __syncthreads();
+        if (!t.inyz || !g.interior(x, t.y, t.z)) return;
+        const double ap = cg_apply(g, x, t.y, t.z, a, c, e, lds[B], r0, q0, qx, qy, qz);
+        const int64_t o = (int64_t)x * P + t.yz;
+        phi[o] = fc + alpha * c;
+        const double rv = rc - alpha * ap;
+        r[o] = rv;
+        s += rv * rv;
+    };
+    using B0 = std::integral_constant<int, 0>;
+    using B1 = std::integral_constant<int, 1>;
+    const int x0 = t.x0, x1 = t.x1;
+    double v0 = center(x0 - 2), v1 = center(x0 - 1), v2 = center(x0), v3 = center(x0 + 1), v4 = center(x0 + 2), v5;
+    double h0 = halo(x0), h1;
+    double ra = ldr(x0), rb, fa = ldphi(x0), fb;
+    for (int x = x0; x < x1; x += 6) {
+        plane(B0{}, v0, v2, v4, v5, h0, h1, ra, rb, fa, fb, x);
+        if (x + 1 >= x1) break;
+        plane(B1{}, v1, v3, v5, v0, h1, h0, rb, ra, fb, fa, x + 1);
+        if (x + 2 >= x1) break;
+        plane(B0{}, v2, v4, v0, v1, h0, h1, ra, rb, fa, fb, x + 2);
+        if (x + 3 >= x1) break;
+        plane(B1{}, v3, v5, v1, v2, h1, h0, rb, ra, fb, fa, x + 3);
+        if (x + 4 >= x1) break;
+        plane(B0{}, v4, v0, v2, v3, h0, h1, ra, rb, fa, fb, x + 4);
+        if (x + 5 >= x1) break;
+        plane(B1{}, v5, v1, v3, v4, h1, h0, rb, ra, fb, fa, x + 5);
+    }
+    return s;
+}
+
+// r0 = -div_c(mask f) over a tile (no LDS: each f component is read at +-1
+// along its own axis only), p_old = 0, phi = 0. Returns r.r and max|div|.
+template <typename T>
+__device__ inline void cg_init_tile(const T* __restrict__ fx, const T* __restrict__ fy, const T* __restrict__ fz,
+                                    double* __restrict__ phi, double* __restrict__ r, double* __restrict__ p_old,
+                                    const Grid& g, double i2hx, double i2hy, double i2hz, const CgTile& t, double& s,
+                                    double& mx) {
+    if (!t.inyz) return;
+    const int64_t P = g.plane();
+    const int64_t R = g.nz;
+    for (int x = t.x0; x < t.x1; ++x) {
+        const int64_t i = (int64_t)x * P + t.yz;
+        double rv = 0.0;
+        if (g.interior(x, t.y, t.z)) {
+            // neighbours on the boundary layer are masked to zero (solver.py:666-668)
+            const double xp = (x + 1 < g.nx - 1) ? (double)fx[i + P] : 0.0;
+            const double xm = (x - 1 > 0) ? (double)fx[i - P] : 0.0;
+            const double yp = (t.y + 1 < g.ny - 1) ? (double)fy[i + R] : 0.0;
+            const double ym = (t.y - 1 > 0) ? (double)fy[i - R] : 0.0;
+            const double zp = (t.z + 1 < g.nz - 1) ? (double)fz[i + 1] : 0.0;
+            const double zm = (t.z - 1 > 0) ? (double)fz[i - 1] : 0.0;
+            const double div = (xp - xm) * i2hx + (yp - ym) * i2hy + (zp - zm) * i2hz;
+            rv = -div;
+            s += rv * rv;
+            mx = fmax(mx, fabs(div));
+        }
+        r[i] = rv;
+        p_old[i] = 0.0;
+        phi[i] = 0.0;
+    }
+}
+
+// f_S = mask (mask f - grad_c phi) over a tile; optionally out = scale f_S inv_rho.
+template <typename T>
+__device__ inline void cg_grad_sub_tile(T* __restrict__ fx, T* __restrict__ fy, T* __restrict__ fz,
+                                        const double* __restrict__ phi, const Grid& g, double i2hx, double i2hy,
+                                        double i2hz, T* __restrict__ ox, T* __restrict__ oy, T* __restrict__ oz,
+                                        const T* __restrict__ inv_rho, double scale, const CgTile& t) {
+    if (!t.inyz) return;
+    const int64_t P = g.plane();
+    const int64_t R = g.nz;
+    for (int x = t.x0; x < t.x1; ++x) {
+        const int64_t i = (int64_t)x * P + t.yz;
+        double a = 0.0, b = 0.0, c = 0.0;
+        if (g.interior(x, t.y, t.z)) {
+            a = (double)fx[i] - (phi[i + P] - phi[i - P]) * i2hx;
+            b = (double)fy[i] - (phi[i + R] - phi[i - R]) * i2hy;
+            c = (double)fz[i] - (phi[i + 1] - phi[i - 1]) * i2hz;
+        }
+        fx[i] = (T)a;
+        fy[i] = (T)b;
+        fz[i] = (T)c;
+        if (ox) {
+            const double w = scale * (inv_rho ? (double)inv_rho[i] : 1.0);
+            ox[i] = (T)(a * w);
+            oy[i] = (T)(b * w);
+            oz[i] = (T)(c * w);
+        }
+    }
+}
+
+}  // namespace sw

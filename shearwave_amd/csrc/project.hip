@@ -20,71 +20,15 @@
 #include <cstdlib>
 #include <type_traits>
 
+#include "march_cg.h"
 #include "sw_common.h"
 
 namespace sw {
 namespace {
 
-struct CgState {
-    double rs_old, rs0, alpha, beta, relres, max_div, tol, pad0;
-    long long iters;
-    int stop, reason;
-    unsigned int counter;
-    int pad1[7];
-};
-static_assert(sizeof(CgState) <= 128, "CgState too large");
-
 constexpr int kMaxRedBlocks = 2048;
 
-__device__ inline double load_coherent(const double* p) {
-    unsigned long long v =
-        __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return __longlong_as_double((long long)v);
-}
-
-// Publish this block's partial(s); returns true (in every thread) in the block that finished last.
-__device__ inline bool publish_partial(double* part_sum, double s, double* part_max, double m, CgState* st) {
-    __shared__ int last;
-    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
-    if (tid == 0) {
-        part_sum[blockIdx.x] = s;
-        if (part_max) part_max[blockIdx.x] = m;
-        __threadfence();
-        unsigned prev = atomicAdd(&st->counter, 1u);
-        last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (last) __threadfence();
-    return last;
-}
-
-// In the last block: fixed-order sum (and max) over all partials.
-__device__ inline void reduce_partials(const double* part_sum, const double* part_max, double* red, double* out_sum,
-                                       double* out_max) {
-    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
-    const int nt = blockDim.x * blockDim.y;
-    double s = 0.0, m = 0.0;
-    for (int k = tid; k < (int)gridDim.x; k += nt) {
-        s += load_coherent(part_sum + k);
-        if (part_max) m = fmax(m, load_coherent(part_max + k));
-    }
-    *out_sum = block_sum(s, red);
-    if (part_max) *out_max = block_max(m, red);
-}
-
-__global__ void cg_reset_kernel(CgState* st, double tol) {
-    st->rs_old = 0.0;
-    st->rs0 = 0.0;
-    st->alpha = 0.0;
-    st->beta = 0.0;
-    st->relres = 0.0;
-    st->max_div = 0.0;
-    st->tol = tol;
-    st->iters = 0;
-    st->stop = 0;
-    st->reason = SW_CG_RUNNING;
-    st->counter = 0u;
-}
+__global__ void cg_reset_kernel(CgState* st, double tol, long long max_iters) { cg_reset(st, tol, max_iters); }
 
 // r = -div_c(mask f) on the interior, p_old = 0, phi = 0; reduces r.r and max|div|.
 template <typename T>
@@ -118,22 +62,10 @@ __global__ void __launch_bounds__(256) cg_init_kernel(const T* __restrict__ fx, 
     });
     s = block_sum(s, red);
     mx = block_max(mx, red);
-    if (publish_partial(part_sum, s, part_max, mx, st)) {
+    if (publish_slot(part_sum, part_max, blockIdx.x, s, mx, &st->counter, gridDim.x)) {
         double tot, m;
-        reduce_partials(part_sum, part_max, red, &tot, &m);
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            const double eps = 2.220446049250313e-16;
-            st->rs_old = tot;
-            st->rs0 = fmax(tot, eps);
-            st->relres = sqrt(tot / fmax(tot, eps));
-            st->max_div = m;
-            st->counter = 0u;
-            if (m < 1e-12) {  // solver.py:672: nothing to remove
-                st->stop = 1;
-                st->reason = SW_CG_EARLY_EXIT;
-                st->relres = 0.0;
-            }
-        }
+        reduce_slots(part_sum, part_max, gridDim.x, red, &tot, &m);
+        if (threadIdx.x == 0 && threadIdx.y == 0) cg_after_init(st, tot, m);
     }
 }
 
@@ -155,7 +87,7 @@ __global__ void __launch_bounds__(256) cg_dir_kernel(const double* __restrict__ 
     SW_FOR_EACH_VOXEL(g, {
         double pv = 0.0;
         if (g.interior(x, y, z)) {
-            auto pn = [&](int64_t k) { return r[i + k] + beta * p_old[i + k]; };
+            auto pn = [&](int64_t k) { return fma(beta, p_old[i + k], r[i + k]); };
             pv = pn(0);
             auto px = [&](int k) { return k == 0 ? pv : pn(k * P); };
             auto py = [&](int k) { return k == 0 ? pv : pn(k * R); };
@@ -167,18 +99,10 @@ __global__ void __launch_bounds__(256) cg_dir_kernel(const double* __restrict__ 
         p[i] = pv;
     });
     s = block_sum(s, red);
-    if (publish_partial(part_sum, s, nullptr, 0.0, st)) {
+    if (publish_slot(part_sum, nullptr, blockIdx.x, s, 0.0, &st->counter, gridDim.x)) {
         double denom, unused;
-        reduce_partials(part_sum, nullptr, red, &denom, &unused);
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            st->counter = 0u;
-            if (!isfinite(denom) || denom <= 0.0) {  // solver.py:1036-1037
-                st->stop = 1;
-                st->reason = SW_CG_DENOM;
-            } else {
-                st->alpha = st->rs_old / denom;
-            }
-        }
+        reduce_slots(part_sum, nullptr, gridDim.x, red, &denom, &unused);
+        if (threadIdx.x == 0 && threadIdx.y == 0) cg_after_dir(st, denom);
     }
 }
 
@@ -207,255 +131,59 @@ __global__ void __launch_bounds__(256) cg_step_kernel(double* __restrict__ phi, 
         }
     });
     s = block_sum(s, red);
-    if (publish_partial(part_sum, s, nullptr, 0.0, st)) {
+    if (publish_slot(part_sum, nullptr, blockIdx.x, s, 0.0, &st->counter, gridDim.x)) {
         double rs_new, unused;
-        reduce_partials(part_sum, nullptr, red, &rs_new, &unused);
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            const double eps = 2.220446049250313e-16;
-            st->counter = 0u;
-            st->iters += 1;
-            if (!isfinite(rs_new)) {  // solver.py:1042-1043
-                st->stop = 1;
-                st->reason = SW_CG_NONFINITE;
-            } else {
-                const double rel = sqrt(rs_new / st->rs0);
-                st->relres = rel;
-                if (rel < st->tol) {  // solver.py:1044-1045
-                    st->stop = 1;
-                    st->reason = SW_CG_CONVERGED;
-                } else {
-                    st->beta = rs_new / fmax(st->rs_old, eps);
-                    st->rs_old = rs_new;
-                }
-            }
-        }
+        reduce_slots(part_sum, nullptr, gridDim.x, red, &rs_new, &unused);
+        if (threadIdx.x == 0 && threadIdx.y == 0) cg_after_step(st, rs_new);
     }
 }
 
-// ---------------------------------------------------------------------------
-// x-marching CG kernels. A (BY x 64) tile of (y, z) columns marches along x;
-// p at planes x-2 .. x+3 of the column sits in a register queue, and plane x
-// (with a two-voxel halo in y and z) is staged in LDS, so r, p_old, p, phi are
-// each read from HBM about once per iteration. The queue's roles rotate over
-// an unrolled loop of six planes, so no prefetched register is moved.
-
-constexpr int kCgBY = 8;   // tile rows in y (block of 512 threads)
-constexpr int kCgLZ = 68;  // 64 + 2 + 2
-
-struct CgTile {
-    int y0, z0, x0, x1, y, z;
-    bool inyz;
-    int64_t yz;
-    bool halo_thread, halo_in;
-    int ly, lz;
-    int64_t hyz;
-};
-
-__device__ inline CgTile cg_tile(const Grid& g, int xchunk, int nty, int ntz) {
-    CgTile t;
-    const int tz = threadIdx.x, ty = threadIdx.y, tid = ty * 64 + tz;
+// Single-solve x-marching CG kernels: one tile per block (tiles z fastest,
+// then y, then x chunk, XCD-aware), one partial per block.
+__device__ inline CgTile cg_block_tile(const Grid& g, int xchunk, int nty, int ntz) {
     const unsigned nb = gridDim.x;
     unsigned b = blockIdx.x;
     if ((nb & 7u) == 0u) b = (b & 7u) * (nb >> 3) + (b >> 3);  // XCD-aware: neighbours share an L2
-    const int tyi = (int)(b % (unsigned)nty);
-    const unsigned rest = b / (unsigned)nty;
-    const int tzi = (int)(rest % (unsigned)ntz);
-    const int txi = (int)(rest / (unsigned)ntz);
-    t.y0 = tyi * kCgBY;
-    t.z0 = tzi * 64;
-    t.x0 = txi * xchunk;
-    t.x1 = min(t.x0 + xchunk, g.nx);
-    t.y = t.y0 + ty;
-    t.z = t.z0 + tz;
-    t.inyz = t.y < g.ny && t.z < g.nz;
-    t.yz = t.inyz ? (int64_t)t.y * g.nz + t.z : 0;
-    // halo: rows y0-2, y0-1, y0+BY, y0+BY+1 (4 x 64 points), columns z0-2, z0-1, z0+64, z0+65 (4 x BY)
-    int hy = -1, hz = -1;
-    t.ly = t.lz = 0;
-    if (tid < 256) {
-        const int rr = tid >> 6;
-        hy = rr < 2 ? t.y0 - 2 + rr : t.y0 + kCgBY + rr - 2;
-        t.ly = rr < 2 ? rr : kCgBY + rr;
-        hz = t.z0 + (tid & 63);
-        t.lz = (tid & 63) + 2;
-    } else if (tid < 256 + 4 * kCgBY) {
-        const int k = tid - 256, cc = k / kCgBY, yy = k % kCgBY;
-        hy = t.y0 + yy;
-        t.ly = yy + 2;
-        hz = cc < 2 ? t.z0 - 2 + cc : t.z0 + 64 + cc - 2;
-        t.lz = cc < 2 ? cc : 64 + cc;
-    }
-    t.halo_thread = tid < 256 + 4 * kCgBY;
-    t.halo_in = t.halo_thread && hy >= 0 && hy < g.ny && hz >= 0 && hz < g.nz;
-    t.hyz = t.halo_in ? (int64_t)hy * g.nz + hz : 0;
-    return t;
+    const int tzi = (int)(b % (unsigned)ntz);                   // z fastest: whole z rows per XCD
+    const unsigned rest = b / (unsigned)ntz;
+    const int tyi = (int)(rest % (unsigned)nty);
+    const int txi = (int)(rest / (unsigned)nty);
+    const int x0 = txi * xchunk;
+    return cg_tile(g, x0, min(x0 + xchunk, g.nx), tyi * kCgBY, tzi * 64);
 }
 
-// A p at an interior voxel: x terms from the queue (a = x-2, c = x, e = x+2),
-// y and z terms from the LDS plane; rows next to the wall drop G on the
-// boundary layer (solver.py:1024-1026 with gradient_center zero there).
-__device__ inline double cg_apply(const Grid& g, int x, int y, int z, double a, double c, double e,
-                                  const double (*L)[kCgLZ], int r, int q, double qx, double qy, double qz) {
-    const double tx = ((x + 1 < g.nx - 1) ? (e - c) : 0.0) - ((x - 1 > 0) ? (c - a) : 0.0);
-    const double ty = ((y + 1 < g.ny - 1) ? (L[r + 2][q] - c) : 0.0) - ((y - 1 > 0) ? (c - L[r - 2][q]) : 0.0);
-    const double tz = ((z + 1 < g.nz - 1) ? (L[r][q + 2] - c) : 0.0) - ((z - 1 > 0) ? (c - L[r][q - 2]) : 0.0);
-    return -(tx * qx + ty * qy + tz * qz);
-}
-
-// p = r + beta p_old (one fma, so every recomputation is bit-identical), A p, p.Ap.
 __global__ void __launch_bounds__(64 * kCgBY) cg_dir_march_kernel(const double* __restrict__ r,
                                                                   const double* __restrict__ p_old,
                                                                   double* __restrict__ p, Grid g, double qx,
                                                                   double qy, double qz, int xchunk, int nty, int ntz,
                                                                   double* part_sum, CgState* st) {
     if (st->stop) return;
-    __shared__ double lds[2][kCgBY + 4][kCgLZ];
+    __shared__ CgLds lds;
     __shared__ double red[16];
-    const double beta = st->beta;
-    const CgTile t = cg_tile(g, xchunk, nty, ntz);
-    const int64_t P = g.plane();
-    const int r0 = threadIdx.y + 2, q0 = threadIdx.x + 2;
-    // voxel 0 is on the boundary layer, where r and p_old are zero
-    auto pn = [&](int64_t o, bool ok) {
-        const int64_t k = ok ? o : 0;
-        return fma(beta, p_old[k], r[k]);
-    };
-    auto center = [&](int xx) { return pn((int64_t)xx * P + t.yz, t.inyz && xx >= 0 && xx < g.nx); };
-    auto halo = [&](int xx) {
-        return t.halo_thread ? pn((int64_t)xx * P + t.hyz, t.halo_in && xx >= 0 && xx < g.nx) : 0.0;
-    };
-    double s = 0.0;
-    auto plane = [&](auto buf, double a, double b, double c, double d, double e, double& f, double hc, double& hn,
-                     int x) {
-        constexpr int B = decltype(buf)::value;
-        (void)b;
-        (void)d;
-        lds[B][r0][q0] = c;
-        if (t.halo_thread) lds[B][t.ly][t.lz] = hc;
-        f = center(x + 3);
-        hn = halo(x + 1);
-        __syncthreads();
-        if (!t.inyz) return;
-        const bool in = g.interior(x, t.y, t.z);
-        if (in) s += c * cg_apply(g, x, t.y, t.z, a, c, e, lds[B], r0, q0, qx, qy, qz);
-        p[(int64_t)x * P + t.yz] = in ? c : 0.0;
-    };
-    using B0 = std::integral_constant<int, 0>;
-    using B1 = std::integral_constant<int, 1>;
-    const int x0 = t.x0, x1 = t.x1;
-    double v0 = center(x0 - 2), v1 = center(x0 - 1), v2 = center(x0), v3 = center(x0 + 1), v4 = center(x0 + 2), v5;
-    double h0 = halo(x0), h1;
-    for (int x = x0; x < x1; x += 6) {
-        plane(B0{}, v0, v1, v2, v3, v4, v5, h0, h1, x);
-        if (x + 1 >= x1) break;
-        plane(B1{}, v1, v2, v3, v4, v5, v0, h1, h0, x + 1);
-        if (x + 2 >= x1) break;
-        plane(B0{}, v2, v3, v4, v5, v0, v1, h0, h1, x + 2);
-        if (x + 3 >= x1) break;
-        plane(B1{}, v3, v4, v5, v0, v1, v2, h1, h0, x + 3);
-        if (x + 4 >= x1) break;
-        plane(B0{}, v4, v5, v0, v1, v2, v3, h0, h1, x + 4);
-        if (x + 5 >= x1) break;
-        plane(B1{}, v5, v0, v1, v2, v3, v4, h1, h0, x + 5);
-    }
+    const CgTile t = cg_block_tile(g, xchunk, nty, ntz);
+    double s = cg_dir_tile(r, p_old, p, g, st->beta, qx, qy, qz, t, lds);
     s = block_sum(s, red);
-    if (publish_partial(part_sum, s, nullptr, 0.0, st)) {
+    if (publish_slot(part_sum, nullptr, blockIdx.x, s, 0.0, &st->counter, gridDim.x)) {
         double denom, unused;
-        reduce_partials(part_sum, nullptr, red, &denom, &unused);
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            st->counter = 0u;
-            if (!isfinite(denom) || denom <= 0.0) {  // solver.py:1036-1037
-                st->stop = 1;
-                st->reason = SW_CG_DENOM;
-            } else {
-                st->alpha = st->rs_old / denom;
-            }
-        }
+        reduce_slots(part_sum, nullptr, gridDim.x, red, &denom, &unused);
+        if (threadIdx.x == 0 && threadIdx.y == 0) cg_after_dir(st, denom);
     }
 }
 
-// phi += alpha p, r -= alpha A p (A p recomputed from p), r.r and the stopping rules.
 __global__ void __launch_bounds__(64 * kCgBY) cg_step_march_kernel(double* __restrict__ phi, double* __restrict__ r,
                                                                    const double* __restrict__ p, Grid g, double qx,
                                                                    double qy, double qz, int xchunk, int nty, int ntz,
                                                                    double* part_sum, CgState* st) {
     if (st->stop) return;
-    __shared__ double lds[2][kCgBY + 4][kCgLZ];
+    __shared__ CgLds lds;
     __shared__ double red[16];
-    const double alpha = st->alpha;
-    const CgTile t = cg_tile(g, xchunk, nty, ntz);
-    const int64_t P = g.plane();
-    const int r0 = threadIdx.y + 2, q0 = threadIdx.x + 2;
-    auto ld = [&](const double* f, int64_t o, bool ok) { return f[ok ? o : 0]; };
-    auto center = [&](int xx) { return ld(p, (int64_t)xx * P + t.yz, t.inyz && xx >= 0 && xx < g.nx); };
-    auto halo = [&](int xx) {
-        return t.halo_thread ? ld(p, (int64_t)xx * P + t.hyz, t.halo_in && xx >= 0 && xx < g.nx) : 0.0;
-    };
-    auto ldr = [&](int xx) { return ld(r, (int64_t)xx * P + t.yz, t.inyz && xx >= 0 && xx < g.nx); };
-    auto ldphi = [&](int xx) { return ld(phi, (int64_t)xx * P + t.yz, t.inyz && xx >= 0 && xx < g.nx); };
-    double s = 0.0;
-    auto plane = [&](auto buf, double a, double b, double c, double d, double e, double& f, double hc, double& hn,
-                     double rc, double& rn, double fc, double& fn, int x) {
-        constexpr int B = decltype(buf)::value;
-        (void)b;
-        (void)d;
-        lds[B][r0][q0] = c;
-        if (t.halo_thread) lds[B][t.ly][t.lz] = hc;
-        f = center(x + 3);
-        hn = halo(x + 1);
-        rn = ldr(x + 1);
-        fn = ldphi(x + 1);
-        __syncthreads();
-        if (!t.inyz || !g.interior(x, t.y, t.z)) return;
-        const double ap = cg_apply(g, x, t.y, t.z, a, c, e, lds[B], r0, q0, qx, qy, qz);
-        const int64_t o = (int64_t)x * P + t.yz;
-        phi[o] = fc + alpha * c;
-        const double rv = rc - alpha * ap;
-        r[o] = rv;
-        s += rv * rv;
-    };
-    using B0 = std::integral_constant<int, 0>;
-    using B1 = std::integral_constant<int, 1>;
-    const int x0 = t.x0, x1 = t.x1;
-    double v0 = center(x0 - 2), v1 = center(x0 - 1), v2 = center(x0), v3 = center(x0 + 1), v4 = center(x0 + 2), v5;
-    double h0 = halo(x0), h1;
-    double ra = ldr(x0), rb, fa = ldphi(x0), fb;
-    for (int x = x0; x < x1; x += 6) {
-        plane(B0{}, v0, v1, v2, v3, v4, v5, h0, h1, ra, rb, fa, fb, x);
-        if (x + 1 >= x1) break;
-        plane(B1{}, v1, v2, v3, v4, v5, v0, h1, h0, rb, ra, fb, fa, x + 1);
-        if (x + 2 >= x1) break;
-        plane(B0{}, v2, v3, v4, v5, v0, v1, h0, h1, ra, rb, fa, fb, x + 2);
-        if (x + 3 >= x1) break;
-        plane(B1{}, v3, v4, v5, v0, v1, v2, h1, h0, rb, ra, fb, fa, x + 3);
-        if (x + 4 >= x1) break;
-        plane(B0{}, v4, v5, v0, v1, v2, v3, h0, h1, ra, rb, fa, fb, x + 4);
-        if (x + 5 >= x1) break;
-        plane(B1{}, v5, v0, v1, v2, v3, v4, h1, h0, rb, ra, fb, fa, x + 5);
-    }
+    const CgTile t = cg_block_tile(g, xchunk, nty, ntz);
+    double s = cg_step_tile(phi, r, p, g, st->alpha, qx, qy, qz, t, lds);
     s = block_sum(s, red);
-    if (publish_partial(part_sum, s, nullptr, 0.0, st)) {
+    if (publish_slot(part_sum, nullptr, blockIdx.x, s, 0.0, &st->counter, gridDim.x)) {
         double rs_new, unused;
-        reduce_partials(part_sum, nullptr, red, &rs_new, &unused);
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            const double eps = 2.220446049250313e-16;
-            st->counter = 0u;
-            st->iters += 1;
-            if (!isfinite(rs_new)) {  // solver.py:1042-1043
-                st->stop = 1;
-                st->reason = SW_CG_NONFINITE;
-            } else {
-                const double rel = sqrt(rs_new / st->rs0);
-                st->relres = rel;
-                if (rel < st->tol) {  // solver.py:1044-1045
-                    st->stop = 1;
-                    st->reason = SW_CG_CONVERGED;
-                } else {
-                    st->beta = rs_new / fmax(st->rs_old, eps);
-                    st->rs_old = rs_new;
-                }
-            }
-        }
+        reduce_slots(part_sum, nullptr, gridDim.x, red, &rs_new, &unused);
+        if (threadIdx.x == 0 && threadIdx.y == 0) cg_after_step(st, rs_new);
     }
 }
 
@@ -598,7 +326,7 @@ int sw_project_vector_field(int dtype, void* fx, void* fy, void* fz, int64_t nx,
     const double i2hx = 1.0 / (2.0 * dx), i2hy = 1.0 / (2.0 * dy), i2hz = 1.0 / (2.0 * dz);
     const double qx = 1.0 / (4.0 * dx * dx), qy = 1.0 / (4.0 * dy * dy), qz = 1.0 / (4.0 * dz * dz);
 
-    hipLaunchKernelGGL(cg_reset_kernel, dim3(1), dim3(1), 0, s, w.st, tol);
+    hipLaunchKernelGGL(cg_reset_kernel, dim3(1), dim3(1), 0, s, w.st, tol, (long long)max_iters);
     SW_LAUNCH_CHECK();
     SW_DISPATCH(dtype, T,
                 hipLaunchKernelGGL(cg_init_kernel<T>, dim3(nb), blk, 0, s, (const T*)fx, (const T*)fy, (const T*)fz,

@@ -95,3 +95,35 @@ def step_bytes(itemsize, form="increment", viscous=True, forced=False):
 
 
 CG_BYTES_PER_ITER = 8 * 8  # dir: r, p_old read, p written; step: phi, r, p read, phi, r written (float64)
+
+
+def batch_48(seed=3, n_solves=48, h=0.5e-3):
+    """The batching workload: 48 independent solves of uneven sizes.
+
+    (nx, ny) ~ U{64..192} and nz ~ U{64..256}, rounded to multiples of 8;
+    h = 0.5 mm; rho = 1000 (1 + U(-0.01, 0.01)); mu = 2 kPa with 3-8 seeded
+    inclusions U(2, 20) kPa; eta U(0.5, 3) Pa s per voxel; nT ~ U{500..2000};
+    a Gaussian push (sigma = 2 mm) along z at a random interior position,
+    active for t < 100 us; float32 storage, CG tol 1e-8; per-solve dt by
+    the stability rule above.
+    """
+    from .batch import SolveSpec
+
+    rng = np.random.default_rng(seed)
+    specs = []
+    for k in range(n_solves):
+        nx, ny = (int(8 * round(rng.integers(64, 193) / 8)) for _ in range(2))
+        nz = int(8 * round(rng.integers(64, 257) / 8))
+        shape = (nx, ny, nz)
+        rho = 1000.0 * (1.0 + rng.uniform(-0.01, 0.01, shape))
+        mu = inclusion_map(shape, h, rng, int(rng.integers(3, 9)), 2000.0, mu_range=(2e3, 20e3))
+        eta = rng.uniform(0.5, 3.0, shape)
+        n_steps = int(rng.integers(500, 2001))
+        dt, dt_cfl = stable_dt(h, float(rho.min()), float(mu.max()), float(eta.max()))
+        centre = [rng.uniform(0.25, 0.75) * (n - 1) * h for n in shape]
+        bz = gaussian_push(shape, h, 2e-3, centre=centre)
+        env = (np.arange(n_steps) * dt < 100e-6).astype(np.float64)
+        specs.append(SolveSpec(shape=shape, h=(h, h, h), dt=dt, rho=rho, mu=mu, eta=eta, force=(None, None, bz),
+                               env=env, n_steps=n_steps, tol=1e-8, max_iters=20000,
+                               meta={"dt_cfl": dt_cfl, "index": k}))
+    return specs

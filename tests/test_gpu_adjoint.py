@@ -1,0 +1,116 @@
+"""The hand-written adjoint: gradients of a sensor-plane misfit w.r.t. mu and eta.
+
+Pinned by mathematics (there is no reference gradient to compare with, and the
+JAX backend cannot run here): the adjoint dot-product identity against central
+differences along a random direction, central differences at single voxels,
+checkpoint-spacing invariance, and float32 against float64.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+SHAPE = (18, 16, 20)
+H = (0.5e-3, 0.5e-3, 0.5e-3)
+N_STEPS = 60
+
+
+def setup(dtype="float64", checkpoint_every=None, seed=0):
+    from shearwave_amd.adjoint import Problem, plane_sensors
+
+    rng = np.random.default_rng(seed)
+    rho = 1000.0 * (1 + rng.uniform(-0.01, 0.01, SHAPE))
+    mu = 3000.0 * (1 + 0.5 * rng.uniform(0, 1, SHAPE))
+    eta = rng.uniform(0.5, 1.5, SHAPE)
+    cmax = math.sqrt(mu.max() * 1.5 / rho.min())
+    dt = min(0.3 * H[0] / (cmax * math.sqrt(3)), 0.4 * rho.min() * H[0] ** 2 / (6 * eta.max() * 1.5))
+    ax = [np.arange(n) - (n - 1) / 2 for n in SHAPE]
+    bz = 300.0 * np.exp(-(ax[0][:, None, None] ** 2 + ax[1][None, :, None] ** 2 + ax[2][None, None, :] ** 2) / 8.0)
+    env = np.zeros(N_STEPS)
+    env[:12] = 1.0
+    prob = Problem(SHAPE, H, dt, rho, (None, None, bz), env, plane_sensors(SHAPE, 5), dtype=dtype, vel_damp=0.01,
+                   tol=1e-12, checkpoint_every=checkpoint_every)
+    dev = torch.device("cuda")
+    mu_t = torch.tensor(mu, dtype=torch.float64, device=dev)
+    eta_t = torch.tensor(eta, dtype=torch.float64, device=dev)
+    # target traces from a perturbed material
+    target, _ = prob.forward(mu_t * (1 + 0.05 * torch.tensor(rng.standard_normal(SHAPE), device=dev)), eta_t,
+                             keep_checkpoints=False)
+    return prob, mu_t, eta_t, target.to(torch.float64)
+
+
+def loss_and_grad(prob, mu, eta, target):
+    from shearwave_amd.adjoint import SensorTraces
+
+    mu = mu.clone().requires_grad_(True)
+    eta = eta.clone().requires_grad_(True)
+    tr = SensorTraces.apply(mu, eta, prob)
+    loss = 0.5 * ((tr.to(torch.float64) - target) ** 2).sum()
+    loss.backward()
+    return float(loss), mu.grad, eta.grad
+
+
+def loss_only(prob, mu, eta, target):
+    tr, _ = prob.forward(mu, eta, keep_checkpoints=False)
+    return float(0.5 * ((tr.to(torch.float64) - target) ** 2).sum())
+
+
+@pytest.fixture(scope="module")
+def p64(sw):
+    return setup("float64")
+
+
+def test_dot_product_identity(p64):
+    prob, mu, eta, target = p64
+    _, gmu, geta = loss_and_grad(prob, mu, eta, target)
+    rng = np.random.default_rng(7)
+    dmu = torch.tensor(rng.standard_normal(SHAPE), device=mu.device) * mu * 1e-4
+    deta = torch.tensor(rng.standard_normal(SHAPE), device=mu.device) * eta * 1e-4
+    jp = loss_only(prob, mu + dmu, eta + deta, target)
+    jm = loss_only(prob, mu - dmu, eta - deta, target)
+    fd = (jp - jm) / 2
+    ad = float((gmu * dmu).sum() + (geta * deta).sum())
+    assert abs(fd - ad) < 1e-6 * abs(ad), (fd, ad)
+
+
+def test_central_differences_at_voxels(p64):
+    prob, mu, eta, target = p64
+    _, gmu, geta = loss_and_grad(prob, mu, eta, target)
+    rng = np.random.default_rng(8)
+    for field in ("mu", "eta"):
+        g = gmu if field == "mu" else geta
+        base = mu if field == "mu" else eta
+        # voxels where the gradient is not negligible
+        flat = torch.argsort(g.abs().flatten(), descending=True)[:200].cpu().numpy()
+        for idx in rng.choice(flat, 3, replace=False):
+            e = torch.zeros(SHAPE, dtype=torch.float64, device=mu.device).flatten()
+            eps = 1e-4 * float(base.flatten()[idx])
+            e[idx] = eps
+            e = e.reshape(SHAPE)
+            if field == "mu":
+                fd = (loss_only(prob, mu + e, eta, target) - loss_only(prob, mu - e, eta, target)) / (2 * eps)
+            else:
+                fd = (loss_only(prob, mu, eta + e, target) - loss_only(prob, mu, eta - e, target)) / (2 * eps)
+            ad = float(g.flatten()[idx])
+            assert abs(fd - ad) < 1e-5 * abs(ad), (field, idx, fd, ad)
+
+
+def test_checkpoint_spacing_invariance(sw, p64):
+    prob, mu, eta, target = p64
+    _, gmu, geta = loss_and_grad(prob, mu, eta, target)
+    prob1, _, _, _ = setup("float64", checkpoint_every=1)
+    _, gmu1, geta1 = loss_and_grad(prob1, mu, eta, target)
+    assert torch.equal(gmu, gmu1) and torch.equal(geta, geta1)
+
+
+def test_fp32_gradient_close_to_fp64(sw, p64):
+    prob, mu, eta, target = p64
+    _, gmu, geta = loss_and_grad(prob, mu, eta, target)
+    prob32, _, _, _ = setup("float32")
+    _, gmu32, geta32 = loss_and_grad(prob32, mu, eta, target)
+    for a, b in ((gmu32, gmu), (geta32, geta)):
+        assert float(torch.linalg.norm(a - b) / torch.linalg.norm(b)) < 1e-3

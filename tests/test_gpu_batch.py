@@ -1,0 +1,115 @@
+"""Batched solves (one persistent launch per step for many uneven solves).
+
+Gates: every solve of a batch within 1e-5 relative L2 of the float64 oracle
+(float32 storage); a solve's result is bit-identical alone and in the mix;
+hipGraph replay equals direct launches bit for bit.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import solver as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def make_spec(shape, seed, n_steps, viscous=True, damp=0.0):
+    from shearwave_amd.batch import SolveSpec
+
+    rng = np.random.default_rng(seed)
+    h = (0.3e-3, 0.3e-3, 0.3e-3)
+    rho = 1000.0 * (1 + rng.uniform(-0.01, 0.01, shape))
+    mu = np.full(shape, 2500.0)
+    ax = [np.arange(n) * h[0] for n in shape]
+    for _ in range(3):
+        r = rng.uniform(0.15, 0.3) * min(shape) * h[0]
+        c = [rng.uniform(r, n * h[0] - r) for n in shape]
+        m = (ax[0][:, None, None] - c[0]) ** 2 + (ax[1][None, :, None] - c[1]) ** 2 + (ax[2][None, None, :] - c[2]) ** 2
+        mu[m < r * r] = rng.uniform(4e3, 15e3)
+    eta = rng.uniform(0.5, 3.0, shape) if viscous else None
+    cmax = math.sqrt(mu.max() / rho.min())
+    dt = 0.3 * h[0] / (cmax * math.sqrt(3))
+    if viscous:
+        dt = min(dt, 0.5 * rho.min() * h[0] ** 2 / (6 * eta.max()))
+    centre = [rng.uniform(0.3, 0.7) * (n - 1) for n in shape]
+    g = np.meshgrid(*(np.arange(n) - cc for n, cc in zip(shape, centre)), indexing="ij")
+    bz = 300.0 * np.exp(-(g[0] ** 2 + g[1] ** 2 + g[2] ** 2) / (2 * 3.0**2))
+    env = np.zeros(n_steps)
+    env[: max(1, n_steps // 5)] = 1.0
+    return SolveSpec(shape=shape, h=h, dt=dt, rho=rho, mu=mu, eta=eta, force=(None, None, bz), env=env,
+                     n_steps=n_steps, vel_damp=damp, tol=1e-10, max_iters=5000)
+
+
+def oracle_of(sp):
+    opts = {"env_t": sp.env, "vel_damp": sp.vel_damp, "poisson_tol": 1e-12, "poisson_max_iters": 5000}
+    if sp.eta is not None:
+        opts["eta"] = sp.eta
+    return O.shear_fdtd(None, None, sp.force[2], sp.rho, sp.mu, *sp.h, sp.dt, sp.n_steps, opts=opts, dtype=np.float64)
+
+
+SHAPES = [((24, 20, 70), 60, True, 0.0), ((30, 18, 40), 45, False, 0.01), ((16, 24, 66), 80, True, 0.02),
+          ((40, 36, 44), 30, True, 0.0), ((20, 9, 33), 55, True, 0.0)]
+
+
+@pytest.fixture(scope="module")
+def specs():
+    return [make_spec(sh, 100 + k, nt, visc, damp) for k, (sh, nt, visc, damp) in enumerate(SHAPES)]
+
+
+@pytest.fixture(scope="module")
+def mixed(sw, specs):
+    from shearwave_amd.batch import Batch
+
+    b = Batch(specs)
+    info = b.project()
+    assert all(i["reason"] == "converged" for i in info), info
+    b.run()
+    return [b.result(k) for k in range(len(specs))]
+
+
+def test_batch_matches_oracle(specs, mixed):
+    for sp, (u, v) in zip(specs, mixed):
+        u_ref, v_ref = oracle_of(sp)
+        assert rel(u, u_ref) < 1e-5, (sp.shape, rel(u, u_ref))
+        assert rel(v, v_ref) < 1e-5, (sp.shape, rel(v, v_ref))
+
+
+def test_alone_equals_in_mix(sw, specs, mixed):
+    from shearwave_amd.batch import Batch
+
+    for k in (0, 3):
+        b = Batch([specs[k]])
+        b.project()
+        b.run()
+        u, v = b.result(0)
+        np.testing.assert_array_equal(u, mixed[k][0])
+        np.testing.assert_array_equal(v, mixed[k][1])
+
+
+def test_graph_replay_equals_direct(sw, specs, mixed):
+    from shearwave_amd.batch import Batch
+
+    b = Batch(specs, graph_steps=7)
+    b.project()
+    b.run()
+    for k in range(len(specs)):
+        u, v = b.result(k)
+        np.testing.assert_array_equal(u, mixed[k][0])
+        np.testing.assert_array_equal(v, mixed[k][1])
+
+
+def test_batch_agrees_with_single_solve_dropin(sw, specs, mixed):
+    k = 2
+    sp = specs[k]
+    opts = {"env_t": sp.env, "vel_damp": sp.vel_damp, "poisson_tol": sp.tol, "poisson_max_iters": sp.max_iters,
+            "eta": sp.eta}
+    u, v = sw.shear_fdtd(None, None, sp.force[2], sp.rho, sp.mu, *sp.h, sp.dt, sp.n_steps, opts)
+    assert rel(u, mixed[k][0]) < 1e-6 and rel(v, mixed[k][1]) < 1e-6
