@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import platform
 import sys
@@ -51,6 +52,11 @@ def parse():
     ap.add_argument("--x-chunk", type=int, default=0)
     ap.add_argument("--no-batch", action="store_true", help="skip the batching comparison")
     ap.add_argument("--batch-solves", type=int, default=48)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
+    ap.add_argument("--serve-rates", default="20,100")
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation run")
+    ap.add_argument("--config1", action="store_true", help="only run the 64^3 CPU-vs-GPU comparison and exit")
     return ap.parse_args()
 
 
@@ -218,6 +224,156 @@ def bench_batch(n_solves=48, check=8):
     }
 
 
+def bench_serve(rates=(20.0, 100.0), n_requests=24, tick_steps=20):
+    """Config 4: Poisson arrivals, continuous batching; solves/s and p50/p99 latency per rate."""
+    import torch
+
+    import shearwave_amd as sw
+    from shearwave_amd.autotune import default_tuner
+    from shearwave_amd.serve import Server, poisson_trace
+
+    tuner = default_tuner()
+    out = {"workload": f"{n_requests} requests per rate, grids U{{64..256}}^3 (multiples of 8), nT U{{300..1500}}, "
+                       "float32, seed 4, tick of 20 steps", "rates": []}
+    plain_rate = None
+    for rate in rates:
+        reqs = poisson_trace(rate, n_requests, seed=4)
+        for r in reqs:
+            tuner.xchunk(r.spec.shape)
+        torch.cuda.synchronize()
+        srv = Server(tick_steps=tick_steps, tuner=tuner)
+        done, summ = srv.run(reqs)
+        summ = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in summ.items()}
+        summ["rate_req_per_s"] = rate
+        out["rates"].append(summ)
+        del srv, done
+        torch.cuda.empty_cache()
+        if plain_rate is None:
+            # the plain drop-in on the same requests, one after another (arrivals ignored: best case)
+            t0 = time.perf_counter()
+            for r in reqs:
+                sp = r.spec
+                opts = {"env_t": sp.env, "eta": sp.eta, "poisson_tol": sp.tol, "poisson_max_iters": sp.max_iters}
+                sw.shear_fdtd(None, None, sp.force[2], sp.rho, sp.mu, *sp.h, sp.dt, sp.n_steps, opts)
+            torch.cuda.synchronize()
+            plain_rate = len(reqs) / (time.perf_counter() - t0)
+    out["plain_dropin_solves_per_s"] = round(plain_rate, 3)
+    best = max(r["solves_per_s"] for r in out["rates"])
+    out["best_served_over_plain"] = round(best / plain_rate, 3)
+    return out
+
+
+def bench_adjoint(n=128, n_steps=800, n_fd=32):
+    """Config 5: gradient of a sensor-plane misfit w.r.t. mu and eta (128^3, nT=800)."""
+    import torch
+
+    from shearwave_amd import workloads as W
+    from shearwave_amd.adjoint import Problem, SensorTraces, plane_sensors
+
+    rng = np.random.default_rng(5)
+    shape = (n, n, n)
+    h = 0.5e-3
+    rho = np.full(shape, 1000.0)
+    mu_bg = np.full(shape, 3000.0)
+    mu_true = W.inclusion_map(shape, h, rng, 3, 3000.0, mu_range=(6e3, 15e3))
+    eta = np.full(shape, 1.0)
+    dt, dt_cfl = W.stable_dt(h, 1000.0, float(mu_true.max()), 1.0)
+    bz = W.gaussian_push(shape, h, 1.5e-3)
+    env = (np.arange(n_steps) * dt < 100e-6).astype(np.float64)
+    sensors = plane_sensors(shape, n // 4)
+    dev = torch.device("cuda")
+    res = {"workload": f"{n}^3, h=0.5 mm, nT={n_steps}, loss = 1/2 sum (u_z - u_z_target)^2 over the y-z plane "
+                       f"x={n // 4} ({n}x{n} sensors) and every step; target mu: 3 seeded inclusions; "
+                       "gradient at the background mu, eta = 1 Pa s"}
+    out = {}
+    for dtype in ("float32", "float64"):
+        prob = Problem(shape, (h,) * 3, dt, rho, (None, None, bz), env, sensors, dtype=dtype, tol=1e-10)
+        mu_t = torch.tensor(mu_bg, device=dev)
+        eta_t = torch.tensor(eta, device=dev)
+        target, _ = prob.forward(torch.tensor(mu_true, device=dev), eta_t, keep_checkpoints=False)
+        target = target.to(torch.float64)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        mu_r = mu_t.clone().requires_grad_(True)
+        eta_r = eta_t.clone().requires_grad_(True)
+        tr = SensorTraces.apply(mu_r, eta_r, prob)
+        loss = 0.5 * ((tr.to(torch.float64) - target) ** 2).sum()
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t0
+        loss.backward()
+        torch.cuda.synchronize()
+        t_all = time.perf_counter() - t0
+        out[dtype] = (prob, mu_t, eta_t, target, mu_r.grad, eta_r.grad)
+        res[dtype] = {"forward_s": round(t_fwd, 3), "forward_plus_adjoint_s": round(t_all, 3),
+                      "checkpoints": math.ceil(n_steps / prob.K), "checkpoint_every": prob.K, "loss": float(loss)}
+    prob, mu_t, eta_t, target, gmu, geta = out["float64"]
+
+    def J(mu, et):
+        tr, _ = prob.forward(mu, et, keep_checkpoints=False)
+        return float(0.5 * ((tr.to(torch.float64) - target) ** 2).sum())
+
+    cand = torch.argsort(gmu.abs().flatten(), descending=True)[: 20 * n_fd].cpu().numpy()
+    picks = rng.choice(cand, n_fd, replace=False)
+    errs = []
+    for idx in picks:
+        e = torch.zeros(shape, dtype=torch.float64, device=dev).flatten()
+        eps = 1e-4 * float(mu_t.flatten()[idx])
+        e[idx] = eps
+        e = e.reshape(shape)
+        fd = (J(mu_t + e, eta_t) - J(mu_t - e, eta_t)) / (2 * eps)
+        ad = float(gmu.flatten()[idx])
+        errs.append(abs(fd - ad) / abs(ad))
+    g32 = out["float32"][4]
+    res["fd_check"] = {"voxels": n_fd, "field": "mu", "dtype": "float64", "max_rel_err": max(errs),
+                       "median_rel_err": float(np.median(errs)), "eps": "1e-4 relative, central"}
+    res["fp32_vs_fp64_grad_rel_l2"] = float(torch.linalg.norm(g32 - gmu) / torch.linalg.norm(gmu))
+    res["jax_gradient_check"] = "not run: jax is not installed and there is no network"
+    del out
+    torch.cuda.empty_cache()
+    return res
+
+
+def config1_cpu_vs_gpu():
+    """Config 1 (the reference's CPU case, 64^3, nT=500) on CPU and GPU: the CPU-baseline leg for it.
+
+    Runs the float64 NumPy oracle in full and the GPU drop-in in float64 and
+    float32 (host arrays in and out), with relative L2 errors against the oracle.
+    """
+    import torch
+
+    import shearwave_amd as sw
+    from oracle import solver as O  # the checker
+    from shearwave_amd import workloads as W
+
+    w = W.plain_64()
+    opts = {"env_t": w["env"], "eta": w["eta"], "poisson_tol": w["tol"], "poisson_max_iters": w["max_iters"]}
+    args = (None, None, w["bz"], w["rho"], w["mu"], w["h"], w["h"], w["h"], w["dt"], w["n_steps"])
+    vox_steps = int(np.prod(w["shape"])) * w["n_steps"]
+    t0 = time.perf_counter()
+    u_ref, v_ref, s_ref = O.shear_fdtd_snapshots(*args, opts, w["snapshot_stride"], dtype=np.float64)
+    t_cpu = time.perf_counter() - t0
+    rec = {"workload": "plain shear_fdtd 64^3, nT=500, eta=1, CG tol 1e-10, snapshot stride 10",
+           "dt": w["dt"], "dt_cfl_unstable": w["dt_cfl"],
+           "cpu_oracle_fp64": {"seconds": round(t_cpu, 3), "gvox_steps_per_s": vox_steps / t_cpu / 1e9,
+                               "cores": 1, "numpy": np.__version__}}
+
+    def rel(a, b):
+        return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+    for dtype in ("float64", "float32"):
+        o = {**opts, "sw_dtype": dtype}
+        sw.shear_fdtd_jax(*args, o, snapshot_stride=w["snapshot_stride"])  # warm
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        u, v, snaps = sw.shear_fdtd_jax(*args, o, snapshot_stride=w["snapshot_stride"])
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        rec[f"gpu_{dtype}"] = {"seconds_host_to_host": round(t, 4), "gvox_steps_per_s": vox_steps / t / 1e9,
+                               "rel_l2_u": rel(u, u_ref), "rel_l2_v": rel(v, v_ref), "rel_l2_snapshots": rel(snaps, s_ref)}
+    return rec
+
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -228,6 +384,9 @@ def main():
 
     from shearwave_amd import workloads as W
 
+    if args.config1:
+        print(json.dumps(config1_cpu_vs_gpu()), flush=True)
+        return
     w = W.dropin_256(n=args.n, n_steps=args.nt)
     item = 4 if args.dtype == "float32" else 8
     vox = int(np.prod(w["shape"]))
@@ -315,6 +474,10 @@ def main():
         del bench
         torch.cuda.empty_cache()
         rec["batch"] = bench_batch(args.batch_solves)
+    if not args.no_serve:
+        rec["serve"] = bench_serve(tuple(float(r) for r in args.serve_rates.split(",")), args.serve_requests)
+    if not args.no_adjoint:
+        rec["adjoint"] = bench_adjoint()
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(w, args.cpu_steps)
     else:

@@ -12,7 +12,8 @@ import ctypes as C
 import os
 import threading
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libshearwave_hip.so")
+LIB_PATH = os.environ.get("SHEARWAVE_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                             "libshearwave_hip.so")
 
 SW_OK = 0
 SW_F32, SW_F64 = 0, 1

@@ -57,16 +57,39 @@ __device__ __forceinline__ Col<T> load_col(const StepArgs<T>& a, int64_t o, bool
     return c;
 }
 
+// Single-use streams (coefficients, force). Built with SW_NT_COEF they are read
+// with the non-temporal hint, so they do not evict halo lines from the L2.
+template <typename T>
+__device__ __forceinline__ T load_once(const T* p) {
+#ifdef SW_NT_COEF
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
+
 template <typename T, bool VISC, bool FORCE>
 __device__ __forceinline__ Coef<T> load_coef(const StepArgs<T>& a, int64_t o, bool ok) {
     const int64_t q = ok ? o : 0;
     Coef<T> k;
-    k.m = a.m[q];
-    k.e = VISC ? a.e[q] : T(0);
-    k.f0 = FORCE ? a.f[0][q] : T(0);
-    k.f1 = FORCE ? a.f[1][q] : T(0);
-    k.f2 = FORCE ? a.f[2][q] : T(0);
+    k.m = load_once(a.m + q);
+    k.e = VISC ? load_once(a.e + q) : T(0);
+    k.f0 = FORCE ? load_once(a.f[0] + q) : T(0);
+    k.f1 = FORCE ? load_once(a.f[1] + q) : T(0);
+    k.f2 = FORCE ? load_once(a.f[2] + q) : T(0);
     return k;
+}
+
+// Output store of the time step. Built with SW_NT_STORES, the next step's
+// state is written with the non-temporal hint (it is re-read one launch later,
+// after the L2 has turned over), leaving the L2 to halo reuse.
+template <typename T>
+__device__ __forceinline__ void store_out(T* p, T v) {
+#ifdef SW_NT_STORES
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
 }
 
 template <typename T, bool VISC>
@@ -171,13 +194,13 @@ __device__ __forceinline__ void march_tile(const StepArgs<T>& a, int x0, int x1,
                                  a.fs * kc.f2);
         if (!in) i0 = i1 = i2 = T(0);
         const int64_t o = (int64_t)x * P + yz;
-        a.Ao[0][o] = in ? c.u0 + i0 : T(0);
-        a.Ao[1][o] = in ? c.u1 + i1 : T(0);
-        a.Ao[2][o] = in ? c.u2 + i2 : T(0);
+        store_out(a.Ao[0] + o, in ? c.u0 + i0 : T(0));
+        store_out(a.Ao[1] + o, in ? c.u1 + i1 : T(0));
+        store_out(a.Ao[2] + o, in ? c.u2 + i2 : T(0));
         if (INCR) {
-            a.Bo[0][o] = i0;
-            a.Bo[1][o] = i1;
-            a.Bo[2][o] = i2;
+            store_out(a.Bo[0] + o, i0);
+            store_out(a.Bo[1] + o, i1);
+            store_out(a.Bo[2] + o, i2);
         }
     };
 

@@ -22,7 +22,7 @@ def rel(a, b):
 def trace():
     from shearwave_amd.serve import poisson_trace
 
-    return poisson_trace(rate=400.0, n_requests=6, seed=11, sizes=(24, 48), steps=(40, 90))
+    return poisson_trace(rate=2000.0, n_requests=6, seed=11, sizes=(24, 48), steps=(40, 90))
 
 
 @pytest.fixture(scope="module")
