@@ -120,9 +120,9 @@ typedef struct sw_fdtd_state {
     void* u_next[3];          /* scratch for u_{n+1}                 (rotated) */
     void* b_next[3];          /* scratch for d_{n+1} (increment form only; rotated) */
     int32_t kernel;           /* SW_KERNEL_* */
-    int32_t tile_y;           /* march kernel: y rows per tile (4 or 8; 0 = auto) */
+    int32_t tile_y;           /* march kernel: y rows per tile (0 = auto) */
     int32_t x_chunk;          /* march kernel: x planes per tile (0 = auto) */
-    int32_t pad;
+    int32_t tile_z;           /* march kernel: z width of a tile, 64 or 256 (0 = auto) */
 } sw_fdtd_state;
 
 /* Optional per-step outputs of sw_fdtd_run. Step indices are global:

@@ -65,7 +65,7 @@ class FdtdState(C.Structure):
         ("kernel", C.c_int32),
         ("tile_y", C.c_int32),
         ("x_chunk", C.c_int32),
-        ("pad", C.c_int32),
+        ("tile_z", C.c_int32),
     ]
 
 

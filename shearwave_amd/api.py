@@ -10,7 +10,7 @@ Build-only options (namespaced ``sw_``):
   * ``sw_dtype``: storage type, ``"float32"`` (default, as the reference) or ``"float64"``;
   * ``sw_form``: ``"increment"`` (default) or ``"leapfrog"`` state storage;
   * ``sw_kernel``: ``"auto"``, ``"march"`` or ``"direct"`` time-step kernel;
-  * ``sw_tile_y``, ``sw_x_chunk``: march-kernel tile shape (0 = automatic);
+  * ``sw_tile_y``, ``sw_tile_z``, ``sw_x_chunk``: march-kernel tile shape (0 = automatic);
   * ``sw_strain_stride``: with ``shear_fdtd(..., return_strain=True)``, the
     four strain invariants every that many steps, computed in-loop.
 """
@@ -169,7 +169,8 @@ def shear_fdtd(  # noqa: PLR0912, PLR0915
 
     solve = E.Solve(shape, (dX, dY, dZ), float(dT), rho_f, mu_f, eta, dtype=dtype, form=form,
                     vel_damp=float(vel_damp), kernel=opts.get("sw_kernel", "auto"),
-                    tile_y=int(opts.get("sw_tile_y", 0)), x_chunk=int(opts.get("sw_x_chunk", 0)))
+                    tile_y=int(opts.get("sw_tile_y", 0)), x_chunk=int(opts.get("sw_x_chunk", 0)),
+                    tile_z=int(opts.get("sw_tile_z", 0)))
     if initial_u is not None or initial_v is not None:
         solve.set_initial(initial_u, initial_v)
 

@@ -16,13 +16,17 @@ import math
 import os
 import threading
 
-CANDIDATES = (16, 24, 32, 48, 64, 96, 128)
+CANDIDATES = (16, 32, 48, 64, 96, 128)
+
+
+def _pow2(v):
+    return 1 << max(0, math.ceil(math.log2(max(int(v), 1))))
 
 
 def grid_class(shape):
-    """Bucket a grid by its x length (the march) and its y/z tile counts."""
+    """Bucket a grid by its x length (the march) and its (y, z) tiles per plane, both to powers of two."""
     nx, ny, nz = (int(s) for s in shape)
-    return (1 << max(0, math.ceil(math.log2(max(nx, 1)))), math.ceil(ny / 32), math.ceil(nz / 64))
+    return (_pow2(nx), _pow2(math.ceil(ny / 8) * math.ceil(nz / 64)))
 
 
 def _cache_path():
@@ -84,41 +88,46 @@ class Tuner:
     def _tune(self, shape):
         timer = self._timer or _time_step_kernel
         cls = grid_class(shape)
-        rep = (min(cls[0], 512), cls[1] * 32, cls[2] * 64)
+        rep = (min(cls[0], 512), 8 * cls[1], 64)  # same x length and tiles per plane
         times = {}
         for xc in CANDIDATES:
             if xc > rep[0]:
                 continue
             times[xc] = float(timer(rep, xc))
         best = min(times, key=times.get)
-        return {"xchunk": best, "shape": list(rep), "us": {str(k): round(v * 1e6, 2) for k, v in times.items()}}
+        return {"xchunk": best, "shape": list(rep), "ps_per_voxel_step": {str(k): round(v * 1e12, 3) for k, v in times.items()}}
 
 
-def _time_step_kernel(shape, xchunk, n_steps=20):
-    """Average time of one batched step on one synthetic solve of `shape` (HIP events)."""
+def _time_step_kernel(shape, xchunk, n_steps=12):
+    """Seconds per voxel-step of the batched step on copies of one `shape` solve.
+
+    Enough copies are batched that the work list is about twice the
+    persistent grid, so the chunk depth is judged as it runs in a full batch
+    (halo and prologue cost against balance), not on a lone solve.
+    """
     import numpy as np
     import torch
 
-    from .batch import Batch, SolveSpec
+    from .batch import STEP_BY, Batch, SolveSpec, step_items
 
-    rng = np.random.default_rng(0)
     h = (0.5e-3,) * 3
-    spec = SolveSpec(shape=shape, h=h, dt=1e-6, rho=1000.0, mu=3000.0, eta=rng.uniform(0.5, 3, shape),
-                     force=(None, None, None), env=np.zeros(n_steps + 5), n_steps=n_steps + 5, xchunk=xchunk)
-    b = Batch([spec])
+    per_copy = math.ceil(shape[0] / xchunk) * math.ceil(shape[1] / STEP_BY) * math.ceil(shape[2] / 64)
+    copies = max(1, min(16, math.ceil(2048 / per_copy)))
+    specs = [SolveSpec(shape=shape, h=h, dt=1e-6, rho=1000.0, mu=3000.0, eta=1.0, force=(None, None, None),
+                       env=np.zeros(n_steps + 3), n_steps=n_steps + 3, xchunk=xchunk) for _ in range(copies)]
+    b = Batch(specs)
     desc = b._desc_array(b.solves)
-    from .batch import step_items
-
     items = step_items(b.solves)
     items_t = torch.from_numpy(items.reshape(-1).copy()).to(desc.device)
-    b._advance(desc, items_t, items.shape[0], 5)
+    b._advance(desc, items_t, items.shape[0], 3)
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
     start.record()
     b._advance(desc, items_t, items.shape[0], n_steps)
     stop.record()
     stop.synchronize()
-    return start.elapsed_time(stop) / 1e3 / n_steps
+    vox = copies * int(np.prod(shape))
+    return start.elapsed_time(stop) / 1e3 / n_steps / vox
 
 
 _DEFAULT = None

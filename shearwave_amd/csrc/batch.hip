@@ -76,14 +76,14 @@ __global__ void __launch_bounds__(64 * BY) batch_step_kernel(const sw_batch_solv
         const bool visc = a.e != nullptr, force = a.fs != 0.f;
         if (visc) {
             if (force)
-                march_tile<float, true, true, true, BY>(a, x0, x1, y0, z0, lds);
+                march_tile<float, true, true, true, BY, 64>(a, x0, x1, y0, z0, lds);
             else
-                march_tile<float, true, true, false, BY>(a, x0, x1, y0, z0, lds);
+                march_tile<float, true, true, false, BY, 64>(a, x0, x1, y0, z0, lds);
         } else {
             if (force)
-                march_tile<float, true, false, true, BY>(a, x0, x1, y0, z0, lds);
+                march_tile<float, true, false, true, BY, 64>(a, x0, x1, y0, z0, lds);
             else
-                march_tile<float, true, false, false, BY>(a, x0, x1, y0, z0, lds);
+                march_tile<float, true, false, false, BY, 64>(a, x0, x1, y0, z0, lds);
         }
         __syncthreads();  // the next item reuses the LDS tile
     }

@@ -71,37 +71,37 @@ __global__ void __launch_bounds__(256) step_direct_kernel(StepArgs<T> a) {
     });
 }
 
-template <typename T, bool INCR, bool VISC, bool FORCE, int BY>
-__global__ void __launch_bounds__(64 * BY) step_march_kernel(StepArgs<T> a, int xchunk, int nty, int ntz) {
-    __shared__ T lds[2][VISC ? 6 : 3][BY + 2][66];
+template <typename T, bool INCR, bool VISC, bool FORCE, int BY, int BZ>
+__global__ void __launch_bounds__(BZ * BY) step_march_kernel(StepArgs<T> a, int xchunk, int nty, int ntz) {
+    __shared__ T lds[2][VISC ? 6 : 3][BY + 2][BZ + 2];
     // XCD-aware tile order: the hardware deals blocks round-robin to the 8 XCDs;
     // renumber so that consecutive tiles (which share halos) land on one XCD.
     const unsigned nb = gridDim.x;
     unsigned b = blockIdx.x;
     if ((nb & 7u) == 0u) b = (b & 7u) * (nb >> 3) + (b >> 3);
-    // z fastest, then y, then x: the tiles an XCD receives form whole z rows
-    // of one x chunk, so z and most y halo lines are fetched once into its L2
+    // z fastest, then y, then x chunk
     const int tzi = (int)(b % (unsigned)ntz);
     const unsigned rest = b / (unsigned)ntz;
     const int tyi = (int)(rest % (unsigned)nty);
     const int txi = (int)(rest / (unsigned)nty);
     const int x0 = txi * xchunk;
-    march_tile<T, INCR, VISC, FORCE, BY>(a, x0, min(x0 + xchunk, a.g.nx), tyi * BY, tzi * 64, lds);
+    march_tile<T, INCR, VISC, FORCE, BY, BZ>(a, x0, min(x0 + xchunk, a.g.nx), tyi * BY, tzi * BZ, lds);
 }
 
 struct MarchShape {
-    int by, xchunk, nty, ntz, ntx;
+    int by, bz, xchunk, nty, ntz, ntx;
     unsigned blocks() const { return (unsigned)nty * ntz * ntx; }
 };
 
-// Launch shape: tile rows `by` (4 or 8), and an x chunk that makes the grid
-// one full wave of resident blocks (occupancy of this instantiation x CUs),
-// unless the caller fixes it.
-MarchShape march_shape(const Grid& g, int by, int x_chunk, int resident_per_cu) {
+// Launch shape: a (bz x by) tile of (y, z) columns and an x chunk that makes
+// the grid one full wave of resident blocks (occupancy of this instantiation
+// x CUs), unless the caller fixes it.
+MarchShape march_shape(const Grid& g, int by, int bz, int x_chunk, int resident_per_cu) {
     MarchShape s;
     s.by = by;
-    s.nty = (g.ny + s.by - 1) / s.by;
-    s.ntz = (g.nz + 63) / 64;
+    s.bz = bz;
+    s.nty = (g.ny + by - 1) / by;
+    s.ntz = (g.nz + bz - 1) / bz;
     if (x_chunk > 0) {
         s.xchunk = x_chunk;
     } else {
@@ -124,27 +124,35 @@ int resident_blocks(K kernel, int threads) {
     return n;
 }
 
+template <typename T, bool INCR, bool VISC, bool FORCE, int BY, int BZ>
+void launch_march(const StepArgs<T>& a, int x_chunk, hipStream_t s) {
+    auto k = step_march_kernel<T, INCR, VISC, FORCE, BY, BZ>;
+    static const int res = resident_blocks(k, BY * BZ);
+    const MarchShape ms = march_shape(a.g, BY, BZ, x_chunk, res);
+    hipLaunchKernelGGL(k, dim3(ms.blocks()), dim3(BZ, BY), 0, s, a, ms.xchunk, ms.nty, ms.ntz);
+}
+
 template <typename T, bool INCR, bool VISC, bool FORCE>
-int launch_step(const StepArgs<T>& a, int kernel, int tile_y, int x_chunk, hipStream_t s) {
+int launch_step(const StepArgs<T>& a, int kernel, int tile_y, int tile_z, int x_chunk, hipStream_t s) {
     const Grid& g = a.g;
     if (kernel == SW_KERNEL_DIRECT) {
         hipLaunchKernelGGL((step_direct_kernel<T, INCR, VISC, FORCE>), dim3(row_grid(g, 0)), row_block(), 0, s, a);
         SW_LAUNCH_CHECK();
         return SW_OK;
     }
-    const int by = tile_y > 0 ? tile_y : (DType<T>::code == SW_F32 ? 8 : 4);
-    if (by == 4) {
-        auto k = step_march_kernel<T, INCR, VISC, FORCE, 4>;
-        static const int res = resident_blocks(k, 256);
-        const MarchShape ms = march_shape(g, 4, x_chunk, res);
-        hipLaunchKernelGGL(k, dim3(ms.blocks()), dim3(64, 4), 0, s, a, ms.xchunk, ms.nty, ms.ntz);
-    } else if (by == 8) {
-        auto k = step_march_kernel<T, INCR, VISC, FORCE, 8>;
-        static const int res = resident_blocks(k, 512);
-        const MarchShape ms = march_shape(g, 8, x_chunk, res);
-        hipLaunchKernelGGL(k, dim3(ms.blocks()), dim3(64, 8), 0, s, a, ms.xchunk, ms.nty, ms.ntz);
+    // default tile: 8 x 64 (fp32) or 8 x 64 (fp64); full-width 256 rows on request
+    const int bz = tile_z > 0 ? tile_z : 64;
+    const int by = tile_y > 0 ? tile_y : (bz == 256 ? (DType<T>::code == SW_F32 ? 4 : 2) : 8);
+    if (bz == 64 && by == 4) {
+        launch_march<T, INCR, VISC, FORCE, 4, 64>(a, x_chunk, s);
+    } else if (bz == 64 && by == 8) {
+        launch_march<T, INCR, VISC, FORCE, 8, 64>(a, x_chunk, s);
+    } else if (bz == 256 && by == 2) {
+        launch_march<T, INCR, VISC, FORCE, 2, 256>(a, x_chunk, s);
+    } else if (bz == 256 && by == 4 && DType<T>::code == SW_F32) {
+        if constexpr (DType<T>::code == SW_F32) launch_march<T, INCR, VISC, FORCE, 4, 256>(a, x_chunk, s);
     } else {
-        sw::set_error("tile_y %d not supported (4 or 8)", by);
+        sw::set_error("tile %d x %d (y x z) not supported: 4x64, 8x64, 2x256, 4x256 (float32)", by, bz);
         return SW_ERR_ARG;
     }
     SW_LAUNCH_CHECK();
@@ -152,10 +160,10 @@ int launch_step(const StepArgs<T>& a, int kernel, int tile_y, int x_chunk, hipSt
 }
 
 template <typename T>
-int launch_step_any(const StepArgs<T>& a, bool incr, bool visc, bool force, int kernel, int ty, int xc,
+int launch_step_any(const StepArgs<T>& a, bool incr, bool visc, bool force, int kernel, int ty, int tz, int xc,
                     hipStream_t s) {
 #define SW_STEP_CASE(I, V, F) \
-    if (incr == I && visc == V && force == F) return launch_step<T, I, V, F>(a, kernel, ty, xc, s);
+    if (incr == I && visc == V && force == F) return launch_step<T, I, V, F>(a, kernel, ty, tz, xc, s);
     SW_STEP_CASE(true, true, true)
     SW_STEP_CASE(true, true, false)
     SW_STEP_CASE(true, false, true)
@@ -204,7 +212,7 @@ int run_steps(sw_fdtd_state* st, int64_t n_steps, const double* env, const sw_fd
             a.Ao[c] = (T*)st->u_next[c];
             a.Bo[c] = (T*)st->b_next[c];
         }
-        int rc = launch_step_any<T>(a, incr, visc, force, kernel, st->tile_y, st->x_chunk, s);
+        int rc = launch_step_any<T>(a, incr, visc, force, kernel, st->tile_y, st->tile_z, st->x_chunk, s);
         if (rc) return rc;
         for (int c = 0; c < 3; ++c) {
             if (incr) {

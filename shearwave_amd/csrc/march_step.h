@@ -57,11 +57,11 @@ __device__ __forceinline__ Col<T> load_col(const StepArgs<T>& a, int64_t o, bool
     return c;
 }
 
-// Single-use streams (coefficients, force). Built with SW_NT_COEF they are read
-// with the non-temporal hint, so they do not evict halo lines from the L2.
+// Single-use streams (coefficients, force) are read with the non-temporal
+// hint, so they do not evict halo lines from the L2 (SW_NO_NT: plain loads).
 template <typename T>
 __device__ __forceinline__ T load_once(const T* p) {
-#ifdef SW_NT_COEF
+#ifndef SW_NO_NT
     return __builtin_nontemporal_load(p);
 #else
     return *p;
@@ -80,12 +80,12 @@ __device__ __forceinline__ Coef<T> load_coef(const StepArgs<T>& a, int64_t o, bo
     return k;
 }
 
-// Output store of the time step. Built with SW_NT_STORES, the next step's
-// state is written with the non-temporal hint (it is re-read one launch later,
-// after the L2 has turned over), leaving the L2 to halo reuse.
+// Output store of the time step: the next step's state is written with the
+// non-temporal hint (it is re-read one launch later, after the L2 has turned
+// over), leaving the L2 to halo reuse. SW_NO_NT builds plain stores.
 template <typename T>
 __device__ __forceinline__ void store_out(T* p, T v) {
-#ifdef SW_NT_STORES
+#ifndef SW_NO_NT
     __builtin_nontemporal_store(v, p);
 #else
     *p = v;
@@ -104,18 +104,18 @@ __device__ __forceinline__ T comp_inc(const StepArgs<T>& a, T u0, T um, T up, T 
     return inc * a.damp;
 }
 
-// One tile of the time step: columns (y0 .. y0+BY-1, z0 .. z0+63), planes
-// x0 .. x1-1. All threads of the (64, BY) block call it together. `lds` holds
-// at least NL = (VISC ? 6 : 3) planes.
-template <typename T, bool INCR, bool VISC, bool FORCE, int BY, int NLD>
+// One tile of the time step: columns (y0 .. y0+BY-1, z0 .. z0+BZ-1), planes
+// x0 .. x1-1; the block is (BZ, BY) threads (BZ a multiple of 64, so a wave
+// never spans two rows) and all of them call this together. `lds` holds at
+// least NL = (VISC ? 6 : 3) planes of (BY+2) x (BZ+2).
+template <typename T, bool INCR, bool VISC, bool FORCE, int BY, int BZ, int NLD>
 __device__ __forceinline__ void march_tile(const StepArgs<T>& a, int x0, int x1, int y0, int z0,
-                                           T (&lds)[2][NLD][BY + 2][66]) {
-    constexpr int BZ = 64;
+                                           T (&lds)[2][NLD][BY + 2][BZ + 2]) {
     constexpr int NL = VISC ? 6 : 3;  // planes staged in LDS: u (3) and, with viscosity, d (3)
     static_assert(NLD >= NL, "LDS too small");
+    static_assert(BZ % 64 == 0 && BY >= 2, "tile shape");
     const Grid g = a.g;
     const int tz = threadIdx.x, ty = threadIdx.y;
-    const int tid = ty * BZ + tz;
 
     const int y = y0 + ty, z = z0 + tz;
     const bool inyz = (y < g.ny) && (z < g.nz);
@@ -123,25 +123,31 @@ __device__ __forceinline__ void march_tile(const StepArgs<T>& a, int x0, int x1,
     const int64_t yz = inyz ? (int64_t)y * g.nz + z : 0;
     const bool yz_interior = inyz && y > 0 && y < g.ny - 1 && z > 0 && z < g.nz - 1;
 
-    // halo points of the tile: two rows of 64 (y0-1, y0+BY) and two columns of BY (z0-1, z0+64)
-    int hy = -1, hz = -1, ly = 0, lz = 0;
-    if (tid < 64) {
-        hy = y0 - 1, hz = z0 + tid, ly = 0, lz = tid + 1;
-    } else if (tid < 128) {
-        hy = y0 + BY, hz = z0 + tid - 64, ly = BY + 1, lz = tid - 63;
-    } else if (tid < 128 + BY) {
-        hy = y0 + tid - 128, hz = z0 - 1, ly = tid - 127, lz = 0;
-    } else if (tid < 128 + 2 * BY) {
-        hy = y0 + tid - 128 - BY, hz = z0 + BZ, ly = tid - 127 - BY, lz = BZ + 1;
-    }
-    const bool halo_thread = tid < 128 + 2 * BY;
-    const bool halo_in = halo_thread && hy >= 0 && hy < g.ny && hz >= 0 && hz < g.nz;
-    const int64_t hyz = halo_in ? (int64_t)hy * g.nz + hz : 0;
+    // Halo of the tile. Row halos (y0-1, y0+BY) are loaded by the first and
+    // last row of threads, at their own z (wave-uniform); column halos
+    // (z0-1, z0+BZ) by the first and last lane of each row. A halo point
+    // outside the grid is read from voxel 0 (on the boundary layer, zero) and
+    // only feeds boundary voxels.
+    const bool row_thread = ty == 0 || ty == BY - 1;
+    const int ry = ty == 0 ? y0 - 1 : y0 + BY;
+    const int rly = ty == 0 ? 0 : BY + 1;
+    const bool row_in = row_thread && ry >= 0 && ry < g.ny && z < g.nz;
+    const int64_t ryz = row_in ? (int64_t)ry * g.nz + z : 0;
+    const bool col_thread = tz == 0 || tz == BZ - 1;
+    const int cz = tz == 0 ? z0 - 1 : z0 + BZ;
+    const int clz = tz == 0 ? 0 : BZ + 1;
+    const bool col_in = col_thread && cz >= 0 && cz < g.nz && y < g.ny;
+    const int64_t cyz = col_in ? (int64_t)y * g.nz + cz : 0;
 
     auto center = [&](int xx) { return load_col<T, INCR>(a, (int64_t)xx * P + yz, inyz && xx >= 0 && xx < g.nx); };
-    auto halo = [&](int xx) {
+    auto row_halo = [&](int xx) {
         Col<T> c{};
-        if (halo_thread) c = load_col<T, INCR>(a, (int64_t)xx * P + hyz, halo_in && xx >= 0 && xx < g.nx);
+        if (row_thread) c = load_col<T, INCR>(a, (int64_t)xx * P + ryz, row_in && xx >= 0 && xx < g.nx);
+        return c;
+    };
+    auto col_halo = [&](int xx) {
+        Col<T> c{};
+        if (col_thread) c = load_col<T, INCR>(a, (int64_t)xx * P + cyz, col_in && xx >= 0 && xx < g.nx);
         return c;
     };
     auto coef = [&](int xx) {
@@ -151,7 +157,7 @@ __device__ __forceinline__ void march_tile(const StepArgs<T>& a, int x0, int x1,
     // One x plane. m, c, p: planes x-1, x, x+1 of this column; n receives x+2.
     // hc: this plane's halo point, hn receives the next one; kc / kn likewise for coefficients.
     auto plane = [&](auto buf, const Col<T>& m, const Col<T>& c, const Col<T>& p, Col<T>& n, const Col<T>& hc,
-                     Col<T>& hn, const Coef<T>& kc, Coef<T>& kn, int x) {
+                     Col<T>& hn, const Col<T>& vc, Col<T>& vn, const Coef<T>& kc, Coef<T>& kn, int x) {
         constexpr int B = decltype(buf)::value;
         auto& L = lds[B];
         L[0][ty + 1][tz + 1] = c.u0;
@@ -162,18 +168,29 @@ __device__ __forceinline__ void march_tile(const StepArgs<T>& a, int x0, int x1,
             L[NL - 2][ty + 1][tz + 1] = c.d1;
             L[NL - 1][ty + 1][tz + 1] = c.d2;
         }
-        if (halo_thread) {
-            L[0][ly][lz] = hc.u0;
-            L[1][ly][lz] = hc.u1;
-            L[2][ly][lz] = hc.u2;
+        if (row_thread) {
+            L[0][rly][tz + 1] = hc.u0;
+            L[1][rly][tz + 1] = hc.u1;
+            L[2][rly][tz + 1] = hc.u2;
             if (VISC) {
-                L[NL - 3][ly][lz] = hc.d0;
-                L[NL - 2][ly][lz] = hc.d1;
-                L[NL - 1][ly][lz] = hc.d2;
+                L[NL - 3][rly][tz + 1] = hc.d0;
+                L[NL - 2][rly][tz + 1] = hc.d1;
+                L[NL - 1][rly][tz + 1] = hc.d2;
+            }
+        }
+        if (col_thread) {
+            L[0][ty + 1][clz] = vc.u0;
+            L[1][ty + 1][clz] = vc.u1;
+            L[2][ty + 1][clz] = vc.u2;
+            if (VISC) {
+                L[NL - 3][ty + 1][clz] = vc.d0;
+                L[NL - 2][ty + 1][clz] = vc.d1;
+                L[NL - 1][ty + 1][clz] = vc.d2;
             }
         }
         n = center(x + 2);
-        hn = halo(x + 1);
+        hn = row_halo(x + 1);
+        vn = col_halo(x + 1);
         kn = coef(x + 1);
         __syncthreads();
         if (!inyz) return;
@@ -207,17 +224,17 @@ __device__ __forceinline__ void march_tile(const StepArgs<T>& a, int x0, int x1,
     using B0 = std::integral_constant<int, 0>;
     using B1 = std::integral_constant<int, 1>;
     Col<T> q0 = center(x0 - 1), q1 = center(x0), q2 = center(x0 + 1), q3;
-    Col<T> h0 = halo(x0), h1;
+    Col<T> h0 = row_halo(x0), h1, c0 = col_halo(x0), c1;
     Coef<T> k0 = coef(x0), k1;
     // unrolled by four so the queue's roles rotate without register moves
     for (int x = x0; x < x1; x += 4) {
-        plane(B0{}, q0, q1, q2, q3, h0, h1, k0, k1, x);
+        plane(B0{}, q0, q1, q2, q3, h0, h1, c0, c1, k0, k1, x);
         if (x + 1 >= x1) break;
-        plane(B1{}, q1, q2, q3, q0, h1, h0, k1, k0, x + 1);
+        plane(B1{}, q1, q2, q3, q0, h1, h0, c1, c0, k1, k0, x + 1);
         if (x + 2 >= x1) break;
-        plane(B0{}, q2, q3, q0, q1, h0, h1, k0, k1, x + 2);
+        plane(B0{}, q2, q3, q0, q1, h0, h1, c0, c1, k0, k1, x + 2);
         if (x + 3 >= x1) break;
-        plane(B1{}, q3, q0, q1, q2, h1, h0, k1, k0, x + 3);
+        plane(B1{}, q3, q0, q1, q2, h1, h0, c1, c0, k1, k0, x + 3);
     }
 }
 

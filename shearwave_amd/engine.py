@@ -126,7 +126,7 @@ class Solve:
     """
 
     def __init__(self, shape, spacing, dt, rho, mu, eta=None, *, dtype="float32", form="increment",
-                 vel_damp=0.0, kernel="auto", tile_y=0, x_chunk=0):
+                 vel_damp=0.0, kernel="auto", tile_y=0, x_chunk=0, tile_z=0):
         self.shape = tuple(int(s) for s in shape)
         self.spacing = tuple(float(h) for h in spacing)
         self.dt = float(dt)
@@ -172,6 +172,7 @@ class Solve:
         s.kernel = _KERNELS[kernel]
         s.tile_y = int(tile_y)
         s.x_chunk = int(x_chunk)
+        s.tile_z = int(tile_z)
         self.state = s
 
     # -------------------------------------------------------------- inputs

@@ -13,9 +13,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def test_grid_class_buckets():
-    assert grid_class((64, 64, 64)) == (64, 2, 1)
-    assert grid_class((65, 64, 64)) == (128, 2, 1)
-    assert grid_class((192, 192, 256)) == (256, 6, 4)
+    assert grid_class((64, 64, 64)) == (64, 8)
+    assert grid_class((65, 64, 64)) == (128, 8)
+    assert grid_class((192, 192, 256)) == (256, 128)
+    assert grid_class((100, 20, 70)) == (128, 8)
 
 
 def test_tuner_picks_fastest_and_caches(tmp_path):
@@ -30,7 +31,7 @@ def test_tuner_picks_fastest_and_caches(tmp_path):
     assert t.xchunk((128, 96, 128)) == 48
     n = len(calls)
     assert n == len([c for c in CANDIDATES if c <= 128])
-    assert t.xchunk((120, 70, 100)) == 48  # same class: no new timing
+    assert t.xchunk((120, 80, 110)) == 48  # same class: no new timing
     assert len(calls) == n
     t2 = Tuner(path=path, timer=fake, arch="gfx950:256")
     assert t2.xchunk((128, 96, 128)) == 48 and len(calls) == n  # reloaded from disk

@@ -214,12 +214,24 @@ def test_fdtd_fp64_matches_oracle(sw, hetero, kernel, form):
     assert rel(v, v_ref) < 1e-10, rel(v, v_ref)
 
 
-@pytest.mark.parametrize("tile_y,x_chunk", [(4, 0), (8, 0), (4, 7), (8, 26)])
-def test_march_tile_shapes_agree(sw, hetero, tile_y, x_chunk):
+@pytest.mark.parametrize("tile_y,tile_z,x_chunk", [(4, 64, 0), (8, 64, 0), (4, 64, 7), (8, 64, 26), (2, 256, 0),
+                                                   (2, 256, 9)])
+def test_march_tile_shapes_agree(sw, hetero, tile_y, tile_z, x_chunk):
     shape, p, u_ref, v_ref = hetero
     u, v = _run_gpu(sw, p, shape, {"sw_dtype": "float64", "sw_kernel": "march", "sw_tile_y": tile_y,
-                                   "sw_x_chunk": x_chunk})
+                                   "sw_tile_z": tile_z, "sw_x_chunk": x_chunk})
     assert rel(u, u_ref) < 1e-10 and rel(v, v_ref) < 1e-10
+
+
+@pytest.mark.parametrize("tile_y,tile_z", [(4, 256), (8, 64)])
+def test_fp32_tiles_bit_identical(sw, tile_y, tile_z):
+    """Every tile shape computes each voxel with the same operations: float32 results are identical."""
+    shape = (20, 19, 300)  # two 256-wide z tiles, partial y tiles
+    p = _problem(shape, seed=17, n_steps=30)
+    base = _run_gpu(sw, p, shape, {"sw_kernel": "march", "sw_tile_y": 4, "sw_tile_z": 64})
+    u, v = _run_gpu(sw, p, shape, {"sw_kernel": "march", "sw_tile_y": tile_y, "sw_tile_z": tile_z})
+    np.testing.assert_array_equal(u, base[0])
+    np.testing.assert_array_equal(v, base[1])
 
 
 @pytest.mark.parametrize("kernel", ["march", "direct"])
