@@ -7,10 +7,11 @@ form), and the four strain invariants every 20 steps into a device-resident
 ring. ``value`` is whole-solve throughput in Gvoxel-steps/s (256^3 * 2000 /
 seconds per solve, projection and strain included).
 
-The JSON line also carries the roofline of the dominant kernel (the
-force-free viscous time step, timed with HIP events on the stream it runs on)
-and a CPU baseline (the NumPy oracle in float64 on a bounded sample of the
-same loop, one core).
+The JSON line also carries the roofline of the dominant kernel (in float32
+the two-step pass, which advances two force-free viscous steps per launch;
+timed with HIP events on the stream it runs on; algorithmic bytes per launch
+= 56 B per voxel-step x 2 steps x voxels) and a CPU baseline (the NumPy
+oracle in float64 on a bounded sample of the same loop, one core).
 
     python bench.py [--steps K] [--warmup W] [--n 256] [--nt 2000] [--no-cpu] [--fp64]
 """
@@ -98,15 +99,19 @@ class DropIn:
         s.run(self.w["n_steps"], self.env, inv_stride=self.w["strain_stride"], inv_out=self.inv)
 
 
-def time_step_kernel(solve, n_launch):
-    """Average duration of the force-free step kernel, HIP events on the launch stream."""
+def time_step_kernel(solve, n_launch, steps_per_launch):
+    """Average duration of one launch of the force-free step kernel (HIP events on the launch stream).
+
+    `steps_per_launch` is 2 for the two-step pass (n_launch launches advance
+    2 n_launch steps), 1 for the single-step kernels.
+    """
     import torch
 
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
     solve.run(10)  # warm
     start.record()
-    solve.run(n_launch)
+    solve.run(n_launch * steps_per_launch)
     stop.record()
     stop.synchronize()
     return start.elapsed_time(stop) / 1e3 / n_launch
@@ -224,7 +229,7 @@ def bench_batch(n_solves=48, check=8):
     }
 
 
-def bench_serve(rates=(20.0, 100.0), n_requests=24, tick_steps=20):
+def bench_serve(rates=(20.0, 100.0), n_requests=24, tick_passes=10):
     """Config 4: Poisson arrivals, continuous batching; solves/s and p50/p99 latency per rate."""
     import torch
 
@@ -234,14 +239,14 @@ def bench_serve(rates=(20.0, 100.0), n_requests=24, tick_steps=20):
 
     tuner = default_tuner()
     out = {"workload": f"{n_requests} requests per rate, grids U{{64..256}}^3 (multiples of 8), nT U{{300..1500}}, "
-                       "float32, seed 4, tick of 20 steps", "rates": []}
+                       f"float32, seed 4, tick of {tick_passes} passes (one or two steps each)", "rates": []}
     plain_rate = None
     for rate in rates:
         reqs = poisson_trace(rate, n_requests, seed=4)
         for r in reqs:
             tuner.xchunk(r.spec.shape)
         torch.cuda.synchronize()
-        srv = Server(tick_steps=tick_steps, tuner=tuner)
+        srv = Server(tick_passes=tick_passes, tuner=tuner)
         done, summ = srv.run(reqs)
         summ = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in summ.items()}
         summ["rate_req_per_s"] = rate
@@ -411,8 +416,12 @@ def main():
     torch.cuda.synchronize()
     t_proj = time.perf_counter() - tp0
 
-    kt = time_step_kernel(s, args.kernel_steps)
-    alg_bytes = W.step_bytes(item, "increment", viscous=True, forced=False) * vox
+    # the dominant kernel: the two-step pass in float32 (auto / march2), else the single step
+    two = args.dtype == "float32" and args.kernel in ("auto", "march2")
+    spl = 2 if two else 1
+    kt = time_step_kernel(s, args.kernel_steps // spl, spl)
+    per_voxel_step = W.step_bytes(item, "increment", viscous=True, forced=False)
+    alg_bytes = per_voxel_step * spl * vox  # algorithmic bytes per launch
     achieved = alg_bytes / kt / 1e9
     traffic = traffic_from_profile() if (args.n == 256 and args.dtype == "float32") else None
 
@@ -446,15 +455,17 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": "step_march_kernel (force-free, viscous)" if args.kernel != "direct" else "step_direct_kernel",
-            "alg_bytes_per_voxel": W.step_bytes(item, "increment", viscous=True, forced=False),
+            "kernel": ("step2_march_kernel (two force-free viscous steps per launch)" if two else
+                       "step_march_kernel (force-free, viscous)" if args.kernel != "direct" else "step_direct_kernel"),
+            "alg_bytes_per_voxel_step": per_voxel_step,
+            "voxel_steps_per_launch": spl * vox,
             "avg_launch_us": round(kt * 1e6, 2),
         },
         "phases": {
             "projection_s": round(t_proj, 4),
             "cg_iters": bench.last_cg["iters"],
             "cg_relres": bench.last_cg["relres"],
-            "loop_gvox_steps_per_s": round(vox / kt / 1e9, 2),
+            "loop_gvox_steps_per_s": round(spl * vox / kt / 1e9, 2),
         },
     }
     if args.fp64:
@@ -465,7 +476,7 @@ def main():
         b64.run()
         torch.cuda.synchronize()
         t64 = time.perf_counter() - t0
-        kt64 = time_step_kernel(b64.solve, args.kernel_steps // 2)
+        kt64 = time_step_kernel(b64.solve, args.kernel_steps // 2, 1)
         ab64 = W.step_bytes(8, "increment", viscous=True, forced=False) * vox
         rec["fp64"] = {"value": round(vox * w["n_steps"] / t64 / 1e9, 3), "ms_per_solve": round(t64 * 1e3, 2),
                        "step_kernel_GBs": round(ab64 / kt64 / 1e9, 1), "frac": round(ab64 / kt64 / 1e9 / HBM_PEAK_GBS, 4)}

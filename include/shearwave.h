@@ -40,7 +40,9 @@ extern "C" {
 /* time-step kernel choice */
 #define SW_KERNEL_AUTO 0
 #define SW_KERNEL_DIRECT 1  /* one voxel per thread, neighbours through the caches */
-#define SW_KERNEL_MARCH 2   /* x-marching LDS tiles with register queues */
+#define SW_KERNEL_MARCH 2   /* x-marching LDS tiles with register queues, one step per pass */
+#define SW_KERNEL_MARCH2 3  /* as MARCH, but two force-free increment-form steps per pass
+                               where no output falls between them (AUTO picks it) */
 
 /* CG stop reasons */
 #define SW_CG_RUNNING 0
@@ -120,7 +122,8 @@ typedef struct sw_fdtd_state {
     void* u_next[3];          /* scratch for u_{n+1}                 (rotated) */
     void* b_next[3];          /* scratch for d_{n+1} (increment form only; rotated) */
     int32_t kernel;           /* SW_KERNEL_* */
-    int32_t tile_y;           /* march kernel: y rows per tile (0 = auto) */
+    int32_t tile_y;           /* march: y rows per tile (0 = auto); MARCH2: rows of the two-step tile
+                                 (float32 8 or 12, float64 4), its single steps use the default */
     int32_t x_chunk;          /* march kernel: x planes per tile (0 = auto) */
     int32_t tile_z;           /* march kernel: z width of a tile, 64 or 256 (0 = auto) */
 } sw_fdtd_state;
@@ -178,12 +181,17 @@ int sw_gather_traces(int dtype, const void* f, const int64_t* idx, int64_t count
 /* ---------------------------------------------------------------- batching */
 
 /* One solve of a batch (float32 storage, increment form), as device memory.
- * A batch advances all its solves with ONE persistent launch per time step:
- * work items are (solve, z tile, y tile, x chunk) quadruples of int32, and a
- * device-side global step counter gives each solve its local step
- * local = gstep - start (active while 0 <= local < n_steps; its state is
- * read from parity local&1 and written to the other parity). No reference
- * counterpart: the reference runs one solve per call (solver.py:27). */
+ * A batch advances all its solves with ONE persistent launch per pass: work
+ * items are (solve, z tile, y tile, x chunk) quadruples of int32 over 8-row,
+ * 64-column tiles, and a device-side global pass counter gives each solve its
+ * pass k = gpass - start (active while 0 <= k < n_passes; its state is read
+ * from parity k&1 and written to the other parity). A pass is one step, or
+ * two force-free steps when plan says so: plan[k] = (first local step << 1)
+ * | (1 if the pass is two steps). plan == NULL means one step per pass
+ * (n_passes = n_steps, local step = k). Each solve's y-z plane (ny*nz) holds
+ * fewer than 2^28 voxels: the kernels address a plane with 32-bit offsets.
+ * No reference counterpart: the reference runs one solve per call
+ * (solver.py:27). */
 typedef struct sw_batch_solve {
     int32_t nx, ny, nz;
     int32_t xchunk;                  /* x planes per work item */
@@ -191,7 +199,7 @@ typedef struct sw_batch_solve {
     float vx, vy, vz;                /* dt / h^2 */
     float damp;                      /* 1 - vel_damp */
     float pad0;
-    int64_t start;                   /* global step of this solve's first step */
+    int64_t start;                   /* global pass of this solve's first pass */
     int64_t n_steps;
     const float* mu_over_rho;
     const float* eta_over_rho;       /* NULL when inviscid */
@@ -199,24 +207,27 @@ typedef struct sw_batch_solve {
     const double* env;               /* device, n_steps values, or NULL */
     float* u[2][3];                  /* u by parity */
     float* d[2][3];                  /* d = u - u_prev by parity */
+    const int32_t* plan;             /* device, n_passes entries, or NULL */
+    int64_t n_passes;
 } sw_batch_solve;
 
-/* Advance the batch by n_steps global steps: per step one persistent launch
- * over all `items` (tile_y = 4 or 8 rows; blocks = 0 picks one resident wave)
- * and one increment of *gstep (device int64). */
-int sw_batch_step(const sw_batch_solve* solves, const int32_t* items, int64_t n_items, int64_t* gstep, int32_t tile_y,
-                  int32_t blocks, int64_t n_steps, void* stream);
+/* Advance the batch by n_passes global passes: per pass one persistent launch
+ * over all `items` (tile_y = 8; blocks = 0 picks one resident wave) and one
+ * increment of *gpass (device int64). */
+int sw_batch_step(const sw_batch_solve* solves, const int32_t* items, int64_t n_items, int64_t* gpass, int32_t tile_y,
+                  int32_t blocks, int64_t n_passes, void* stream);
 
-/* The same n_steps captured once into a hipGraph (no memset nodes), to be
+/* The same n_passes captured once into a hipGraph (no memset nodes), to be
  * replayed with sw_graph_launch; the graph keeps the pointers it was built with. */
 typedef struct sw_graph sw_graph;
-int sw_batch_graph_create(const sw_batch_solve* solves, const int32_t* items, int64_t n_items, int64_t* gstep,
-                          int32_t tile_y, int32_t blocks, int64_t n_steps, sw_graph** graph);
+int sw_batch_graph_create(const sw_batch_solve* solves, const int32_t* items, int64_t n_items, int64_t* gpass,
+                          int32_t tile_y, int32_t blocks, int64_t n_passes, sw_graph** graph);
 int sw_graph_launch(sw_graph* graph, void* stream);
 int sw_graph_destroy(sw_graph* graph);
 
 /* One solve's projection in a batched projection (project_vector_field,
- * solver.py:644-683, for many solves at once), as device memory. */
+ * solver.py:644-683, for many solves at once), as device memory; ny*nz
+ * below 2^28 as for sw_batch_solve. */
 typedef struct sw_batch_cg {
     int32_t nx, ny, nz;
     int32_t xchunk;                  /* x planes per CG work item */
@@ -235,7 +246,7 @@ typedef struct sw_batch_cg {
 int64_t sw_cg_state_bytes(void);
 
 /* Project every solve of the batch; CG items are (solve, z tile, y tile,
- * x chunk) quadruples over 8-row tiles, grouped by solve. `states` is device
+ * x chunk) quadruples over 16-row tiles (64 z columns wide), grouped by solve. `states` is device
  * memory of n_solves * sw_cg_state_bytes() bytes. Runs until every solve has
  * stopped; info (host, n_solves entries) receives each solve's report. */
 int sw_batch_project(const sw_batch_cg* solves, int32_t n_solves, const int32_t* items, int64_t n_items,

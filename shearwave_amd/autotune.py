@@ -1,8 +1,9 @@
 """Autotuner for the batched time-step launch shape.
 
 The batched step kernel marches x-chunks of a (8 x 64) tile of (y, z)
-columns; the chunk depth trades halo re-reads (two planes per chunk) against
-the number of work items available to balance the persistent grid. The tuner
+columns; the chunk depth trades the chunk's prologue and halo re-reads
+(three planes per chunk for a two-step pass) against the number of work
+items available to balance the persistent grid. The tuner
 times the step kernel on a synthetic solve of each grid class with every
 candidate chunk depth, keeps the fastest, and caches the choice per device
 architecture in a JSON file (``$SW_AUTOTUNE_CACHE`` or
@@ -98,8 +99,8 @@ class Tuner:
         return {"xchunk": best, "shape": list(rep), "ps_per_voxel_step": {str(k): round(v * 1e12, 3) for k, v in times.items()}}
 
 
-def _time_step_kernel(shape, xchunk, n_steps=12):
-    """Seconds per voxel-step of the batched step on copies of one `shape` solve.
+def _time_step_kernel(shape, xchunk, n_passes=6):
+    """Seconds per voxel-step of the batched two-step passes on copies of one `shape` solve.
 
     Enough copies are batched that the work list is about twice the
     persistent grid, so the chunk depth is judged as it runs in a full batch
@@ -113,8 +114,9 @@ def _time_step_kernel(shape, xchunk, n_steps=12):
     h = (0.5e-3,) * 3
     per_copy = math.ceil(shape[0] / xchunk) * math.ceil(shape[1] / STEP_BY) * math.ceil(shape[2] / 64)
     copies = max(1, min(16, math.ceil(2048 / per_copy)))
+    n_steps = 2 * (n_passes + 3)  # force-free: every pass is two steps
     specs = [SolveSpec(shape=shape, h=h, dt=1e-6, rho=1000.0, mu=3000.0, eta=1.0, force=(None, None, None),
-                       env=np.zeros(n_steps + 3), n_steps=n_steps + 3, xchunk=xchunk) for _ in range(copies)]
+                       env=np.zeros(n_steps), n_steps=n_steps, xchunk=xchunk) for _ in range(copies)]
     b = Batch(specs)
     desc = b._desc_array(b.solves)
     items = step_items(b.solves)
@@ -123,11 +125,11 @@ def _time_step_kernel(shape, xchunk, n_steps=12):
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
     start.record()
-    b._advance(desc, items_t, items.shape[0], n_steps)
+    b._advance(desc, items_t, items.shape[0], n_passes)
     stop.record()
     stop.synchronize()
     vox = copies * int(np.prod(shape))
-    return start.elapsed_time(stop) / 1e3 / n_steps / vox
+    return start.elapsed_time(stop) / 1e3 / (2 * n_passes) / vox
 
 
 _DEFAULT = None

@@ -1,10 +1,12 @@
-"""Batched solves: many shear_fdtd problems of uneven sizes in one persistent launch per step.
+"""Batched solves: many shear_fdtd problems of uneven sizes in one persistent launch per pass.
 
 :class:`Batch` keeps every solve's fields resident in HBM (float32 storage,
 increment form), projects all their forces with one batched CG
 (``sw_batch_project``), and advances all active solves with one persistent
-kernel launch per time step (``sw_batch_step``), optionally replayed from a
-captured hipGraph (``sw_batch_graph_create``). Work items are (solve, z tile,
+kernel launch per pass (``sw_batch_step``), optionally replayed from a
+captured hipGraph (``sw_batch_graph_create``). A solve's pass is one time
+step, or two force-free steps in one read of its state (the two-step tile),
+as its pass plan (:func:`pass_plan`) says. Work items are (solve, z tile,
 y tile, x chunk) quadruples; when solves finish, the item list is rebuilt so
 finished solves cost nothing. A solve's arithmetic is the single-solve tile
 code with per-solve reductions in a fixed order, so its result does not
@@ -24,8 +26,9 @@ from . import _abi
 from . import engine as E
 from ._abi import check, lib
 
-STEP_BY = 8  # tile rows of the batched time step (512-thread blocks)
-CG_BY = 8  # tile rows of the batched CG (fixed by the kernels)
+STEP_BY = 8  # output rows of a batched step tile (blocks of 64 x 11 threads)
+MAX_PLANE = 1 << 28  # the kernels address a y-z plane with 32-bit offsets
+CG_BY = 16  # tile rows of the batched CG (fixed by the kernels)
 
 
 @dataclass
@@ -55,10 +58,14 @@ class SolveSpec:
 class _Solve:
     """Device buffers of one solve in a batch."""
 
-    def __init__(self, spec: SolveSpec):
+    def __init__(self, spec: SolveSpec, two_step=True):
         dev = E.device()
         self.spec = spec
         shape = tuple(int(s) for s in spec.shape)
+        if len(shape) != 3 or min(shape) < 1:
+            raise ValueError(f"solve shape {shape} must be three positive sizes")
+        if shape[1] * shape[2] >= MAX_PLANE:
+            raise ValueError(f"solve shape {shape}: the y-z plane must hold fewer than {MAX_PLANE} voxels")
         self.shape = shape
         st = E.stream_ptr()
         rho64 = E.to_device(spec.rho, "float64", shape)
@@ -85,11 +92,33 @@ class _Solve:
         if env.shape[0] != spec.n_steps:
             raise ValueError("env length must equal n_steps")
         self.env = torch.from_numpy(np.ascontiguousarray(env)).to(dev)
-        self.start = 0
+        plan = pass_plan(env, spec.n_steps, two_step)
+        self.n_passes = int(plan.shape[0])
+        self.plan = torch.from_numpy(plan).to(dev)
+        self.start = 0  # global pass of the first pass
 
     def footprint(self):
         n = int(np.prod(self.shape))
         return n * 4 * (3 + 12 + 6) + self.spec.n_steps * 8
+
+
+def pass_plan(env, n_steps, two_step=True):
+    """Passes covering steps 0 .. n_steps-1: entry (first step << 1) | 1 for a two-step pass.
+
+    Two consecutive steps share a pass when neither applies the force
+    (env == 0); forced steps, and a lone last step, are single passes.
+    """
+    env = np.zeros(n_steps) if env is None else np.asarray(env, dtype=np.float64)
+    plan = []
+    t = 0
+    while t < n_steps:
+        if two_step and t + 1 < n_steps and env[t] == 0.0 and env[t + 1] == 0.0:
+            plan.append((t << 1) | 1)
+            t += 2
+        else:
+            plan.append(t << 1)
+            t += 1
+    return np.asarray(plan, dtype=np.int32)
 
 
 def solve_bytes(spec: SolveSpec, with_cg=True) -> int:
@@ -122,16 +151,16 @@ def default_xchunk(shape):
 class Batch:
     """A set of solves advanced together on one GPU (float32 storage, increment form)."""
 
-    def __init__(self, specs, *, tile_y=STEP_BY, tuner=None, graph_steps=0):
+    def __init__(self, specs, *, tile_y=STEP_BY, tuner=None, graph_steps=0, two_step=True):
         self.specs = list(specs)
         self.tile_y = int(tile_y)
-        self.graph_steps = int(graph_steps)
+        self.graph_steps = int(graph_steps)  # passes per captured graph (0: no graph)
         self.solves = []
         for sp in self.specs:
-            s = _Solve(sp)
+            s = _Solve(sp, two_step)
             s.xchunk = int(sp.xchunk) if sp.xchunk else (tuner.xchunk(sp.shape) if tuner else default_xchunk(sp.shape))
             self.solves.append(s)
-        self.gstep = torch.zeros(1, dtype=torch.int64, device=E.device())
+        self.gpass = torch.zeros(1, dtype=torch.int64, device=E.device())
         self.cg_info = None
 
     # --------------------------------------------------------- descriptors
@@ -157,6 +186,8 @@ class Batch:
                     d.u[par][c] = s.u[par, c].data_ptr()
                     d.d[par][c] = s.d[par, c].data_ptr()
             d.env = s.env.data_ptr()
+            d.plan = s.plan.data_ptr()
+            d.n_passes = s.n_passes
         raw = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8).to(E.device())
         return raw
 
@@ -208,37 +239,37 @@ class Batch:
         return self.cg_info
 
     # ---------------------------------------------------------------- loop
-    def _advance(self, desc, items_t, n_items, n_steps):
-        if n_steps <= 0:
+    def _advance(self, desc, items_t, n_items, n_passes):
+        if n_passes <= 0:
             return
-        if self.graph_steps and n_steps >= self.graph_steps:
+        if self.graph_steps and n_passes >= self.graph_steps:
             g = C.c_void_p()
-            check(lib().sw_batch_graph_create(desc.data_ptr(), items_t.data_ptr(), n_items, self.gstep.data_ptr(),
+            check(lib().sw_batch_graph_create(desc.data_ptr(), items_t.data_ptr(), n_items, self.gpass.data_ptr(),
                                               self.tile_y, 0, self.graph_steps, C.byref(g)))
             try:
-                reps = n_steps // self.graph_steps
+                reps = n_passes // self.graph_steps
                 for _ in range(reps):
                     check(lib().sw_graph_launch(g, E.stream_ptr()))
-                rest = n_steps - reps * self.graph_steps
+                rest = n_passes - reps * self.graph_steps
             finally:
                 torch.cuda.current_stream().synchronize()
                 lib().sw_graph_destroy(g)
         else:
-            rest = n_steps
+            rest = n_passes
         if rest:
-            check(lib().sw_batch_step(desc.data_ptr(), items_t.data_ptr(), n_items, self.gstep.data_ptr(), self.tile_y,
+            check(lib().sw_batch_step(desc.data_ptr(), items_t.data_ptr(), n_items, self.gpass.data_ptr(), self.tile_y,
                                       0, rest, E.stream_ptr()))
 
     def run(self):
         """All solves start together; the item list shrinks as solves finish."""
         for s in self.solves:
             s.start = 0
-        self.gstep.zero_()
+        self.gpass.zero_()
         desc = self._desc_array(self.solves)
-        ends = sorted(set(s.spec.n_steps for s in self.solves))
+        ends = sorted(set(s.n_passes for s in self.solves))
         done = 0
         for end in ends:
-            active = [k for k, s in enumerate(self.solves) if s.spec.n_steps > done]
+            active = [k for k, s in enumerate(self.solves) if s.n_passes > done]
             items = step_items([self.solves[k] for k in active], self.tile_y)
             # map local solve index back to the full descriptor array
             items[:, 0] = np.asarray(active, dtype=np.int32)[items[:, 0]]
@@ -251,7 +282,7 @@ class Batch:
     def state(self, k):
         """(u, d) of solve k after its n_steps steps, as (3, nx, ny, nz) device tensors."""
         s = self.solves[k]
-        par = s.spec.n_steps & 1
+        par = s.n_passes & 1
         return s.u[par], s.d[par]
 
     def result(self, k, to_host=True):

@@ -1,6 +1,7 @@
 // Batched solves on gfx950: many shear_fdtd solves of uneven grid sizes and
-// material maps advance in ONE persistent launch per time step, and their
-// projections run as one batched CG.
+// material maps advance in ONE persistent launch per pass (one step, or two
+// force-free steps, per solve as its pass plan says), and their projections
+// run as one batched CG.
 //
 // Work items are (solve, z tile, y tile, x chunk) quadruples. A persistent
 // grid (one wave of resident blocks) walks the item list; each XCD takes a
@@ -12,6 +13,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "march2_step.h"
 #include "march_cg.h"
 #include "march_step.h"
 #include "sw_common.h"
@@ -42,19 +44,38 @@ __device__ inline ItemWalk item_walk(int64_t n_items) {
     return w;
 }
 
-template <int BY>
-__global__ void __launch_bounds__(64 * BY) batch_step_kernel(const sw_batch_solve* __restrict__ solves,
-                                                             const int4* __restrict__ items, int64_t n_items,
-                                                             const long long* __restrict__ gstep) {
-    __shared__ float lds[2][6][BY + 2][66];
-    const long long gs = *gstep;
+// Tile rows of the batched step: 8 output rows; the block has 8 + 3 rows of
+// threads so that a two-step pass (march2_step.h) fits, and single steps leave
+// the three extra rows to the barriers.
+constexpr int kBatchBY = 8;
+constexpr int kBatchRows = kBatchBY + 3;
+
+// LDS of one block: the two-step tile's planes, or the single-step tile's.
+union BatchLds {
+    Step2Lds<float, true, kBatchBY> two_visc;
+    Step2Lds<float, false, kBatchBY> two_inviscid;
+    float one[2][6][kBatchBY + 2][66];
+};
+
+__global__ void __launch_bounds__(64 * kBatchRows) batch_step_kernel(const sw_batch_solve* __restrict__ solves,
+                                                                     const int4* __restrict__ items, int64_t n_items,
+                                                                     const long long* __restrict__ gpass) {
+    __shared__ BatchLds lds;
+    const long long gp = *gpass;
     const ItemWalk w = item_walk(n_items);
     for (int64_t k = w.first; k < w.hi; k += w.stride) {
         const int4 it = items[k];
         const sw_batch_solve& S = solves[it.x];
-        const long long local = gs - S.start;
-        if (local < 0 || local >= S.n_steps) continue;  // uniform in the block
-        const int par = (int)(local & 1);
+        const long long pass = gp - S.start;
+        if (pass < 0 || pass >= S.n_passes) continue;  // uniform in the block
+        long long local = pass;
+        bool two = false;
+        if (S.plan) {
+            const int e = S.plan[pass];
+            local = e >> 1;
+            two = e & 1;
+        }
+        const int par = (int)(pass & 1);
         StepArgs<float> a;
         for (int c = 0; c < 3; ++c) {
             a.A[c] = S.u[par][c];
@@ -69,27 +90,33 @@ __global__ void __launch_bounds__(64 * BY) batch_step_kernel(const sw_batch_solv
         a.cx = S.cx, a.cy = S.cy, a.cz = S.cz;
         a.vx = S.vx, a.vy = S.vy, a.vz = S.vz;
         a.damp = S.damp;
-        a.fs = (S.env && S.force[0]) ? (float)S.env[local] : 0.f;
+        a.fs = (!two && S.env && S.force[0]) ? (float)S.env[local] : 0.f;
         const int x0 = it.w * S.xchunk;
         const int x1 = min(x0 + S.xchunk, S.nx);
-        const int y0 = it.z * BY, z0 = it.y * 64;
+        const int y0 = it.z * kBatchBY, z0 = it.y * 64;
         const bool visc = a.e != nullptr, force = a.fs != 0.f;
-        if (visc) {
-            if (force)
-                march_tile<float, true, true, true, BY, 64>(a, x0, x1, y0, z0, lds);
+        constexpr int B = kBatchBY, R = kBatchRows;
+        if (two) {  // the plan pairs force-free steps only
+            if (visc)
+                march2_tile<float, true, B>(a, x0, x1, y0, z0, lds.two_visc);
             else
-                march_tile<float, true, true, false, BY, 64>(a, x0, x1, y0, z0, lds);
+                march2_tile<float, false, B>(a, x0, x1, y0, z0, lds.two_inviscid);
+        } else if (visc) {
+            if (force)
+                march_tile<float, true, true, true, B, 64, 6, R>(a, x0, x1, y0, z0, lds.one);
+            else
+                march_tile<float, true, true, false, B, 64, 6, R>(a, x0, x1, y0, z0, lds.one);
         } else {
             if (force)
-                march_tile<float, true, false, true, BY, 64>(a, x0, x1, y0, z0, lds);
+                march_tile<float, true, false, true, B, 64, 6, R>(a, x0, x1, y0, z0, lds.one);
             else
-                march_tile<float, true, false, false, BY, 64>(a, x0, x1, y0, z0, lds);
+                march_tile<float, true, false, false, B, 64, 6, R>(a, x0, x1, y0, z0, lds.one);
         }
-        __syncthreads();  // the next item reuses the LDS tile
+        __syncthreads();  // the next item reuses the LDS
     }
 }
 
-__global__ void advance_kernel(long long* gstep) { *gstep += 1; }
+__global__ void advance_kernel(long long* counter) { *counter += 1; }
 
 // ------------------------------------------------------------ batched CG
 
@@ -203,32 +230,22 @@ __global__ void __launch_bounds__(64 * kCgBY) batch_cg_grad_sub_kernel(const sw_
     }
 }
 
-template <int BY>
-unsigned persistent_blocks(int requested) {
+unsigned persistent_step_blocks(int requested) {
     if (requested > 0) return (unsigned)requested;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_step_kernel<BY>, 64 * BY, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_step_kernel, 64 * kBatchRows, 0) != hipSuccess ||
         per_cu < 1)
         per_cu = 1;
     return (unsigned)(per_cu * device_cu_count());  // a multiple of 8 on an 8-XCD device
 }
 
-int launch_batch_steps(const sw_batch_solve* solves, const int32_t* items, int64_t n_items, int64_t* gstep,
-                       int tile_y, int blocks, int64_t n_steps, hipStream_t s) {
-    if (tile_y == 4) {
-        const unsigned nb = persistent_blocks<4>(blocks);
-        for (int64_t k = 0; k < n_steps; ++k) {
-            hipLaunchKernelGGL(batch_step_kernel<4>, dim3(nb), dim3(64, 4), 0, s, solves, (const int4*)items, n_items,
-                               (const long long*)gstep);
-            hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(1), 0, s, (long long*)gstep);
-        }
-    } else {
-        const unsigned nb = persistent_blocks<8>(blocks);
-        for (int64_t k = 0; k < n_steps; ++k) {
-            hipLaunchKernelGGL(batch_step_kernel<8>, dim3(nb), dim3(64, 8), 0, s, solves, (const int4*)items, n_items,
-                               (const long long*)gstep);
-            hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(1), 0, s, (long long*)gstep);
-        }
+int launch_batch_passes(const sw_batch_solve* solves, const int32_t* items, int64_t n_items, int64_t* gpass,
+                        int blocks, int64_t n_passes, hipStream_t s) {
+    const unsigned nb = persistent_step_blocks(blocks);
+    for (int64_t k = 0; k < n_passes; ++k) {
+        hipLaunchKernelGGL(batch_step_kernel, dim3(nb), dim3(64, kBatchRows), 0, s, solves, (const int4*)items, n_items,
+                           (const long long*)gpass);
+        hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(1), 0, s, (long long*)gpass);
     }
     SW_LAUNCH_CHECK();
     return SW_OK;
@@ -246,20 +263,20 @@ using namespace sw;
 
 extern "C" {
 
-int sw_batch_step(const sw_batch_solve* solves, const int32_t* items, int64_t n_items, int64_t* gstep, int32_t tile_y,
-                  int32_t blocks, int64_t n_steps, void* stream) {
-    SW_REQUIRE(solves && gstep && (items || n_items == 0), SW_ERR_ARG, "null pointer");
-    SW_REQUIRE(tile_y == 4 || tile_y == 8, SW_ERR_ARG, "tile_y %d must be 4 or 8", tile_y);
-    SW_REQUIRE(n_items >= 0 && n_steps >= 0 && blocks >= 0, SW_ERR_ARG, "negative size");
-    return launch_batch_steps(solves, items, n_items, gstep, tile_y, blocks, n_steps, (hipStream_t)stream);
+int sw_batch_step(const sw_batch_solve* solves, const int32_t* items, int64_t n_items, int64_t* gpass, int32_t tile_y,
+                  int32_t blocks, int64_t n_passes, void* stream) {
+    SW_REQUIRE(solves && gpass && (items || n_items == 0), SW_ERR_ARG, "null pointer");
+    SW_REQUIRE(tile_y == kBatchBY, SW_ERR_ARG, "tile_y %d must be %d", tile_y, kBatchBY);
+    SW_REQUIRE(n_items >= 0 && n_passes >= 0 && blocks >= 0, SW_ERR_ARG, "negative size");
+    return launch_batch_passes(solves, items, n_items, gpass, blocks, n_passes, (hipStream_t)stream);
 }
 
-int sw_batch_graph_create(const sw_batch_solve* solves, const int32_t* items, int64_t n_items, int64_t* gstep,
-                          int32_t tile_y, int32_t blocks, int64_t n_steps, sw_graph** graph) {
+int sw_batch_graph_create(const sw_batch_solve* solves, const int32_t* items, int64_t n_items, int64_t* gpass,
+                          int32_t tile_y, int32_t blocks, int64_t n_passes, sw_graph** graph) {
     SW_REQUIRE(graph, SW_ERR_ARG, "null graph handle");
-    SW_REQUIRE(solves && gstep && (items || n_items == 0), SW_ERR_ARG, "null pointer");
-    SW_REQUIRE(tile_y == 4 || tile_y == 8, SW_ERR_ARG, "tile_y %d must be 4 or 8", tile_y);
-    SW_REQUIRE(n_items >= 0 && n_steps >= 1 && blocks >= 0, SW_ERR_ARG, "bad size");
+    SW_REQUIRE(solves && gpass && (items || n_items == 0), SW_ERR_ARG, "null pointer");
+    SW_REQUIRE(tile_y == kBatchBY, SW_ERR_ARG, "tile_y %d must be %d", tile_y, kBatchBY);
+    SW_REQUIRE(n_items >= 0 && n_passes >= 1 && blocks >= 0, SW_ERR_ARG, "bad size");
     *graph = nullptr;
     hipStream_t cap;
     SW_TRY_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
@@ -268,7 +285,7 @@ int sw_batch_graph_create(const sw_batch_solve* solves, const int32_t* items, in
         hipStreamDestroy(cap);
         SW_TRY_HIP(e);
     }
-    const int rc = launch_batch_steps(solves, items, n_items, gstep, tile_y, blocks, n_steps, cap);
+    const int rc = launch_batch_passes(solves, items, n_items, gpass, blocks, n_passes, cap);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(cap, &g);
     hipStreamDestroy(cap);

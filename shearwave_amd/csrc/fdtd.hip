@@ -28,6 +28,7 @@
 #include <type_traits>
 #include <utility>
 
+#include "march2_step.h"
 #include "march_step.h"
 #include "sw_common.h"
 
@@ -86,6 +87,20 @@ __global__ void __launch_bounds__(BZ * BY) step_march_kernel(StepArgs<T> a, int 
     const int txi = (int)(rest / (unsigned)nty);
     const int x0 = txi * xchunk;
     march_tile<T, INCR, VISC, FORCE, BY, BZ>(a, x0, min(x0 + xchunk, a.g.nx), tyi * BY, tzi * BZ, lds);
+}
+
+template <typename T, bool VISC, int BY>
+__global__ void __launch_bounds__(64 * (BY + 3)) step2_march_kernel(StepArgs<T> a, int xchunk, int nty, int ntz) {
+    __shared__ Step2Lds<T, VISC, BY> lds;
+    const unsigned nb = gridDim.x;
+    unsigned b = blockIdx.x;
+    if ((nb & 7u) == 0u) b = (b & 7u) * (nb >> 3) + (b >> 3);  // XCD-aware, as step_march_kernel
+    const int tzi = (int)(b % (unsigned)ntz);
+    const unsigned rest = b / (unsigned)ntz;
+    const int tyi = (int)(rest % (unsigned)nty);
+    const int txi = (int)(rest / (unsigned)nty);
+    const int x0 = txi * xchunk;
+    march2_tile<T, VISC, BY>(a, x0, min(x0 + xchunk, a.g.nx), tyi * BY, tzi * 64, lds);
 }
 
 struct MarchShape {
@@ -159,6 +174,45 @@ int launch_step(const StepArgs<T>& a, int kernel, int tile_y, int tile_z, int x_
     return SW_OK;
 }
 
+template <typename T, bool VISC, int BY>
+void launch_march2(const StepArgs<T>& a, int x_chunk, hipStream_t s) {
+    auto k = step2_march_kernel<T, VISC, BY>;
+    static const int res = resident_blocks(k, 64 * (BY + 3));
+    const MarchShape ms = march_shape(a.g, BY, 64, x_chunk, res);
+    hipLaunchKernelGGL(k, dim3(ms.blocks()), dim3(64, BY + 3), 0, s, a, ms.xchunk, ms.nty, ms.ntz);
+}
+
+// Two force-free increment-form steps in one pass (u_n, d_n -> u_{n+2}, d_{n+2}).
+// Tile rows: float32 8 (default) or 12; float64 4 (a 7-wave block leaves each
+// wave the registers the float64 queues need).
+template <typename T>
+int launch_step2(const StepArgs<T>& a, bool visc, int tile_y, int x_chunk, hipStream_t s) {
+    if constexpr (DType<T>::code == SW_F32) {
+        const int by = tile_y > 0 ? tile_y : 8;
+        if (by == 8) {
+            if (visc) launch_march2<T, true, 8>(a, x_chunk, s);
+            else launch_march2<T, false, 8>(a, x_chunk, s);
+        } else if (by == 12) {
+            if (visc) launch_march2<T, true, 12>(a, x_chunk, s);
+            else launch_march2<T, false, 12>(a, x_chunk, s);
+        } else {
+            sw::set_error("two-step pass: float32 tile_y %d not supported (8 or 12)", by);
+            return SW_ERR_ARG;
+        }
+    } else {
+        const int by = tile_y > 0 ? tile_y : 4;
+        if (by == 4) {
+            if (visc) launch_march2<T, true, 4>(a, x_chunk, s);
+            else launch_march2<T, false, 4>(a, x_chunk, s);
+        } else {
+            sw::set_error("two-step pass: float64 tile_y %d not supported (4)", by);
+            return SW_ERR_ARG;
+        }
+    }
+    SW_LAUNCH_CHECK();
+    return SW_OK;
+}
+
 template <typename T>
 int launch_step_any(const StepArgs<T>& a, bool incr, bool visc, bool force, int kernel, int ty, int tz, int xc,
                     hipStream_t s) {
@@ -184,8 +238,24 @@ int run_steps(sw_fdtd_state* st, int64_t n_steps, const double* env, const sw_fd
     const bool visc = st->eta_over_rho != nullptr;
     const bool have_force = st->force[0] && st->force[1] && st->force[2];
     int kernel = st->kernel;
-    if (kernel == SW_KERNEL_AUTO) kernel = (g.nz >= 32 && g.ny >= 8 && g.nx >= 8) ? SW_KERNEL_MARCH : SW_KERNEL_DIRECT;
+    const bool march_fits = g.nz >= 32 && g.ny >= 8 && g.nx >= 8 && g.plane() < kMaxMarchPlane;
+    if (kernel == SW_KERNEL_AUTO) kernel = march_fits ? SW_KERNEL_MARCH2 : SW_KERNEL_DIRECT;
+    SW_REQUIRE(kernel == SW_KERNEL_DIRECT || g.plane() < kMaxMarchPlane, SW_ERR_ARG,
+               "march kernels need a y-z plane of fewer than %lld voxels", (long long)kMaxMarchPlane);
+    // two-step passes (increment form, force-free, no output in between); the
+    // remaining single steps of such a run use the one-step march kernel
+    const bool pairs = kernel == SW_KERNEL_MARCH2 && incr;
+    const int one_kernel = kernel == SW_KERNEL_MARCH2 ? SW_KERNEL_MARCH : kernel;
+    // tile_y names the two-step tile in a two-step run; its single steps use the default tile
+    const int one_tile_y = kernel == SW_KERNEL_MARCH2 ? 0 : st->tile_y;
     const int64_t step0 = out ? out->step0 : 0;
+    // whether an output reads the state after global step `step`
+    auto output_after = [&](int64_t step) {
+        if (!out) return false;
+        if (out->snap_stride > 0 && (step + 1) % out->snap_stride == 0) return true;
+        if (out->inv_stride > 0 && (step + 1) % out->inv_stride == 0) return true;
+        return out->n_traces > 0;
+    };
 
     StepArgs<T> a{};
     a.m = (const T*)st->mu_over_rho;
@@ -201,10 +271,12 @@ int run_steps(sw_fdtd_state* st, int64_t n_steps, const double* env, const sw_fd
     a.vz = (T)(dt / (st->dz * st->dz));
     a.damp = (T)(1.0 - st->vel_damp);
 
-    for (int64_t it = 0; it < n_steps; ++it) {
+    for (int64_t it = 0; it < n_steps;) {
         const double fs = env ? env[it] : 0.0;
         const bool force = fs != 0.0;
         SW_REQUIRE(!force || have_force, SW_ERR_ARG, "env[%lld] != 0 but the state has no force", (long long)it);
+        const bool two = pairs && !force && it + 1 < n_steps && (!env || env[it + 1] == 0.0) &&
+                         !output_after(step0 + it);
         a.fs = (T)fs;
         for (int c = 0; c < 3; ++c) {
             a.A[c] = (const T*)st->u[c];
@@ -212,7 +284,8 @@ int run_steps(sw_fdtd_state* st, int64_t n_steps, const double* env, const sw_fd
             a.Ao[c] = (T*)st->u_next[c];
             a.Bo[c] = (T*)st->b_next[c];
         }
-        int rc = launch_step_any<T>(a, incr, visc, force, kernel, st->tile_y, st->tile_z, st->x_chunk, s);
+        int rc = two ? launch_step2<T>(a, visc, st->tile_y, st->x_chunk, s)
+                     : launch_step_any<T>(a, incr, visc, force, one_kernel, one_tile_y, st->tile_z, st->x_chunk, s);
         if (rc) return rc;
         for (int c = 0; c < 3; ++c) {
             if (incr) {
@@ -225,8 +298,9 @@ int run_steps(sw_fdtd_state* st, int64_t n_steps, const double* env, const sw_fd
                 st->u_next[c] = old_b;
             }
         }
+        it += two ? 2 : 1;
         if (!out) continue;
-        const int64_t step = step0 + it;
+        const int64_t step = step0 + it - 1;  // the last step of this pass
         if (out->snap_stride > 0 && (step + 1) % out->snap_stride == 0) {
             const int64_t k = (step + 1) / out->snap_stride - 1;
             rc = sw_soa_to_aos3(DType<T>::code, st->u[0], st->u[1], st->u[2], (T*)out->snap_out + k * 3 * n, n, s);
@@ -262,7 +336,7 @@ extern "C" int sw_fdtd_run(sw_fdtd_state* st, int64_t n_steps, const double* env
     if ((rc = check_grid(st->nx, st->ny, st->nz))) return rc;
     SW_REQUIRE(st->form == SW_FORM_INCREMENT || st->form == SW_FORM_LEAPFROG, SW_ERR_ARG, "unknown form %d",
                st->form);
-    SW_REQUIRE(st->kernel >= SW_KERNEL_AUTO && st->kernel <= SW_KERNEL_MARCH, SW_ERR_ARG, "unknown kernel %d",
+    SW_REQUIRE(st->kernel >= SW_KERNEL_AUTO && st->kernel <= SW_KERNEL_MARCH2, SW_ERR_ARG, "unknown kernel %d",
                st->kernel);
     SW_REQUIRE(n_steps >= 0, SW_ERR_ARG, "n_steps must be >= 0");
     SW_REQUIRE(st->dt > 0 && st->dx > 0 && st->dy > 0 && st->dz > 0, SW_ERR_ARG, "dt and spacing must be > 0");

@@ -1,0 +1,181 @@
+// Two time steps per pass over the state (temporal blocking of the loop in
+// shear_fdtd, solver.py:194-234): one x-marching tile reads u_n, d_n and the
+// coefficients once and writes u_{n+2}, d_{n+2}, halving the HBM traffic per
+// voxel-step of the single-step tile. Increment form, no force (the pass is
+// used for the force-free steps between outputs).
+//
+// Tile: output columns (y0 .. y0+BY-1, z0 .. z0+63). Step n+2 at a point
+// needs step n+1 at its six neighbours, so step n+1 is evaluated on the tile
+// plus a one-voxel ring, which needs step n on the tile plus two voxels.
+// The block is 64 x (BY + 3) threads (one wave per row):
+//   rows 0 .. BY-1   output points: step n+1 and step n+2;
+//   row  BY          ring row y0-1 (step n+1), loads outer row y0-2;
+//   row  BY+1        ring row y0+BY (step n+1), loads outer row y0+BY+1;
+//   row  BY+2        lanes 0..BY-1: ring column z0-1 at row y0+lane, loads z0-2;
+//                    lanes BY..2BY-1: ring column z0+64, loads z0+65;
+//                    lanes 2BY..2BY+3: the four ring corners (step n only).
+// Each x iteration stages step n of plane x in LDS (tile + 2), evaluates
+// step n+1 at plane x on the tile + 1 ring (staged in a second LDS plane),
+// and step n+2 at plane x-1 from the step n+1 planes x-2, x-1, x. Both LDS
+// planes are double-buffered, so one barrier per plane suffices. Step n+1
+// and n+2 call the single-step arithmetic (point_inc), so a pass equals two
+// single steps bit for bit.
+#pragma once
+
+#include <type_traits>
+
+#include "march_step.h"
+
+namespace sw {
+
+template <typename T, bool VISC, int BY>
+struct Step2Lds {
+    static constexpr int NF = VISC ? 6 : 3;  // u and, with viscosity, d
+    T s0[2][NF][BY + 4][68];                 // step n, two-voxel halo
+    T s1[2][NF][BY + 2][66];                 // step n+1, one-voxel halo
+};
+
+template <int K>
+using Kc = std::integral_constant<int, K>;
+
+// body(Kc<k>{}, x) for x = xb .. xe (inclusive) with k = (x - xb) mod 4.
+template <typename F>
+__device__ __forceinline__ void march4(int xb, int xe, F&& body) {
+    for (int x = xb; x <= xe; x += 4) {
+        body(Kc<0>{}, x);
+        if (x + 1 > xe) break;
+        body(Kc<1>{}, x + 1);
+        if (x + 2 > xe) break;
+        body(Kc<2>{}, x + 2);
+        if (x + 3 > xe) break;
+        body(Kc<3>{}, x + 3);
+    }
+}
+
+// Write a column's fields into plane L at [r][q].
+template <typename T, bool VISC, typename Plane>
+__device__ __forceinline__ void stage(Plane& L, int r, int q, const Col<T>& c) {
+    L[0][r][q] = c.u0;
+    L[1][r][q] = c.u1;
+    L[2][r][q] = c.u2;
+    if (VISC) {
+        L[3][r][q] = c.d0;
+        L[4][r][q] = c.d1;
+        L[5][r][q] = c.d2;
+    }
+}
+
+template <typename T, bool VISC, int BY>
+__device__ __forceinline__ void march2_tile(const StepArgs<T>& a, int x0, int x1, int y0, int z0,
+                                            Step2Lds<T, VISC, BY>& lds) {
+    static_assert(2 * BY + 4 <= 64 && BY + 3 <= 16, "tile rows");
+    const Grid g = a.g;
+    const int tz = threadIdx.x, ty = threadIdx.y;
+    const int64_t P = g.plane();
+
+    // this thread's step-n+1 point (prim) and outer step-n point (outer)
+    int py = y0, pz = z0, oy = y0, oz = z0;
+    bool prim = true, outer = false;
+    if (ty < BY) {
+        py = y0 + ty;
+        pz = z0 + tz;
+    } else if (ty == BY || ty == BY + 1) {
+        py = ty == BY ? y0 - 1 : y0 + BY;
+        pz = z0 + tz;
+        oy = ty == BY ? y0 - 2 : y0 + BY + 1;
+        oz = pz;
+        outer = true;
+    } else if (tz < 2 * BY) {
+        py = oy = y0 + (tz < BY ? tz : tz - BY);
+        pz = tz < BY ? z0 - 1 : z0 + 64;
+        oz = tz < BY ? z0 - 2 : z0 + 65;
+        outer = true;
+    } else if (tz < 2 * BY + 4) {
+        const int k = tz - 2 * BY;
+        prim = false;
+        oy = k < 2 ? y0 - 1 : y0 + BY;
+        oz = (k & 1) ? z0 + 64 : z0 - 1;
+        outer = true;
+    } else {
+        prim = false;
+    }
+    const bool out_thread = ty < BY;
+    auto in_grid = [&](int y, int z) { return y >= 0 && y < g.ny && z >= 0 && z < g.nz; };
+    const bool p_in = prim && in_grid(py, pz);
+    const bool p_int = p_in && py > 0 && py < g.ny - 1 && pz > 0 && pz < g.nz - 1;
+    const uint32_t pyz = p_in ? (uint32_t)(py * g.nz + pz) * (uint32_t)sizeof(T) : 0u;
+    const bool o_in = outer && in_grid(oy, oz);
+    const uint32_t oyz = o_in ? (uint32_t)(oy * g.nz + oz) * (uint32_t)sizeof(T) : 0u;
+    const int r0 = py - y0 + 2, q0 = pz - z0 + 2;  // in the step-n plane
+    const int r1 = py - y0 + 1, q1 = pz - z0 + 1;  // in the step-n+1 plane
+    const int ro = oy - y0 + 2, qo = oz - z0 + 2;
+
+    auto po = [&](int xx) { return (int64_t)min(max(xx, 0), g.nx - 1) * P; };
+    auto col = [&](int xx, uint32_t o) { return load_col<T, true>(a, po(xx), o); };
+    auto coef = [&](int xx) { return load_coef<T, VISC, false>(a, po(xx), pyz); };
+    auto outer_col = [&](int xx) {
+        Col<T> c{};
+        if (outer) c = col(xx, oyz);
+        return c;
+    };
+
+    // rings, slot (plane - xb) & 3 with xb = x0 - 1 the first iteration
+    Col<T> q[4];   // step n of this column: planes x-1 .. x+2
+    Coef<T> k[4];  // coefficients: planes x-1 .. x+1
+    Col<T> s[4];   // step n+1 of this column: planes x-2 .. x
+    Col<T> o[4];   // step n at the outer point: planes x, x+1
+    const int xb = x0 - 1;
+    q[3] = col(x0 - 2, pyz);
+    q[0] = col(x0 - 1, pyz);
+    q[1] = col(x0, pyz);
+    k[0] = coef(x0 - 1);
+    o[0] = outer_col(x0 - 1);
+
+    march4(xb, x1, [&](auto kc, int x) {
+        constexpr int K = decltype(kc)::value;
+        constexpr int B = K & 1;
+        auto& S0 = lds.s0[B];
+        auto& S1 = lds.s1[B];
+        // stage step n of plane x
+        if (prim) stage<T, VISC>(S0, r0, q0, q[K]);
+        if (outer) stage<T, VISC>(S0, ro, qo, o[K]);
+        // prefetch: step n at x+2, coefficients and outer point at x+1
+        q[(K + 2) & 3] = col(x + 2, pyz);
+        k[(K + 1) & 3] = coef(x + 1);
+        o[(K + 1) & 3] = outer_col(x + 1);
+        __syncthreads();
+        // step n+1 at plane x on the tile and its ring
+        {
+            T i0, i1, i2;
+            point_inc<T, VISC, false>(a, q[(K + 3) & 3], q[K], q[(K + 1) & 3], S0, r0, q0, k[K], i0, i1, i2);
+            const bool in = p_int && x > 0 && x < g.nx - 1;
+            Col<T> n;
+            n.u0 = in ? q[K].u0 + i0 : T(0);
+            n.u1 = in ? q[K].u1 + i1 : T(0);
+            n.u2 = in ? q[K].u2 + i2 : T(0);
+            n.d0 = in ? i0 : T(0);
+            n.d1 = in ? i1 : T(0);
+            n.d2 = in ? i2 : T(0);
+            s[K] = n;
+            if (prim) stage<T, VISC>(S1, r1, q1, n);
+        }
+        // step n+2 at plane x-1 (staged in the other step-n+1 buffer one plane ago)
+        if (out_thread && x - 1 >= x0 && p_in) {
+            auto& S1p = lds.s1[B ^ 1];
+            const Col<T>& c = s[(K + 3) & 3];
+            T i0, i1, i2;
+            point_inc<T, VISC, false>(a, s[(K + 2) & 3], c, s[K], S1p, r1, q1, k[(K + 3) & 3], i0, i1, i2);
+            const bool in = p_int && x - 1 > 0 && x - 1 < g.nx - 1;
+            if (!in) i0 = i1 = i2 = T(0);
+            const int64_t px = (int64_t)(x - 1) * P;
+            store_out(&at_byte(a.Ao[0] + px, pyz), in ? c.u0 + i0 : T(0));
+            store_out(&at_byte(a.Ao[1] + px, pyz), in ? c.u1 + i1 : T(0));
+            store_out(&at_byte(a.Ao[2] + px, pyz), in ? c.u2 + i2 : T(0));
+            store_out(&at_byte(a.Bo[0] + px, pyz), i0);
+            store_out(&at_byte(a.Bo[1] + px, pyz), i1);
+            store_out(&at_byte(a.Bo[2] + px, pyz), i2);
+        }
+    });
+}
+
+}  // namespace sw

@@ -133,19 +133,33 @@ __device__ inline void cg_reset(CgState* st, double tol, long long max_iters) {
 // queue's roles rotate over a loop unrolled by six, so no prefetched register
 // is moved.
 
-constexpr int kCgBY = 8;   // tile rows in y (block of 512 threads)
+#ifndef SW_CG_BY
+#define SW_CG_BY 16
+#endif
+constexpr int kCgBY = SW_CG_BY;  // tile rows in y (block of 64 x kCgBY threads)
 constexpr int kCgLZ = 68;  // 64 + 2 + 2
 
 using CgLds = double[2][kCgBY + 4][kCgLZ];
 
+// Offsets inside a plane are 32-bit (planes of fewer than kMaxMarchPlane
+// voxels, which the launchers check): a load is then a uniform plane pointer
+// plus a lane offset, with no 64-bit address arithmetic per load.
 struct CgTile {
     int x0, x1, y, z;
     bool inyz;
-    int64_t yz;
+    uint32_t yz;   // element offset of (y, z) in a plane
+    uint32_t yzb;  // the same in bytes of a double
     bool halo_thread, halo_in;
     int ly, lz;
-    int64_t hyz;
+    uint32_t hyzb;  // byte offset of this thread's halo point
 };
+
+// Plane xx of f, clamped into the grid: a uniform pointer. Values read from a
+// clamped plane only feed terms that the boundary masks drop.
+template <typename T>
+__device__ __forceinline__ T* plane_at(T* f, int xx, const Grid& g) {
+    return f + (int64_t)min(max(xx, 0), g.nx - 1) * g.plane();
+}
 
 __device__ inline CgTile cg_tile(const Grid& g, int x0, int x1, int y0, int z0) {
     CgTile t;
@@ -155,7 +169,8 @@ __device__ inline CgTile cg_tile(const Grid& g, int x0, int x1, int y0, int z0) 
     t.y = y0 + ty;
     t.z = z0 + tz;
     t.inyz = t.y < g.ny && t.z < g.nz;
-    t.yz = t.inyz ? (int64_t)t.y * g.nz + t.z : 0;
+    t.yz = t.inyz ? (uint32_t)(t.y * g.nz + t.z) : 0u;
+    t.yzb = t.yz * (uint32_t)sizeof(double);
     // halo: rows y0-2, y0-1, y0+BY, y0+BY+1 (4 x 64 points), columns z0-2, z0-1, z0+64, z0+65 (4 x BY)
     int hy = -1, hz = -1;
     t.ly = t.lz = 0;
@@ -174,7 +189,7 @@ __device__ inline CgTile cg_tile(const Grid& g, int x0, int x1, int y0, int z0) 
     }
     t.halo_thread = tid < 256 + 4 * kCgBY;
     t.halo_in = t.halo_thread && hy >= 0 && hy < g.ny && hz >= 0 && hz < g.nz;
-    t.hyz = t.halo_in ? (int64_t)hy * g.nz + hz : 0;
+    t.hyzb = t.halo_in ? (uint32_t)(hy * g.nz + hz) * (uint32_t)sizeof(double) : 0u;
     return t;
 }
 
@@ -183,114 +198,142 @@ __device__ inline CgTile cg_tile(const Grid& g, int x0, int x1, int y0, int z0) 
 // boundary layer (solver.py:1024-1026 with gradient_center zero there).
 __device__ inline double cg_apply(const Grid& g, int x, int y, int z, double a, double c, double e,
                                   const double (*L)[kCgLZ], int r, int q, double qx, double qy, double qz) {
+    // the four LDS reads are unconditional (always inside the tile's halo), so
+    // they issue together and the masks below are selects, not branches
+    const double yp = L[r + 2][q], ym = L[r - 2][q], zp = L[r][q + 2], zm = L[r][q - 2];
     const double tx = ((x + 1 < g.nx - 1) ? (e - c) : 0.0) - ((x - 1 > 0) ? (c - a) : 0.0);
-    const double ty = ((y + 1 < g.ny - 1) ? (L[r + 2][q] - c) : 0.0) - ((y - 1 > 0) ? (c - L[r - 2][q]) : 0.0);
-    const double tz = ((z + 1 < g.nz - 1) ? (L[r][q + 2] - c) : 0.0) - ((z - 1 > 0) ? (c - L[r][q - 2]) : 0.0);
+    const double ty = ((y + 1 < g.ny - 1) ? (yp - c) : 0.0) - ((y - 1 > 0) ? (c - ym) : 0.0);
+    const double tz = ((z + 1 < g.nz - 1) ? (zp - c) : 0.0) - ((z - 1 > 0) ? (c - zm) : 0.0);
     return -(tx * qx + ty * qy + tz * qz);
 }
+
+// Prefetch distance of the CG tiles, in x planes: a plane's loads are issued
+// this many iterations before the one that consumes them, and nothing is
+// computed on a loaded value before then, so no wait for memory falls inside
+// the iteration that issued it (1..3: the rings below hold four planes).
+#ifndef SW_CG_DIST
+#define SW_CG_DIST 2
+#endif
+constexpr int kCgDist = SW_CG_DIST;
+static_assert(kCgDist >= 1 && kCgDist <= 3, "CG prefetch distance");
+
+template <int K>
+using Ic = std::integral_constant<int, K>;
+
+// body(Ic<k>{}, x) for x = x0 .. x1-1 with k = (x - x0) mod 8: ring slots
+// indexed by k are compile-time, so the rings live in registers and their
+// roles rotate without moves.
+template <typename F>
+__device__ __forceinline__ void march8(int x0, int x1, F&& body) {
+    for (int x = x0; x < x1; x += 8) {
+        body(Ic<0>{}, x);
+        if (x + 1 >= x1) break;
+        body(Ic<1>{}, x + 1);
+        if (x + 2 >= x1) break;
+        body(Ic<2>{}, x + 2);
+        if (x + 3 >= x1) break;
+        body(Ic<3>{}, x + 3);
+        if (x + 4 >= x1) break;
+        body(Ic<4>{}, x + 4);
+        if (x + 5 >= x1) break;
+        body(Ic<5>{}, x + 5);
+        if (x + 6 >= x1) break;
+        body(Ic<6>{}, x + 6);
+        if (x + 7 >= x1) break;
+        body(Ic<7>{}, x + 7);
+    }
+}
+
+// r and p_old of one point, as loaded.
+struct CgRaw {
+    double r, q;
+};
 
 // p = r + beta p_old (one fma, so every recomputation is bit-identical), A p.
 // Writes p over the tile; returns this thread's share of p.Ap.
 __device__ inline double cg_dir_tile(const double* __restrict__ r, const double* __restrict__ p_old,
                                      double* __restrict__ p, const Grid& g, double beta, double qx, double qy,
                                      double qz, const CgTile& t, CgLds& lds) {
-    const int64_t P = g.plane();
     const int r0 = threadIdx.y + 2, q0 = threadIdx.x + 2;
-    // voxel 0 is on the boundary layer, where r and p_old are zero
-    auto pn = [&](int64_t o, bool ok) {
-        const int64_t k = ok ? o : 0;
-        return fma(beta, p_old[k], r[k]);
-    };
-    auto center = [&](int xx) { return pn((int64_t)xx * P + t.yz, t.inyz && xx >= 0 && xx < g.nx); };
-    auto halo = [&](int xx) {
-        return t.halo_thread ? pn((int64_t)xx * P + t.hyz, t.halo_in && xx >= 0 && xx < g.nx) : 0.0;
-    };
+    auto raw = [&](int xx, uint32_t o) { return CgRaw{at_byte(plane_at(r, xx, g), o), at_byte(plane_at(p_old, xx, g), o)}; };
+    auto center = [&](int xx) { return raw(xx, t.yzb); };
+    // halo loads are unconditional (lanes without a halo point read offset 0),
+    // so no branch separates a load from the wait that precedes its use
+    auto halo = [&](int xx) { return raw(xx, t.hyzb); };
+    auto pv = [&](const CgRaw& v) { return fma(beta, v.q, v.r); };
+    const int x0 = t.x0, x1 = t.x1;
+    double pq[8];  // p of this column at plane x0 + k in slot k & 7 (planes x-2 .. x+2 live)
+    CgRaw rq[4];   // raw loads of this column in flight, slot k & 3 (planes x+3 .. x+2+D)
+    CgRaw hq[4];   // raw loads of this thread's halo point, slot k & 3 (planes x .. x+D)
+#pragma unroll
+    for (int k = -2; k < 2; ++k) pq[k & 7] = pv(center(x0 + k));
+#pragma unroll
+    for (int k = 2; k < 2 + kCgDist; ++k) rq[k & 3] = center(x0 + k);
+#pragma unroll
+    for (int k = 0; k < kCgDist; ++k) hq[k & 3] = halo(x0 + k);
     double s = 0.0;
-    auto plane = [&](auto buf, double a, double c, double e, double& f, double hc, double& hn, int x) {
-        constexpr int B = decltype(buf)::value;
+    march8(x0, x1, [&](auto kc, int x) {
+        constexpr int K = decltype(kc)::value;
+        constexpr int B = K & 1;
+        const double e = pv(rq[(K + 2) & 3]);  // plane x+2, loaded D iterations ago
+        pq[(K + 2) & 7] = e;
+        const double a = pq[(K + 6) & 7], c = pq[K & 7];
         lds[B][r0][q0] = c;
-        if (t.halo_thread) lds[B][t.ly][t.lz] = hc;
-        f = center(x + 3);
-        hn = halo(x + 1);
+        if (t.halo_thread) lds[B][t.ly][t.lz] = pv(hq[K & 3]);
+        rq[(K + 2 + kCgDist) & 3] = center(x + 2 + kCgDist);
+        hq[(K + kCgDist) & 3] = halo(x + kCgDist);
         __syncthreads();
         if (!t.inyz) return;
         const bool in = g.interior(x, t.y, t.z);
         if (in) s += c * cg_apply(g, x, t.y, t.z, a, c, e, lds[B], r0, q0, qx, qy, qz);
-        p[(int64_t)x * P + t.yz] = in ? c : 0.0;
-    };
-    using B0 = std::integral_constant<int, 0>;
-    using B1 = std::integral_constant<int, 1>;
-    const int x0 = t.x0, x1 = t.x1;
-    double v0 = center(x0 - 2), v1 = center(x0 - 1), v2 = center(x0), v3 = center(x0 + 1), v4 = center(x0 + 2), v5;
-    double h0 = halo(x0), h1;
-    for (int x = x0; x < x1; x += 6) {
-        plane(B0{}, v0, v2, v4, v5, h0, h1, x);
-        if (x + 1 >= x1) break;
-        plane(B1{}, v1, v3, v5, v0, h1, h0, x + 1);
-        if (x + 2 >= x1) break;
-        plane(B0{}, v2, v4, v0, v1, h0, h1, x + 2);
-        if (x + 3 >= x1) break;
-        plane(B1{}, v3, v5, v1, v2, h1, h0, x + 3);
-        if (x + 4 >= x1) break;
-        plane(B0{}, v4, v0, v2, v3, h0, h1, x + 4);
-        if (x + 5 >= x1) break;
-        plane(B1{}, v5, v1, v3, v4, h1, h0, x + 5);
-    }
+        at_byte(plane_at(p, x, g), t.yzb) = in ? c : 0.0;
+    });
     return s;
 }
 
 // phi += alpha p, r -= alpha A p (A p recomputed from p) over the tile;
-// returns this thread's share of r.r.
+// returns this thread's share of r.r. r and phi are read and written once
+// here (non-temporal); p keeps plain loads, its lines serve the halos of the
+// neighbouring tiles.
 __device__ inline double cg_step_tile(double* __restrict__ phi, double* __restrict__ r, const double* __restrict__ p,
                                       const Grid& g, double alpha, double qx, double qy, double qz, const CgTile& t,
                                       CgLds& lds) {
-    const int64_t P = g.plane();
     const int r0 = threadIdx.y + 2, q0 = threadIdx.x + 2;
-    auto ld = [&](const double* f, int64_t o, bool ok) { return f[ok ? o : 0]; };
-    auto center = [&](int xx) { return ld(p, (int64_t)xx * P + t.yz, t.inyz && xx >= 0 && xx < g.nx); };
-    auto halo = [&](int xx) {
-        return t.halo_thread ? ld(p, (int64_t)xx * P + t.hyz, t.halo_in && xx >= 0 && xx < g.nx) : 0.0;
-    };
-    auto ldr = [&](int xx) { return ld(r, (int64_t)xx * P + t.yz, t.inyz && xx >= 0 && xx < g.nx); };
-    auto ldphi = [&](int xx) { return ld(phi, (int64_t)xx * P + t.yz, t.inyz && xx >= 0 && xx < g.nx); };
+    auto center = [&](int xx) { return at_byte(plane_at(p, xx, g), t.yzb); };
+    auto halo = [&](int xx) { return at_byte(plane_at(p, xx, g), t.hyzb); };
+    auto once = [&](double* f, int xx) { return load_once(&at_byte(plane_at(f, xx, g), t.yzb)); };
+    const int x0 = t.x0, x1 = t.x1;
+    double pq[8];  // p of this column, slot k & 7 (planes x-2 .. x+2+D live)
+    double hq[4];  // p at this thread's halo point, slot k & 3 (planes x .. x+D)
+    double rq[4];  // r of this column, slot k & 3
+    double fq[4];  // phi of this column, slot k & 3
+#pragma unroll
+    for (int k = -2; k < 2 + kCgDist; ++k) pq[k & 7] = center(x0 + k);
+#pragma unroll
+    for (int k = 0; k < kCgDist; ++k) {
+        hq[k & 3] = halo(x0 + k);
+        rq[k & 3] = once(r, x0 + k);
+        fq[k & 3] = once(phi, x0 + k);
+    }
     double s = 0.0;
-    auto plane = [&](auto buf, double a, double c, double e, double& f, double hc, double& hn, double rc, double& rn,
-                     double fc, double& fn, int x) {
-        constexpr int B = decltype(buf)::value;
+    march8(x0, x1, [&](auto kc, int x) {
+        constexpr int K = decltype(kc)::value;
+        constexpr int B = K & 1;
+        const double a = pq[(K + 6) & 7], c = pq[K & 7], e = pq[(K + 2) & 7];
         lds[B][r0][q0] = c;
-        if (t.halo_thread) lds[B][t.ly][t.lz] = hc;
-        f = center(x + 3);
-        hn = halo(x + 1);
-        rn = ldr(x + 1);
-        fn = ldphi(x + 1);
+        if (t.halo_thread) lds[B][t.ly][t.lz] = hq[K & 3];
+        pq[(K + 2 + kCgDist) & 7] = center(x + 2 + kCgDist);
+        hq[(K + kCgDist) & 3] = halo(x + kCgDist);
+        rq[(K + kCgDist) & 3] = once(r, x + kCgDist);
+        fq[(K + kCgDist) & 3] = once(phi, x + kCgDist);
         __syncthreads();
         if (!t.inyz || !g.interior(x, t.y, t.z)) return;
         const double ap = cg_apply(g, x, t.y, t.z, a, c, e, lds[B], r0, q0, qx, qy, qz);
-        const int64_t o = (int64_t)x * P + t.yz;
-        phi[o] = fc + alpha * c;
-        const double rv = rc - alpha * ap;
-        r[o] = rv;
+        store_out(&at_byte(plane_at(phi, x, g), t.yzb), fq[K & 3] + alpha * c);
+        const double rv = rq[K & 3] - alpha * ap;
+        store_out(&at_byte(plane_at(r, x, g), t.yzb), rv);
         s += rv * rv;
-    };
-    using B0 = std::integral_constant<int, 0>;
-    using B1 = std::integral_constant<int, 1>;
-    const int x0 = t.x0, x1 = t.x1;
-    double v0 = center(x0 - 2), v1 = center(x0 - 1), v2 = center(x0), v3 = center(x0 + 1), v4 = center(x0 + 2), v5;
-    double h0 = halo(x0), h1;
-    double ra = ldr(x0), rb, fa = ldphi(x0), fb;
-    for (int x = x0; x < x1; x += 6) {
-        plane(B0{}, v0, v2, v4, v5, h0, h1, ra, rb, fa, fb, x);
-        if (x + 1 >= x1) break;
-        plane(B1{}, v1, v3, v5, v0, h1, h0, rb, ra, fb, fa, x + 1);
-        if (x + 2 >= x1) break;
-        plane(B0{}, v2, v4, v0, v1, h0, h1, ra, rb, fa, fb, x + 2);
-        if (x + 3 >= x1) break;
-        plane(B1{}, v3, v5, v1, v2, h1, h0, rb, ra, fb, fa, x + 3);
-        if (x + 4 >= x1) break;
-        plane(B0{}, v4, v0, v2, v3, h0, h1, ra, rb, fa, fb, x + 4);
-        if (x + 5 >= x1) break;
-        plane(B1{}, v5, v1, v3, v4, h1, h0, rb, ra, fb, fa, x + 5);
-    }
+    });
     return s;
 }
 

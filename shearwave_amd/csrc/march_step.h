@@ -37,16 +37,17 @@ struct Coef {
     T m, e, f0, f1, f2;
 };
 
-// Loads are unconditional from a clamped offset (voxel 0 when `ok` is false).
-// Values loaded for points outside the grid only ever feed boundary voxels,
-// whose outputs are overwritten with zero, so no select follows the load and
-// the load's latency stays hidden behind the next barrier. (Voxel 0 is on the
-// boundary layer, which holds zero in every state array.)
+// Loads are unconditional: plane xx clamped into the grid (a uniform offset
+// `po`) at a 32-bit lane byte offset `o` inside the plane (0 for lanes
+// outside the grid). Values loaded for points outside the grid only ever feed boundary
+// voxels, whose outputs are overwritten with zero, so no select follows the
+// load and the load's latency stays hidden behind the next barrier. The
+// address is a scalar base plus the lane offset: no 64-bit arithmetic per
+// load. Planes hold fewer than kMaxMarchPlane voxels (checked at launch).
 template <typename T, bool INCR>
-__device__ __forceinline__ Col<T> load_col(const StepArgs<T>& a, int64_t o, bool ok) {
-    const int64_t q = ok ? o : 0;
-    const T u0 = a.A[0][q], u1 = a.A[1][q], u2 = a.A[2][q];
-    const T b0 = a.B[0][q], b1 = a.B[1][q], b2 = a.B[2][q];
+__device__ __forceinline__ Col<T> load_col(const StepArgs<T>& a, int64_t po, uint32_t o) {
+    const T u0 = at_byte(a.A[0] + po, o), u1 = at_byte(a.A[1] + po, o), u2 = at_byte(a.A[2] + po, o);
+    const T b0 = at_byte(a.B[0] + po, o), b1 = at_byte(a.B[1] + po, o), b2 = at_byte(a.B[2] + po, o);
     Col<T> c;
     c.u0 = u0;
     c.u1 = u1;
@@ -57,58 +58,67 @@ __device__ __forceinline__ Col<T> load_col(const StepArgs<T>& a, int64_t o, bool
     return c;
 }
 
-// Single-use streams (coefficients, force) are read with the non-temporal
-// hint, so they do not evict halo lines from the L2 (SW_NO_NT: plain loads).
-template <typename T>
-__device__ __forceinline__ T load_once(const T* p) {
-#ifndef SW_NO_NT
-    return __builtin_nontemporal_load(p);
-#else
-    return *p;
-#endif
-}
-
+// Coefficients and force are single-use streams (load_once: non-temporal).
 template <typename T, bool VISC, bool FORCE>
-__device__ __forceinline__ Coef<T> load_coef(const StepArgs<T>& a, int64_t o, bool ok) {
-    const int64_t q = ok ? o : 0;
+__device__ __forceinline__ Coef<T> load_coef(const StepArgs<T>& a, int64_t po, uint32_t o) {
     Coef<T> k;
-    k.m = load_once(a.m + q);
-    k.e = VISC ? load_once(a.e + q) : T(0);
-    k.f0 = FORCE ? load_once(a.f[0] + q) : T(0);
-    k.f1 = FORCE ? load_once(a.f[1] + q) : T(0);
-    k.f2 = FORCE ? load_once(a.f[2] + q) : T(0);
+    k.m = load_once(&at_byte(a.m + po, o));
+    k.e = VISC ? load_once(&at_byte(a.e + po, o)) : T(0);
+    k.f0 = FORCE ? load_once(&at_byte(a.f[0] + po, o)) : T(0);
+    k.f1 = FORCE ? load_once(&at_byte(a.f[1] + po, o)) : T(0);
+    k.f2 = FORCE ? load_once(&at_byte(a.f[2] + po, o)) : T(0);
     return k;
 }
 
-// Output store of the time step: the next step's state is written with the
-// non-temporal hint (it is re-read one launch later, after the L2 has turned
-// over), leaving the L2 to halo reuse. SW_NO_NT builds plain stores.
-template <typename T>
-__device__ __forceinline__ void store_out(T* p, T v) {
-#ifndef SW_NO_NT
-    __builtin_nontemporal_store(v, p);
-#else
-    *p = v;
-#endif
-}
+// The next step's state is written with store_out (non-temporal): it is
+// re-read one launch later, so the L2 is left to halo reuse.
 
-template <typename T, bool VISC>
+// Increment of one component at one point (solver.py:194-234 in increment
+// form): (d + m lap(u) + e lap(d) + f) (1 - vel_damp), with lap the 7-point
+// Laplacian scaled per axis. Contraction is off and every fused multiply-add
+// is written out, so the result does not depend on the kernel it is inlined
+// into: one pass of the two-step kernel equals two single steps bit for bit.
+template <typename T, bool VISC, bool FORCE>
 __device__ __forceinline__ T comp_inc(const StepArgs<T>& a, T u0, T um, T up, T uyp, T uym, T uzp, T uzm, T d0, T dm,
                                       T dp, T dyp, T dym, T dzp, T dzm, T m, T e, T f) {
-    const T lu = a.cx * (up + um - T(2) * u0) + a.cy * (uyp + uym - T(2) * u0) + a.cz * (uzp + uzm - T(2) * u0);
-    T inc = d0 + m * lu + f;
+#pragma clang fp contract(off)
+    const T lu = fma(a.cx, fma(T(-2), u0, up + um), fma(a.cy, fma(T(-2), u0, uyp + uym), a.cz * fma(T(-2), u0, uzp + uzm)));
+    T inc = fma(m, lu, d0);
     if (VISC) {
-        const T ld = a.vx * (dp + dm - T(2) * d0) + a.vy * (dyp + dym - T(2) * d0) + a.vz * (dzp + dzm - T(2) * d0);
-        inc += e * ld;
+        const T ld =
+            fma(a.vx, fma(T(-2), d0, dp + dm), fma(a.vy, fma(T(-2), d0, dyp + dym), a.vz * fma(T(-2), d0, dzp + dzm)));
+        inc = fma(e, ld, inc);
     }
+    if (FORCE) inc = inc + f;
     return inc * a.damp;
 }
 
+// Increments of the three components at one point: m, c, p the point's
+// column at x-1, x, x+1; L an LDS plane of fields u0, u1, u2 (and d0, d1, d2
+// with viscosity) holding the point at [r][q] and its y/z neighbours.
+template <typename T, bool VISC, bool FORCE, typename Plane>
+__device__ __forceinline__ void point_inc(const StepArgs<T>& a, const Col<T>& m, const Col<T>& c, const Col<T>& p,
+                                          const Plane& L, int r, int q, const Coef<T>& k, T& i0, T& i1, T& i2) {
+    constexpr int NL = VISC ? 6 : 3;
+    auto nb = [&](int f, int dy, int dz) { return L[f][r + dy][q + dz]; };
+    auto dn = [&](int f, int dy, int dz) { return VISC ? L[NL - 3 + f][r + dy][q + dz] : T(0); };
+    i0 = comp_inc<T, VISC, FORCE>(a, c.u0, m.u0, p.u0, nb(0, 1, 0), nb(0, -1, 0), nb(0, 0, 1), nb(0, 0, -1), c.d0,
+                                  m.d0, p.d0, dn(0, 1, 0), dn(0, -1, 0), dn(0, 0, 1), dn(0, 0, -1), k.m, k.e,
+                                  FORCE ? a.fs * k.f0 : T(0));
+    i1 = comp_inc<T, VISC, FORCE>(a, c.u1, m.u1, p.u1, nb(1, 1, 0), nb(1, -1, 0), nb(1, 0, 1), nb(1, 0, -1), c.d1,
+                                  m.d1, p.d1, dn(1, 1, 0), dn(1, -1, 0), dn(1, 0, 1), dn(1, 0, -1), k.m, k.e,
+                                  FORCE ? a.fs * k.f1 : T(0));
+    i2 = comp_inc<T, VISC, FORCE>(a, c.u2, m.u2, p.u2, nb(2, 1, 0), nb(2, -1, 0), nb(2, 0, 1), nb(2, 0, -1), c.d2,
+                                  m.d2, p.d2, dn(2, 1, 0), dn(2, -1, 0), dn(2, 0, 1), dn(2, 0, -1), k.m, k.e,
+                                  FORCE ? a.fs * k.f2 : T(0));
+}
+
 // One tile of the time step: columns (y0 .. y0+BY-1, z0 .. z0+BZ-1), planes
-// x0 .. x1-1; the block is (BZ, BY) threads (BZ a multiple of 64, so a wave
-// never spans two rows) and all of them call this together. `lds` holds at
-// least NL = (VISC ? 6 : 3) planes of (BY+2) x (BZ+2).
-template <typename T, bool INCR, bool VISC, bool FORCE, int BY, int BZ, int NLD>
+// x0 .. x1-1; the block is (BZ, ROWS >= BY) threads (BZ a multiple of 64, so a
+// wave never spans two rows) and all of them call this together; rows BY and
+// up only keep the barriers. `lds` holds at least NL = (VISC ? 6 : 3) planes
+// of (BY+2) x (BZ+2).
+template <typename T, bool INCR, bool VISC, bool FORCE, int BY, int BZ, int NLD, int ROWS = BY>
 __device__ __forceinline__ void march_tile(const StepArgs<T>& a, int x0, int x1, int y0, int z0,
                                            T (&lds)[2][NLD][BY + 2][BZ + 2]) {
     constexpr int NL = VISC ? 6 : 3;  // planes staged in LDS: u (3) and, with viscosity, d (3)
@@ -117,42 +127,42 @@ __device__ __forceinline__ void march_tile(const StepArgs<T>& a, int x0, int x1,
     const Grid g = a.g;
     const int tz = threadIdx.x, ty = threadIdx.y;
 
+    const bool live = ROWS == BY || ty < BY;  // rows past BY only take part in the barriers
     const int y = y0 + ty, z = z0 + tz;
-    const bool inyz = (y < g.ny) && (z < g.nz);
+    const bool inyz = live && (y < g.ny) && (z < g.nz);
     const int64_t P = g.plane();
-    const int64_t yz = inyz ? (int64_t)y * g.nz + z : 0;
+    const uint32_t yz = inyz ? (uint32_t)(y * g.nz + z) * (uint32_t)sizeof(T) : 0u;  // bytes
     const bool yz_interior = inyz && y > 0 && y < g.ny - 1 && z > 0 && z < g.nz - 1;
 
     // Halo of the tile. Row halos (y0-1, y0+BY) are loaded by the first and
     // last row of threads, at their own z (wave-uniform); column halos
     // (z0-1, z0+BZ) by the first and last lane of each row. A halo point
-    // outside the grid is read from voxel 0 (on the boundary layer, zero) and
-    // only feeds boundary voxels.
+    // outside the grid is read at offset 0 of its plane and only feeds
+    // boundary voxels.
     const bool row_thread = ty == 0 || ty == BY - 1;
     const int ry = ty == 0 ? y0 - 1 : y0 + BY;
     const int rly = ty == 0 ? 0 : BY + 1;
     const bool row_in = row_thread && ry >= 0 && ry < g.ny && z < g.nz;
-    const int64_t ryz = row_in ? (int64_t)ry * g.nz + z : 0;
-    const bool col_thread = tz == 0 || tz == BZ - 1;
+    const uint32_t ryz = row_in ? (uint32_t)(ry * g.nz + z) * (uint32_t)sizeof(T) : 0u;
+    const bool col_thread = live && (tz == 0 || tz == BZ - 1);
     const int cz = tz == 0 ? z0 - 1 : z0 + BZ;
     const int clz = tz == 0 ? 0 : BZ + 1;
     const bool col_in = col_thread && cz >= 0 && cz < g.nz && y < g.ny;
-    const int64_t cyz = col_in ? (int64_t)y * g.nz + cz : 0;
+    const uint32_t cyz = col_in ? (uint32_t)(y * g.nz + cz) * (uint32_t)sizeof(T) : 0u;
 
-    auto center = [&](int xx) { return load_col<T, INCR>(a, (int64_t)xx * P + yz, inyz && xx >= 0 && xx < g.nx); };
+    auto po = [&](int xx) { return (int64_t)min(max(xx, 0), g.nx - 1) * P; };
+    auto center = [&](int xx) { return load_col<T, INCR>(a, po(xx), yz); };
     auto row_halo = [&](int xx) {
         Col<T> c{};
-        if (row_thread) c = load_col<T, INCR>(a, (int64_t)xx * P + ryz, row_in && xx >= 0 && xx < g.nx);
+        if (row_thread) c = load_col<T, INCR>(a, po(xx), ryz);
         return c;
     };
     auto col_halo = [&](int xx) {
         Col<T> c{};
-        if (col_thread) c = load_col<T, INCR>(a, (int64_t)xx * P + cyz, col_in && xx >= 0 && xx < g.nx);
+        if (col_thread) c = load_col<T, INCR>(a, po(xx), cyz);
         return c;
     };
-    auto coef = [&](int xx) {
-        return load_coef<T, VISC, FORCE>(a, (int64_t)xx * P + yz, yz_interior && xx > 0 && xx < g.nx - 1);
-    };
+    auto coef = [&](int xx) { return load_coef<T, VISC, FORCE>(a, po(xx), yz); };
 
     // One x plane. m, c, p: planes x-1, x, x+1 of this column; n receives x+2.
     // hc: this plane's halo point, hn receives the next one; kc / kn likewise for coefficients.
@@ -160,13 +170,15 @@ __device__ __forceinline__ void march_tile(const StepArgs<T>& a, int x0, int x1,
                      Col<T>& hn, const Col<T>& vc, Col<T>& vn, const Coef<T>& kc, Coef<T>& kn, int x) {
         constexpr int B = decltype(buf)::value;
         auto& L = lds[B];
-        L[0][ty + 1][tz + 1] = c.u0;
-        L[1][ty + 1][tz + 1] = c.u1;
-        L[2][ty + 1][tz + 1] = c.u2;
-        if (VISC) {
-            L[NL - 3][ty + 1][tz + 1] = c.d0;
-            L[NL - 2][ty + 1][tz + 1] = c.d1;
-            L[NL - 1][ty + 1][tz + 1] = c.d2;
+        if (live) {
+            L[0][ty + 1][tz + 1] = c.u0;
+            L[1][ty + 1][tz + 1] = c.u1;
+            L[2][ty + 1][tz + 1] = c.u2;
+            if (VISC) {
+                L[NL - 3][ty + 1][tz + 1] = c.d0;
+                L[NL - 2][ty + 1][tz + 1] = c.d1;
+                L[NL - 1][ty + 1][tz + 1] = c.d2;
+            }
         }
         if (row_thread) {
             L[0][rly][tz + 1] = hc.u0;
@@ -195,29 +207,17 @@ __device__ __forceinline__ void march_tile(const StepArgs<T>& a, int x0, int x1,
         __syncthreads();
         if (!inyz) return;
         const bool in = yz_interior && x > 0 && x < g.nx - 1;
-        const int r = ty + 1, q = tz + 1;
-        auto nb = [&](int f, int dy, int dz) { return L[f][r + dy][q + dz]; };
-        T i0 = comp_inc<T, VISC>(a, c.u0, m.u0, p.u0, nb(0, 1, 0), nb(0, -1, 0), nb(0, 0, 1), nb(0, 0, -1), c.d0, m.d0,
-                                 p.d0, VISC ? nb(NL - 3, 1, 0) : T(0), VISC ? nb(NL - 3, -1, 0) : T(0),
-                                 VISC ? nb(NL - 3, 0, 1) : T(0), VISC ? nb(NL - 3, 0, -1) : T(0), kc.m, kc.e,
-                                 a.fs * kc.f0);
-        T i1 = comp_inc<T, VISC>(a, c.u1, m.u1, p.u1, nb(1, 1, 0), nb(1, -1, 0), nb(1, 0, 1), nb(1, 0, -1), c.d1, m.d1,
-                                 p.d1, VISC ? nb(NL - 2, 1, 0) : T(0), VISC ? nb(NL - 2, -1, 0) : T(0),
-                                 VISC ? nb(NL - 2, 0, 1) : T(0), VISC ? nb(NL - 2, 0, -1) : T(0), kc.m, kc.e,
-                                 a.fs * kc.f1);
-        T i2 = comp_inc<T, VISC>(a, c.u2, m.u2, p.u2, nb(2, 1, 0), nb(2, -1, 0), nb(2, 0, 1), nb(2, 0, -1), c.d2, m.d2,
-                                 p.d2, VISC ? nb(NL - 1, 1, 0) : T(0), VISC ? nb(NL - 1, -1, 0) : T(0),
-                                 VISC ? nb(NL - 1, 0, 1) : T(0), VISC ? nb(NL - 1, 0, -1) : T(0), kc.m, kc.e,
-                                 a.fs * kc.f2);
+        T i0, i1, i2;
+        point_inc<T, VISC, FORCE>(a, m, c, p, L, ty + 1, tz + 1, kc, i0, i1, i2);
         if (!in) i0 = i1 = i2 = T(0);
-        const int64_t o = (int64_t)x * P + yz;
-        store_out(a.Ao[0] + o, in ? c.u0 + i0 : T(0));
-        store_out(a.Ao[1] + o, in ? c.u1 + i1 : T(0));
-        store_out(a.Ao[2] + o, in ? c.u2 + i2 : T(0));
+        const int64_t px = (int64_t)x * P;
+        store_out(&at_byte(a.Ao[0] + px, yz), in ? c.u0 + i0 : T(0));
+        store_out(&at_byte(a.Ao[1] + px, yz), in ? c.u1 + i1 : T(0));
+        store_out(&at_byte(a.Ao[2] + px, yz), in ? c.u2 + i2 : T(0));
         if (INCR) {
-            store_out(a.Bo[0] + o, i0);
-            store_out(a.Bo[1] + o, i1);
-            store_out(a.Bo[2] + o, i2);
+            store_out(&at_byte(a.Bo[0] + px, yz), i0);
+            store_out(&at_byte(a.Bo[1] + px, yz), i1);
+            store_out(&at_byte(a.Bo[2] + px, yz), i2);
         }
     };
 

@@ -187,6 +187,12 @@ __global__ void __launch_bounds__(64 * kCgBY) cg_step_march_kernel(double* __res
     }
 }
 
+// One 1024-thread block per CU: long x chunks keep the per-chunk prologue
+// (six planes of p, r, p_old loaded before the first output) small.
+#ifndef SW_CG_BLOCKS_PER_CU
+#define SW_CG_BLOCKS_PER_CU 1
+#endif
+
 struct CgShape {
     int xchunk, nty, ntz, ntx;
     unsigned blocks() const { return (unsigned)nty * ntz * ntx; }
@@ -197,8 +203,8 @@ CgShape cg_shape(const Grid& g) {
     c.nty = (g.ny + kCgBY - 1) / kCgBY;
     c.ntz = (g.nz + 63) / 64;
     const int64_t cols = (int64_t)c.nty * c.ntz;
-    // about four 512-thread blocks per CU in one wave, chunks of at least 12 planes
-    const int64_t target = (int64_t)device_cu_count() * 4;
+    // about SW_CG_BLOCKS_PER_CU 512-thread blocks per CU in one wave, chunks of at least 12 planes
+    const int64_t target = (int64_t)device_cu_count() * SW_CG_BLOCKS_PER_CU;
     int64_t ntx = (target + cols - 1) / cols;
     if (ntx < 1) ntx = 1;
     int64_t xc = (g.nx + ntx - 1) / ntx;
@@ -320,7 +326,7 @@ int sw_project_vector_field(int dtype, void* fx, void* fy, void* fz, int64_t nx,
     const CgShape cs = cg_shape(g);
     // the march kernels need at least one 64-wide z tile and 8 rows; tiny grids use the direct kernels
     const char* env_k = getenv("SW_CG_KERNEL");
-    bool march = nz >= 32 && ny >= 8;
+    bool march = nz >= 32 && ny >= 8 && g.plane() < kMaxMarchPlane;
     if (env_k && env_k[0] == 'd') march = false;
     const dim3 blk = row_block();
     const double i2hx = 1.0 / (2.0 * dx), i2hy = 1.0 / (2.0 * dy), i2hz = 1.0 / (2.0 * dz);

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "shearwave.h"
 
@@ -59,6 +60,28 @@ inline int check_grid(int64_t nx, int64_t ny, int64_t nz) {
     return SW_OK;
 }
 
+// The x-marching kernels address a plane with 32-bit lane byte offsets.
+constexpr int64_t kMaxMarchPlane = int64_t(1) << 28;
+
+// A pointer that is the same in every lane, marked so (readfirstlane): the
+// compiler keeps it in scalar registers and cannot fold lane offsets into it.
+template <typename T>
+__device__ __forceinline__ T* uniform_ptr(T* p) {
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return reinterpret_cast<T*>(((uint64_t)hi << 32) | lo);
+}
+
+// Element at a 32-bit byte offset from a base that is uniform across the wave
+// (callers guarantee it): the access takes the scalar-base +
+// 32-bit-vector-offset form, so no 64-bit address is formed or kept per lane.
+template <typename T>
+__device__ __forceinline__ T& at_byte(T* base, uint32_t off) {
+    using Byte = std::conditional_t<std::is_const<T>::value, const char, char>;
+    return *reinterpret_cast<T*>(reinterpret_cast<Byte*>(uniform_ptr(base)) + off);
+}
+
 // Row-structured launch used by the pointwise and small-stencil kernels:
 // block (64, 4); blockIdx.x walks groups of 4 (x, y) rows; lanes walk z.
 constexpr int kRowsPerBlock = 4;
@@ -111,6 +134,28 @@ __device__ inline double block_max(double v, double* red) {
     double s = red[0];
     for (int w = 1; w < nwaves; ++w) s = fmax(s, red[w]);
     return s;
+}
+
+// Single-use streams are read with the non-temporal hint, so they do not
+// evict lines that neighbouring tiles re-read from the L2 (SW_NO_NT: plain).
+template <typename T>
+__device__ __forceinline__ T load_once(const T* p) {
+#ifndef SW_NO_NT
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
+
+// Stores of data that is next read one launch later, after the L2 has turned
+// over, carry the non-temporal hint too (SW_NO_NT: plain stores).
+template <typename T>
+__device__ __forceinline__ void store_out(T* p, T v) {
+#ifndef SW_NO_NT
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
 }
 
 // Visit every voxel of g once: rows (x, y) by block-row, z by lane.

@@ -21,7 +21,8 @@ _DTYPES = {
     "float32": (torch.float32, _abi.SW_F32, np.float32),
     "float64": (torch.float64, _abi.SW_F64, np.float64),
 }
-_KERNELS = {"auto": _abi.SW_KERNEL_AUTO, "direct": _abi.SW_KERNEL_DIRECT, "march": _abi.SW_KERNEL_MARCH}
+_KERNELS = {"auto": _abi.SW_KERNEL_AUTO, "direct": _abi.SW_KERNEL_DIRECT, "march": _abi.SW_KERNEL_MARCH,
+            "march2": _abi.SW_KERNEL_MARCH2}
 _FORMS = {"increment": _abi.SW_FORM_INCREMENT, "leapfrog": _abi.SW_FORM_LEAPFROG}
 
 

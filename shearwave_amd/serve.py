@@ -5,9 +5,9 @@ its HBM footprint fits (fields, state, force and, during its projection, the
 float64 CG vectors) and a slot is free. Admission (upload, coefficients, and
 one batched projection of every request admitted together) runs on a second
 stream, so the solves already on the GPU keep stepping meanwhile. Admitted
-solves join the running batch at the next tick: every tick is `tick_steps`
-global steps of ONE persistent launch per step over all active solves, each
-at its own local step. Finished solves are exported (u, v in the reference's
+solves join the running batch at the next tick: every tick is `tick_passes`
+global passes of ONE persistent launch per pass over all active solves, each
+at its own pass (one step, or two force-free steps in one read of its state). Finished solves are exported (u, v in the reference's
 AoS layout) into pinned host memory; latency is arrival -> result on the host.
 
 A solve's arithmetic is the same whichever solves share its ticks (per-solve
@@ -38,7 +38,8 @@ class Request:
     arrival: float  # seconds after the server starts
     admitted: float = math.nan
     completed: float = math.nan
-    start_step: int = -1
+    start_pass: int = -1
+    n_passes: int = 0
     cg: dict = field(default_factory=dict)
     u: np.ndarray = None
     v: np.ndarray = None
@@ -51,8 +52,9 @@ class Request:
 class Server:
     """Continuous-batching solve server (float32 storage, increment form)."""
 
-    def __init__(self, tick_steps=20, max_active=128, budget_bytes=None, tuner=None, tile_y=8):
-        self.tick_steps = int(tick_steps)
+    def __init__(self, tick_passes=10, max_active=128, budget_bytes=None, tuner=None, tile_y=8, two_step=True):
+        self.tick_passes = int(tick_passes)
+        self.two_step = bool(two_step)
         self.max_active = int(max_active)
         self.tile_y = int(tile_y)
         self.tuner = tuner
@@ -63,8 +65,8 @@ class Server:
         self.budget = int(budget_bytes)
         self.main = torch.cuda.current_stream(dev)
         self.admit_stream = torch.cuda.Stream(dev)  # default priority
-        self.gstep_dev = torch.zeros(1, dtype=torch.int64, device=dev)
-        self.gstep = 0
+        self.gpass_dev = torch.zeros(1, dtype=torch.int64, device=dev)
+        self.gpass = 0
         self._helper = Batch([], tuner=tuner)  # descriptor / projection helpers
 
     # ------------------------------------------------------------ admission
@@ -73,7 +75,7 @@ class Server:
         with torch.cuda.stream(self.admit_stream):
             solves = []
             for r in reqs:
-                s = _Solve(r.spec)
+                s = _Solve(r.spec, self.two_step)
                 s.xchunk = self.tuner.xchunk(r.spec.shape) if self.tuner else min(r.spec.shape[0], 64)
                 solves.append(s)
             info = self._helper.project(solves)
@@ -82,11 +84,12 @@ class Server:
         self.main.wait_event(ev)
         now = time.perf_counter() - t0
         for r, s, i in zip(reqs, solves, info):
-            for t in (s.m, s.e, s.inv_rho, s.u, s.d, s.f, s.force, s.env):
+            for t in (s.m, s.e, s.inv_rho, s.u, s.d, s.f, s.force, s.env, s.plan):
                 if t is not None:
                     t.record_stream(self.main)
-            s.start = self.gstep
-            r.start_step = self.gstep
+            s.start = self.gpass
+            r.start_pass = self.gpass
+            r.n_passes = s.n_passes
             r.admitted = now
             r.cg = i
         return solves
@@ -106,7 +109,7 @@ class Server:
         changed = False
         t0 = time.perf_counter()
         done = 0
-        busy_steps = 0
+
         while done < len(reqs):
             now = time.perf_counter() - t0
             while nxt < len(reqs) and reqs[nxt].arrival <= now:
@@ -136,16 +139,15 @@ class Server:
                 # at most two ticks in flight
                 while len(tick_events) >= 2:
                     tick_events.popleft().synchronize()
-                check(lib().sw_batch_step(desc.data_ptr(), items_t.data_ptr(), n_items, self.gstep_dev.data_ptr(),
-                                          self.tile_y, 0, self.tick_steps, self.main.cuda_stream))
-                self.gstep += self.tick_steps
-                busy_steps += self.tick_steps
+                check(lib().sw_batch_step(desc.data_ptr(), items_t.data_ptr(), n_items, self.gpass_dev.data_ptr(),
+                                          self.tile_y, 0, self.tick_passes, self.main.cuda_stream))
+                self.gpass += self.tick_passes
                 ev = torch.cuda.Event()
                 ev.record(self.main)
                 tick_events.append(ev)
                 still = []
                 for r, s in active:
-                    if s.start + s.spec.n_steps <= self.gstep:
+                    if s.start + s.n_passes <= self.gpass:
                         pu, pv = self._export(s)
                         e2 = torch.cuda.Event()
                         e2.record(self.main)
@@ -179,14 +181,14 @@ class Server:
             "p50_latency_s": float(np.percentile(lat, 50)),
             "p99_latency_s": float(np.percentile(lat, 99)),
             "mean_latency_s": float(lat.mean()),
-            "global_steps": self.gstep,
+            "global_passes": self.gpass,
             "voxel_steps": int(sum(r.spec.voxels * r.spec.n_steps for r in reqs)),
         }
         return reqs, summary
 
     def _export(self, s):
         """u, v of a finished solve into pinned host buffers (queued on the main stream)."""
-        par = s.spec.n_steps & 1
+        par = s.n_passes & 1
         u, d = s.u[par], s.d[par]
         aos = s.shape + (3,)
         ua = torch.empty(aos, dtype=torch.float32, device=u.device)

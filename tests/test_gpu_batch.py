@@ -113,3 +113,17 @@ def test_batch_agrees_with_single_solve_dropin(sw, specs, mixed):
             "eta": sp.eta}
     u, v = sw.shear_fdtd(None, None, sp.force[2], sp.rho, sp.mu, *sp.h, sp.dt, sp.n_steps, opts)
     assert rel(u, mixed[k][0]) < 1e-6 and rel(v, mixed[k][1]) < 1e-6
+
+
+def test_two_step_passes_equal_single_steps(sw, specs, mixed):
+    """Pass plans that pair force-free steps give the same bits as one step per pass."""
+    from shearwave_amd.batch import Batch
+
+    b = Batch(specs, two_step=False)
+    assert all(s.n_passes == s.spec.n_steps for s in b.solves)
+    b.project()
+    b.run()
+    for k in range(len(specs)):
+        u, v = b.result(k)
+        np.testing.assert_array_equal(u, mixed[k][0])
+        np.testing.assert_array_equal(v, mixed[k][1])

@@ -204,7 +204,7 @@ def hetero():
     return shape, p, u, v
 
 
-@pytest.mark.parametrize("kernel", ["march", "direct"])
+@pytest.mark.parametrize("kernel", ["march2", "march", "direct"])
 @pytest.mark.parametrize("form", ["increment", "leapfrog"])
 def test_fdtd_fp64_matches_oracle(sw, hetero, kernel, form):
     shape, p, u_ref, v_ref = hetero
@@ -234,7 +234,7 @@ def test_fp32_tiles_bit_identical(sw, tile_y, tile_z):
     np.testing.assert_array_equal(v, base[1])
 
 
-@pytest.mark.parametrize("kernel", ["march", "direct"])
+@pytest.mark.parametrize("kernel", ["march2", "march", "direct"])
 def test_fdtd_fp32_within_1e5_of_fp64_oracle(sw, kernel):
     shape = (34, 32, 36)
     p = _problem(shape, seed=12, n_steps=800)

@@ -29,7 +29,7 @@ def trace():
 def served(sw, trace):
     from shearwave_amd.serve import Server
 
-    srv = Server(tick_steps=7)
+    srv = Server(tick_passes=7)
     reqs, summary = srv.run(copy.deepcopy(trace))
     return reqs, summary
 
@@ -39,9 +39,9 @@ def test_all_served_and_overlapping(served):
     assert summary["requests"] == len(reqs) and all(np.isfinite(r.latency) for r in reqs)
     assert summary["p99_latency_s"] >= summary["p50_latency_s"] > 0
     # at least two requests were on the GPU at the same time
-    spans = sorted((r.start_step, r.start_step + r.spec.n_steps) for r in reqs)
+    spans = sorted((r.start_pass, r.start_pass + r.n_passes) for r in reqs)
     assert any(spans[k + 1][0] < spans[k][1] for k in range(len(spans) - 1))
-    assert len({r.start_step for r in reqs}) > 1  # and they joined at different ticks
+    assert len({r.start_pass for r in reqs}) > 1  # and they joined at different ticks
 
 
 def test_same_alone_and_in_the_mix(sw, trace, served):
@@ -51,7 +51,7 @@ def test_same_alone_and_in_the_mix(sw, trace, served):
     for k in (0, 3, 5):
         alone = copy.deepcopy(trace[k])
         alone.arrival = 0.0
-        (r,), _ = Server(tick_steps=5).run([alone])
+        (r,), _ = Server(tick_passes=5).run([alone])
         np.testing.assert_array_equal(r.u, reqs[k].u)
         np.testing.assert_array_equal(r.v, reqs[k].v)
 
