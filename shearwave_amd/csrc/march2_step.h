@@ -38,10 +38,11 @@ struct Step2Lds {
 template <int K>
 using Kc = std::integral_constant<int, K>;
 
-// body(Kc<k>{}, x) for x = xb .. xe (inclusive) with k = (x - xb) mod 4.
-template <typename F>
-__device__ __forceinline__ void march4(int xb, int xe, F&& body) {
-    for (int x = xb; x <= xe; x += 4) {
+// body(Kc<k>{}, x) for x = xb .. xe (inclusive) with k = (x - xb) mod U, U = 4 or 8.
+template <int U, typename F>
+__device__ __forceinline__ void march_incl(int xb, int xe, F&& body) {
+    static_assert(U == 4 || U == 8, "unroll");
+    for (int x = xb; x <= xe; x += U) {
         body(Kc<0>{}, x);
         if (x + 1 > xe) break;
         body(Kc<1>{}, x + 1);
@@ -49,7 +50,30 @@ __device__ __forceinline__ void march4(int xb, int xe, F&& body) {
         body(Kc<2>{}, x + 2);
         if (x + 3 > xe) break;
         body(Kc<3>{}, x + 3);
+        if constexpr (U == 8) {
+            if (x + 4 > xe) break;
+            body(Kc<4 % U>{}, x + 4);
+            if (x + 5 > xe) break;
+            body(Kc<5 % U>{}, x + 5);
+            if (x + 6 > xe) break;
+            body(Kc<6 % U>{}, x + 6);
+            if (x + 7 > xe) break;
+            body(Kc<7 % U>{}, x + 7);
+        }
     }
+}
+
+// Planes of step n loaded ahead of the one being staged (1 or 2; the
+// coefficient ring holds x-1 .. x+2). A block holds one CU on its own, so its
+// waves must keep several planes of loads in flight; two planes fit the
+// registers of the float32 8-row tile (11 waves), one plane the others.
+template <typename T, int BY>
+constexpr int step2_dist() {
+#ifdef SW_STEP2_DIST
+    return SW_STEP2_DIST;
+#else
+    return (sizeof(T) == 4 && BY <= 8) ? 2 : 1;
+#endif
 }
 
 // Write a column's fields into plane L at [r][q].
@@ -119,35 +143,40 @@ __device__ __forceinline__ void march2_tile(const StepArgs<T>& a, int x0, int x1
         return c;
     };
 
-    // rings, slot (plane - xb) & 3 with xb = x0 - 1 the first iteration
-    Col<T> q[4];   // step n of this column: planes x-1 .. x+2
-    Coef<T> k[4];  // coefficients: planes x-1 .. x+1
+    // rings, slot (plane - xb) & (QN-1) (q) or & 3 (the others), xb = x0 - 1 the first iteration
+    constexpr int D = step2_dist<T, BY>();
+    static_assert(D >= 1 && D <= 2, "two-step prefetch distance");
+    constexpr int QN = D == 1 ? 4 : 8, QM = QN - 1;  // the loop is unrolled by QN
+    Col<T> q[QN];  // step n of this column: planes x-1 .. x+1+D
+    Coef<T> k[4];  // coefficients: planes x-1 .. x+D
     Col<T> s[4];   // step n+1 of this column: planes x-2 .. x
-    Col<T> o[4];   // step n at the outer point: planes x, x+1
+    Col<T> o[4];   // step n at the outer point: planes x .. x+D
     const int xb = x0 - 1;
-    q[3] = col(x0 - 2, pyz);
-    q[0] = col(x0 - 1, pyz);
-    q[1] = col(x0, pyz);
-    k[0] = coef(x0 - 1);
-    o[0] = outer_col(x0 - 1);
+#pragma unroll
+    for (int j = -1; j <= D; ++j) q[(j + QN) & QM] = col(x0 - 1 + j, pyz);  // planes x0-2 .. x0-1+D
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        k[j] = coef(x0 - 1 + j);
+        o[j] = outer_col(x0 - 1 + j);
+    }
 
-    march4(xb, x1, [&](auto kc, int x) {
+    march_incl<QN>(xb, x1, [&](auto kc, int x) {
         constexpr int K = decltype(kc)::value;
         constexpr int B = K & 1;
         auto& S0 = lds.s0[B];
         auto& S1 = lds.s1[B];
         // stage step n of plane x
         if (prim) stage<T, VISC>(S0, r0, q0, q[K]);
-        if (outer) stage<T, VISC>(S0, ro, qo, o[K]);
-        // prefetch: step n at x+2, coefficients and outer point at x+1
-        q[(K + 2) & 3] = col(x + 2, pyz);
-        k[(K + 1) & 3] = coef(x + 1);
-        o[(K + 1) & 3] = outer_col(x + 1);
+        if (outer) stage<T, VISC>(S0, ro, qo, o[K & 3]);
+        // prefetch: step n at x+1+D, coefficients and outer point at x+D
+        q[(K + 1 + D) & QM] = col(x + 1 + D, pyz);
+        k[(K + D) & 3] = coef(x + D);
+        o[(K + D) & 3] = outer_col(x + D);
         __syncthreads();
         // step n+1 at plane x on the tile and its ring
         {
             T i0, i1, i2;
-            point_inc<T, VISC, false>(a, q[(K + 3) & 3], q[K], q[(K + 1) & 3], S0, r0, q0, k[K], i0, i1, i2);
+            point_inc<T, VISC, false>(a, q[(K + QM) & QM], q[K], q[(K + 1) & QM], S0, r0, q0, k[K & 3], i0, i1, i2);
             const bool in = p_int && x > 0 && x < g.nx - 1;
             Col<T> n;
             n.u0 = in ? q[K].u0 + i0 : T(0);
@@ -156,7 +185,7 @@ __device__ __forceinline__ void march2_tile(const StepArgs<T>& a, int x0, int x1
             n.d0 = in ? i0 : T(0);
             n.d1 = in ? i1 : T(0);
             n.d2 = in ? i2 : T(0);
-            s[K] = n;
+            s[K & 3] = n;
             if (prim) stage<T, VISC>(S1, r1, q1, n);
         }
         // step n+2 at plane x-1 (staged in the other step-n+1 buffer one plane ago)
@@ -164,7 +193,7 @@ __device__ __forceinline__ void march2_tile(const StepArgs<T>& a, int x0, int x1
             auto& S1p = lds.s1[B ^ 1];
             const Col<T>& c = s[(K + 3) & 3];
             T i0, i1, i2;
-            point_inc<T, VISC, false>(a, s[(K + 2) & 3], c, s[K], S1p, r1, q1, k[(K + 3) & 3], i0, i1, i2);
+            point_inc<T, VISC, false>(a, s[(K + 2) & 3], c, s[K & 3], S1p, r1, q1, k[(K + 3) & 3], i0, i1, i2);
             const bool in = p_int && x - 1 > 0 && x - 1 < g.nx - 1;
             if (!in) i0 = i1 = i2 = T(0);
             const int64_t px = (int64_t)(x - 1) * P;

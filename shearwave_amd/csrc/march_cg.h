@@ -286,7 +286,11 @@ __device__ inline double cg_dir_tile(const double* __restrict__ r, const double*
         if (!t.inyz) return;
         const bool in = g.interior(x, t.y, t.z);
         if (in) s += c * cg_apply(g, x, t.y, t.z, a, c, e, lds[B], r0, q0, qx, qy, qz);
+#ifdef SW_CG_NT_P
+        store_out(&at_byte(plane_at(p, x, g), t.yzb), in ? c : 0.0);
+#else
         at_byte(plane_at(p, x, g), t.yzb) = in ? c : 0.0;
+#endif
     });
     return s;
 }
