@@ -41,8 +41,8 @@ extern "C" {
 #define SW_KERNEL_AUTO 0
 #define SW_KERNEL_DIRECT 1  /* one voxel per thread, neighbours through the caches */
 #define SW_KERNEL_MARCH 2   /* x-marching LDS tiles with register queues, one step per pass */
-#define SW_KERNEL_MARCH2 3  /* as MARCH, but two force-free increment-form steps per pass
-                               where no output falls between them (AUTO picks it) */
+#define SW_KERNEL_MARCH2 3  /* as MARCH, but two increment-form steps per pass wherever
+                               no output falls between them (AUTO picks it) */
 
 /* CG stop reasons */
 #define SW_CG_RUNNING 0
@@ -186,7 +186,7 @@ int sw_gather_traces(int dtype, const void* f, const int64_t* idx, int64_t count
  * 64-column tiles, and a device-side global pass counter gives each solve its
  * pass k = gpass - start (active while 0 <= k < n_passes; its state is read
  * from parity k&1 and written to the other parity). A pass is one step, or
- * two force-free steps when plan says so: plan[k] = (first local step << 1)
+ * two steps when plan says so: plan[k] = (first local step << 1)
  * | (1 if the pass is two steps). plan == NULL means one step per pass
  * (n_passes = n_steps, local step = k). Each solve's y-z plane (ny*nz) holds
  * fewer than 2^28 voxels: the kernels address a plane with 32-bit offsets.

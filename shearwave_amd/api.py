@@ -9,8 +9,8 @@ the reference's dtypes, or device tensors when asked for.
 Build-only options (namespaced ``sw_``):
   * ``sw_dtype``: storage type, ``"float32"`` (default, as the reference) or ``"float64"``;
   * ``sw_form``: ``"increment"`` (default) or ``"leapfrog"`` state storage;
-  * ``sw_kernel``: ``"auto"``, ``"march2"`` (two force-free steps per pass where
-    no output falls between them), ``"march"`` (one step per pass) or ``"direct"``
+  * ``sw_kernel``: ``"auto"``, ``"march2"`` (two steps per pass wherever no
+    output falls between them), ``"march"`` (one step per pass) or ``"direct"``
     time-step kernel; ``"auto"`` is ``"march2"`` on grids the march tiles fit;
   * ``sw_tile_y``, ``sw_tile_z``, ``sw_x_chunk``: march-kernel tile shape (0 = automatic);
   * ``sw_strain_stride``: with ``shear_fdtd(..., return_strain=True)``, the

@@ -4,9 +4,9 @@
 increment form), projects all their forces with one batched CG
 (``sw_batch_project``), and advances all active solves with one persistent
 kernel launch per pass (``sw_batch_step``), optionally replayed from a
-captured hipGraph (``sw_batch_graph_create``). A solve's pass is one time
-step, or two force-free steps in one read of its state (the two-step tile),
-as its pass plan (:func:`pass_plan`) says. Work items are (solve, z tile,
+captured hipGraph (``sw_batch_graph_create``). A solve's pass is two time
+steps in one read of its state (the two-step tile), or one step, as its pass
+plan (:func:`pass_plan`) says. Work items are (solve, z tile,
 y tile, x chunk) quadruples; when solves finish, the item list is rebuilt so
 finished solves cost nothing. A solve's arithmetic is the single-solve tile
 code with per-solve reductions in a fixed order, so its result does not
@@ -92,7 +92,7 @@ class _Solve:
         if env.shape[0] != spec.n_steps:
             raise ValueError("env length must equal n_steps")
         self.env = torch.from_numpy(np.ascontiguousarray(env)).to(dev)
-        plan = pass_plan(env, spec.n_steps, two_step)
+        plan = pass_plan(spec.n_steps, two_step)
         self.n_passes = int(plan.shape[0])
         self.plan = torch.from_numpy(plan).to(dev)
         self.start = 0  # global pass of the first pass
@@ -102,17 +102,17 @@ class _Solve:
         return n * 4 * (3 + 12 + 6) + self.spec.n_steps * 8
 
 
-def pass_plan(env, n_steps, two_step=True):
+def pass_plan(n_steps, two_step=True):
     """Passes covering steps 0 .. n_steps-1: entry (first step << 1) | 1 for a two-step pass.
 
-    Two consecutive steps share a pass when neither applies the force
-    (env == 0); forced steps, and a lone last step, are single passes.
+    With ``two_step`` consecutive steps are paired (the pass scales the force
+    by each step's own env value) and an odd last step is a single pass;
+    without it every pass is one step.
     """
-    env = np.zeros(n_steps) if env is None else np.asarray(env, dtype=np.float64)
     plan = []
     t = 0
     while t < n_steps:
-        if two_step and t + 1 < n_steps and env[t] == 0.0 and env[t + 1] == 0.0:
+        if two_step and t + 1 < n_steps:
             plan.append((t << 1) | 1)
             t += 2
         else:

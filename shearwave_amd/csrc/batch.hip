@@ -1,7 +1,7 @@
 // Batched solves on gfx950: many shear_fdtd solves of uneven grid sizes and
-// material maps advance in ONE persistent launch per pass (one step, or two
-// force-free steps, per solve as its pass plan says), and their projections
-// run as one batched CG.
+// material maps advance in ONE persistent launch per pass (two steps, or one,
+// per solve as its pass plan says), and their projections run as one batched
+// CG.
 //
 // Work items are (solve, z tile, y tile, x chunk) quadruples. A persistent
 // grid (one wave of resident blocks) walks the item list; each XCD takes a
@@ -90,17 +90,26 @@ __global__ void __launch_bounds__(64 * kBatchRows) batch_step_kernel(const sw_ba
         a.cx = S.cx, a.cy = S.cy, a.cz = S.cz;
         a.vx = S.vx, a.vy = S.vy, a.vz = S.vz;
         a.damp = S.damp;
-        a.fs = (!two && S.env && S.force[0]) ? (float)S.env[local] : 0.f;
+        const bool forcing = S.env && S.force[0];
+        a.fs = forcing ? (float)S.env[local] : 0.f;
+        a.fs2 = (forcing && two) ? (float)S.env[local + 1] : 0.f;
         const int x0 = it.w * S.xchunk;
         const int x1 = min(x0 + S.xchunk, S.nx);
         const int y0 = it.z * kBatchBY, z0 = it.y * 64;
-        const bool visc = a.e != nullptr, force = a.fs != 0.f;
+        const bool visc = a.e != nullptr, force = a.fs != 0.f || a.fs2 != 0.f;
         constexpr int B = kBatchBY, R = kBatchRows;
-        if (two) {  // the plan pairs force-free steps only
-            if (visc)
-                march2_tile<float, true, B>(a, x0, x1, y0, z0, lds.two_visc);
-            else
-                march2_tile<float, false, B>(a, x0, x1, y0, z0, lds.two_inviscid);
+        if (two) {
+            if (visc) {
+                if (force)
+                    march2_tile<float, true, true, B>(a, x0, x1, y0, z0, lds.two_visc);
+                else
+                    march2_tile<float, true, false, B>(a, x0, x1, y0, z0, lds.two_visc);
+            } else {
+                if (force)
+                    march2_tile<float, false, true, B>(a, x0, x1, y0, z0, lds.two_inviscid);
+                else
+                    march2_tile<float, false, false, B>(a, x0, x1, y0, z0, lds.two_inviscid);
+            }
         } else if (visc) {
             if (force)
                 march_tile<float, true, true, true, B, 64, 6, R>(a, x0, x1, y0, z0, lds.one);

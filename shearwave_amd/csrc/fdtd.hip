@@ -89,7 +89,7 @@ __global__ void __launch_bounds__(BZ * BY) step_march_kernel(StepArgs<T> a, int 
     march_tile<T, INCR, VISC, FORCE, BY, BZ>(a, x0, min(x0 + xchunk, a.g.nx), tyi * BY, tzi * BZ, lds);
 }
 
-template <typename T, bool VISC, int BY>
+template <typename T, bool VISC, bool FORCE, int BY>
 __global__ void __launch_bounds__(64 * (BY + 3)) step2_march_kernel(StepArgs<T> a, int xchunk, int nty, int ntz) {
     __shared__ Step2Lds<T, VISC, BY> lds;
     const unsigned nb = gridDim.x;
@@ -100,7 +100,7 @@ __global__ void __launch_bounds__(64 * (BY + 3)) step2_march_kernel(StepArgs<T> 
     const int tyi = (int)(rest % (unsigned)nty);
     const int txi = (int)(rest / (unsigned)nty);
     const int x0 = txi * xchunk;
-    march2_tile<T, VISC, BY>(a, x0, min(x0 + xchunk, a.g.nx), tyi * BY, tzi * 64, lds);
+    march2_tile<T, VISC, FORCE, BY>(a, x0, min(x0 + xchunk, a.g.nx), tyi * BY, tzi * 64, lds);
 }
 
 struct MarchShape {
@@ -174,43 +174,48 @@ int launch_step(const StepArgs<T>& a, int kernel, int tile_y, int tile_z, int x_
     return SW_OK;
 }
 
-template <typename T, bool VISC, int BY>
+template <typename T, bool VISC, bool FORCE, int BY>
 void launch_march2(const StepArgs<T>& a, int x_chunk, hipStream_t s) {
-    auto k = step2_march_kernel<T, VISC, BY>;
+    auto k = step2_march_kernel<T, VISC, FORCE, BY>;
     static const int res = resident_blocks(k, 64 * (BY + 3));
     const MarchShape ms = march_shape(a.g, BY, 64, x_chunk, res);
     hipLaunchKernelGGL(k, dim3(ms.blocks()), dim3(64, BY + 3), 0, s, a, ms.xchunk, ms.nty, ms.ntz);
 }
 
-// Two force-free increment-form steps in one pass (u_n, d_n -> u_{n+2}, d_{n+2}).
-// Tile rows: float32 8 (default) or 12; float64 4 (a 7-wave block leaves each
-// wave the registers the float64 queues need).
-template <typename T>
-int launch_step2(const StepArgs<T>& a, bool visc, int tile_y, int x_chunk, hipStream_t s) {
+// Two increment-form steps in one pass (u_n, d_n -> u_{n+2}, d_{n+2}); the
+// force, when either step applies it, is scaled by a.fs and a.fs2. Tile rows:
+// float32 8 (default) or 12 (force-free passes; forced ones use 8); float64 4
+// (a 7-wave block leaves each wave the registers the float64 queues need).
+template <typename T, bool VISC, bool FORCE>
+int launch_step2_rows(const StepArgs<T>& a, int tile_y, int x_chunk, hipStream_t s) {
     if constexpr (DType<T>::code == SW_F32) {
         const int by = tile_y > 0 ? tile_y : 8;
-        if (by == 8) {
-            if (visc) launch_march2<T, true, 8>(a, x_chunk, s);
-            else launch_march2<T, false, 8>(a, x_chunk, s);
+        if (by == 8 || (by == 12 && FORCE)) {
+            launch_march2<T, VISC, FORCE, 8>(a, x_chunk, s);
         } else if (by == 12) {
-            if (visc) launch_march2<T, true, 12>(a, x_chunk, s);
-            else launch_march2<T, false, 12>(a, x_chunk, s);
+            if constexpr (!FORCE) launch_march2<T, VISC, false, 12>(a, x_chunk, s);
         } else {
             sw::set_error("two-step pass: float32 tile_y %d not supported (8 or 12)", by);
             return SW_ERR_ARG;
         }
     } else {
         const int by = tile_y > 0 ? tile_y : 4;
-        if (by == 4) {
-            if (visc) launch_march2<T, true, 4>(a, x_chunk, s);
-            else launch_march2<T, false, 4>(a, x_chunk, s);
-        } else {
+        if (by != 4) {
             sw::set_error("two-step pass: float64 tile_y %d not supported (4)", by);
             return SW_ERR_ARG;
         }
+        launch_march2<T, VISC, FORCE, 4>(a, x_chunk, s);
     }
     SW_LAUNCH_CHECK();
     return SW_OK;
+}
+
+template <typename T>
+int launch_step2(const StepArgs<T>& a, bool visc, bool force, int tile_y, int x_chunk, hipStream_t s) {
+    if (visc) return force ? launch_step2_rows<T, true, true>(a, tile_y, x_chunk, s)
+                           : launch_step2_rows<T, true, false>(a, tile_y, x_chunk, s);
+    return force ? launch_step2_rows<T, false, true>(a, tile_y, x_chunk, s)
+                 : launch_step2_rows<T, false, false>(a, tile_y, x_chunk, s);
 }
 
 template <typename T>
@@ -242,8 +247,8 @@ int run_steps(sw_fdtd_state* st, int64_t n_steps, const double* env, const sw_fd
     if (kernel == SW_KERNEL_AUTO) kernel = march_fits ? SW_KERNEL_MARCH2 : SW_KERNEL_DIRECT;
     SW_REQUIRE(kernel == SW_KERNEL_DIRECT || g.plane() < kMaxMarchPlane, SW_ERR_ARG,
                "march kernels need a y-z plane of fewer than %lld voxels", (long long)kMaxMarchPlane);
-    // two-step passes (increment form, force-free, no output in between); the
-    // remaining single steps of such a run use the one-step march kernel
+    // two-step passes (increment form, no output in between); the remaining
+    // single steps of such a run use the one-step march kernel
     const bool pairs = kernel == SW_KERNEL_MARCH2 && incr;
     const int one_kernel = kernel == SW_KERNEL_MARCH2 ? SW_KERNEL_MARCH : kernel;
     // tile_y names the two-step tile in a two-step run; its single steps use the default tile
@@ -273,18 +278,20 @@ int run_steps(sw_fdtd_state* st, int64_t n_steps, const double* env, const sw_fd
 
     for (int64_t it = 0; it < n_steps;) {
         const double fs = env ? env[it] : 0.0;
-        const bool force = fs != 0.0;
-        SW_REQUIRE(!force || have_force, SW_ERR_ARG, "env[%lld] != 0 but the state has no force", (long long)it);
-        const bool two = pairs && !force && it + 1 < n_steps && (!env || env[it + 1] == 0.0) &&
-                         !output_after(step0 + it);
+        const bool two = pairs && it + 1 < n_steps && !output_after(step0 + it);
+        const double fs2 = (two && env) ? env[it + 1] : 0.0;
+        const bool force = fs != 0.0 || fs2 != 0.0;
+        SW_REQUIRE(!force || have_force, SW_ERR_ARG, "env[%lld] != 0 but the state has no force",
+                   (long long)(fs != 0.0 ? it : it + 1));
         a.fs = (T)fs;
+        a.fs2 = (T)fs2;
         for (int c = 0; c < 3; ++c) {
             a.A[c] = (const T*)st->u[c];
             a.B[c] = (const T*)st->b[c];
             a.Ao[c] = (T*)st->u_next[c];
             a.Bo[c] = (T*)st->b_next[c];
         }
-        int rc = two ? launch_step2<T>(a, visc, st->tile_y, st->x_chunk, s)
+        int rc = two ? launch_step2<T>(a, visc, force, st->tile_y, st->x_chunk, s)
                      : launch_step_any<T>(a, incr, visc, force, one_kernel, one_tile_y, st->tile_z, st->x_chunk, s);
         if (rc) return rc;
         for (int c = 0; c < 3; ++c) {

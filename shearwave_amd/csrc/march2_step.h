@@ -1,8 +1,8 @@
 // Two time steps per pass over the state (temporal blocking of the loop in
-// shear_fdtd, solver.py:194-234): one x-marching tile reads u_n, d_n and the
-// coefficients once and writes u_{n+2}, d_{n+2}, halving the HBM traffic per
-// voxel-step of the single-step tile. Increment form, no force (the pass is
-// used for the force-free steps between outputs).
+// shear_fdtd, solver.py:194-234): one x-marching tile reads u_n, d_n, the
+// coefficients (and the force, when either step applies it) once and writes
+// u_{n+2}, d_{n+2}, halving the HBM traffic per voxel-step of the single-step
+// tile. Increment form; step n+1 scales the force by a.fs, step n+2 by a.fs2.
 //
 // Tile: output columns (y0 .. y0+BY-1, z0 .. z0+63). Step n+2 at a point
 // needs step n+1 at its six neighbours, so step n+1 is evaluated on the tile
@@ -67,12 +67,12 @@ __device__ __forceinline__ void march_incl(int xb, int xe, F&& body) {
 // coefficient ring holds x-1 .. x+2). A block holds one CU on its own, so its
 // waves must keep several planes of loads in flight; two planes fit the
 // registers of the float32 8-row tile (11 waves), one plane the others.
-template <typename T, int BY>
+template <typename T, int BY, bool FORCE>
 constexpr int step2_dist() {
 #ifdef SW_STEP2_DIST
     return SW_STEP2_DIST;
 #else
-    return (sizeof(T) == 4 && BY <= 8) ? 2 : 1;
+    return (sizeof(T) == 4 && BY <= 8 && !FORCE) ? 2 : 1;
 #endif
 }
 
@@ -89,7 +89,7 @@ __device__ __forceinline__ void stage(Plane& L, int r, int q, const Col<T>& c) {
     }
 }
 
-template <typename T, bool VISC, int BY>
+template <typename T, bool VISC, bool FORCE, int BY>
 __device__ __forceinline__ void march2_tile(const StepArgs<T>& a, int x0, int x1, int y0, int z0,
                                             Step2Lds<T, VISC, BY>& lds) {
     static_assert(2 * BY + 4 <= 64 && BY + 3 <= 16, "tile rows");
@@ -136,7 +136,9 @@ __device__ __forceinline__ void march2_tile(const StepArgs<T>& a, int x0, int x1
 
     auto po = [&](int xx) { return (int64_t)min(max(xx, 0), g.nx - 1) * P; };
     auto col = [&](int xx, uint32_t o) { return load_col<T, true>(a, po(xx), o); };
-    auto coef = [&](int xx) { return load_coef<T, VISC, false>(a, po(xx), pyz); };
+    auto coef = [&](int xx) { return load_coef<T, VISC, FORCE>(a, po(xx), pyz); };
+    StepArgs<T> a2 = a;  // step n+2: the second step's force scale
+    a2.fs = a.fs2;
     auto outer_col = [&](int xx) {
         Col<T> c{};
         if (outer) c = col(xx, oyz);
@@ -144,7 +146,7 @@ __device__ __forceinline__ void march2_tile(const StepArgs<T>& a, int x0, int x1
     };
 
     // rings, slot (plane - xb) & (QN-1) (q) or & 3 (the others), xb = x0 - 1 the first iteration
-    constexpr int D = step2_dist<T, BY>();
+    constexpr int D = step2_dist<T, BY, FORCE>();
     static_assert(D >= 1 && D <= 2, "two-step prefetch distance");
     constexpr int QN = D == 1 ? 4 : 8, QM = QN - 1;  // the loop is unrolled by QN
     Col<T> q[QN];  // step n of this column: planes x-1 .. x+1+D
@@ -176,7 +178,7 @@ __device__ __forceinline__ void march2_tile(const StepArgs<T>& a, int x0, int x1
         // step n+1 at plane x on the tile and its ring
         {
             T i0, i1, i2;
-            point_inc<T, VISC, false>(a, q[(K + QM) & QM], q[K], q[(K + 1) & QM], S0, r0, q0, k[K & 3], i0, i1, i2);
+            point_inc<T, VISC, FORCE>(a, q[(K + QM) & QM], q[K], q[(K + 1) & QM], S0, r0, q0, k[K & 3], i0, i1, i2);
             const bool in = p_int && x > 0 && x < g.nx - 1;
             Col<T> n;
             n.u0 = in ? q[K].u0 + i0 : T(0);
@@ -193,7 +195,7 @@ __device__ __forceinline__ void march2_tile(const StepArgs<T>& a, int x0, int x1
             auto& S1p = lds.s1[B ^ 1];
             const Col<T>& c = s[(K + 3) & 3];
             T i0, i1, i2;
-            point_inc<T, VISC, false>(a, s[(K + 2) & 3], c, s[K & 3], S1p, r1, q1, k[(K + 3) & 3], i0, i1, i2);
+            point_inc<T, VISC, FORCE>(a2, s[(K + 2) & 3], c, s[K & 3], S1p, r1, q1, k[(K + 3) & 3], i0, i1, i2);
             const bool in = p_int && x - 1 > 0 && x - 1 < g.nx - 1;
             if (!in) i0 = i1 = i2 = T(0);
             const int64_t px = (int64_t)(x - 1) * P;

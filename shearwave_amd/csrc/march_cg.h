@@ -207,15 +207,22 @@ __device__ inline double cg_apply(const Grid& g, int x, int y, int z, double a, 
     return -(tx * qx + ty * qy + tz * qz);
 }
 
-// Prefetch distance of the CG tiles, in x planes: a plane's loads are issued
+// Prefetch distances of the CG tiles, in x planes: a plane's loads are issued
 // this many iterations before the one that consumes them, and nothing is
 // computed on a loaded value before then, so no wait for memory falls inside
-// the iteration that issued it (1..3: the rings below hold four planes).
-#ifndef SW_CG_DIST
-#define SW_CG_DIST 2
+// the iteration that issued it. One 16-wave block holds a CU, so deeper rings
+// keep more loads in flight. The dir tile's rings hold eight planes (1..7),
+// the step tile's p ring eight planes from x-2 (1..3).
+#ifndef SW_CG_DIST_DIR
+#define SW_CG_DIST_DIR 3
 #endif
+#ifndef SW_CG_DIST
+#define SW_CG_DIST 3
+#endif
+constexpr int kCgDistDir = SW_CG_DIST_DIR;
 constexpr int kCgDist = SW_CG_DIST;
-static_assert(kCgDist >= 1 && kCgDist <= 3, "CG prefetch distance");
+static_assert(kCgDistDir >= 1 && kCgDistDir <= 7, "CG dir prefetch distance");
+static_assert(kCgDist >= 1 && kCgDist <= 3, "CG step prefetch distance");
 
 template <int K>
 using Ic = std::integral_constant<int, K>;
@@ -255,33 +262,40 @@ __device__ inline double cg_dir_tile(const double* __restrict__ r, const double*
                                      double* __restrict__ p, const Grid& g, double beta, double qx, double qy,
                                      double qz, const CgTile& t, CgLds& lds) {
     const int r0 = threadIdx.y + 2, q0 = threadIdx.x + 2;
+#ifdef SW_CG_NT_DIR
+    auto raw = [&](int xx, uint32_t o) {
+        return CgRaw{load_once(&at_byte(plane_at(r, xx, g), o)), load_once(&at_byte(plane_at(p_old, xx, g), o))};
+    };
+#else
     auto raw = [&](int xx, uint32_t o) { return CgRaw{at_byte(plane_at(r, xx, g), o), at_byte(plane_at(p_old, xx, g), o)}; };
+#endif
     auto center = [&](int xx) { return raw(xx, t.yzb); };
     // halo loads are unconditional (lanes without a halo point read offset 0),
     // so no branch separates a load from the wait that precedes its use
     auto halo = [&](int xx) { return raw(xx, t.hyzb); };
     auto pv = [&](const CgRaw& v) { return fma(beta, v.q, v.r); };
     const int x0 = t.x0, x1 = t.x1;
+    constexpr int D = kCgDistDir;
     double pq[8];  // p of this column at plane x0 + k in slot k & 7 (planes x-2 .. x+2 live)
-    CgRaw rq[4];   // raw loads of this column in flight, slot k & 3 (planes x+3 .. x+2+D)
-    CgRaw hq[4];   // raw loads of this thread's halo point, slot k & 3 (planes x .. x+D)
+    CgRaw rq[8];   // raw loads of this column in flight, slot k & 7 (planes x+3 .. x+2+D)
+    CgRaw hq[8];   // raw loads of this thread's halo point, slot k & 7 (planes x .. x+D)
 #pragma unroll
     for (int k = -2; k < 2; ++k) pq[k & 7] = pv(center(x0 + k));
 #pragma unroll
-    for (int k = 2; k < 2 + kCgDist; ++k) rq[k & 3] = center(x0 + k);
+    for (int k = 2; k < 2 + D; ++k) rq[k & 7] = center(x0 + k);
 #pragma unroll
-    for (int k = 0; k < kCgDist; ++k) hq[k & 3] = halo(x0 + k);
+    for (int k = 0; k < D; ++k) hq[k & 7] = halo(x0 + k);
     double s = 0.0;
     march8(x0, x1, [&](auto kc, int x) {
         constexpr int K = decltype(kc)::value;
         constexpr int B = K & 1;
-        const double e = pv(rq[(K + 2) & 3]);  // plane x+2, loaded D iterations ago
+        const double e = pv(rq[(K + 2) & 7]);  // plane x+2, loaded D iterations ago
         pq[(K + 2) & 7] = e;
         const double a = pq[(K + 6) & 7], c = pq[K & 7];
         lds[B][r0][q0] = c;
-        if (t.halo_thread) lds[B][t.ly][t.lz] = pv(hq[K & 3]);
-        rq[(K + 2 + kCgDist) & 3] = center(x + 2 + kCgDist);
-        hq[(K + kCgDist) & 3] = halo(x + kCgDist);
+        if (t.halo_thread) lds[B][t.ly][t.lz] = pv(hq[K & 7]);
+        rq[(K + 2 + D) & 7] = center(x + 2 + D);
+        hq[(K + D) & 7] = halo(x + D);
         __syncthreads();
         if (!t.inyz) return;
         const bool in = g.interior(x, t.y, t.z);
@@ -303,8 +317,13 @@ __device__ inline double cg_step_tile(double* __restrict__ phi, double* __restri
                                       const Grid& g, double alpha, double qx, double qy, double qz, const CgTile& t,
                                       CgLds& lds) {
     const int r0 = threadIdx.y + 2, q0 = threadIdx.x + 2;
+#ifdef SW_CG_NT_STEP_P
+    auto center = [&](int xx) { return load_once(&at_byte(plane_at(p, xx, g), t.yzb)); };
+    auto halo = [&](int xx) { return load_once(&at_byte(plane_at(p, xx, g), t.hyzb)); };
+#else
     auto center = [&](int xx) { return at_byte(plane_at(p, xx, g), t.yzb); };
     auto halo = [&](int xx) { return at_byte(plane_at(p, xx, g), t.hyzb); };
+#endif
     auto once = [&](double* f, int xx) { return load_once(&at_byte(plane_at(f, xx, g), t.yzb)); };
     const int x0 = t.x0, x1 = t.x1;
     double pq[8];  // p of this column, slot k & 7 (planes x-2 .. x+2+D live)

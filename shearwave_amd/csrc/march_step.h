@@ -23,6 +23,7 @@ struct StepArgs {
     T vx, vy, vz;  // dT / h^2 per axis
     T damp;        // 1 - vel_damp
     T fs;          // env[it]
+    T fs2;         // env[it+1] (the second step of a two-step pass)
 };
 
 // One (y, z) column's state at one x plane: u and d = u - u_prev.
@@ -89,7 +90,7 @@ __device__ __forceinline__ T comp_inc(const StepArgs<T>& a, T u0, T um, T up, T 
             fma(a.vx, fma(T(-2), d0, dp + dm), fma(a.vy, fma(T(-2), d0, dyp + dym), a.vz * fma(T(-2), d0, dzp + dzm)));
         inc = fma(e, ld, inc);
     }
-    if (FORCE) inc = inc + f;
+    if (FORCE && a.fs != T(0)) inc = inc + f;  // a step with env = 0 adds nothing, as the force-free kernel
     return inc * a.damp;
 }
 

@@ -7,7 +7,7 @@ one batched projection of every request admitted together) runs on a second
 stream, so the solves already on the GPU keep stepping meanwhile. Admitted
 solves join the running batch at the next tick: every tick is `tick_passes`
 global passes of ONE persistent launch per pass over all active solves, each
-at its own pass (one step, or two force-free steps in one read of its state). Finished solves are exported (u, v in the reference's
+at its own pass (two steps in one read of its state, or one step). Finished solves are exported (u, v in the reference's
 AoS layout) into pinned host memory; latency is arrival -> result on the host.
 
 A solve's arithmetic is the same whichever solves share its ticks (per-solve

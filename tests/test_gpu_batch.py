@@ -116,7 +116,7 @@ def test_batch_agrees_with_single_solve_dropin(sw, specs, mixed):
 
 
 def test_two_step_passes_equal_single_steps(sw, specs, mixed):
-    """Pass plans that pair force-free steps give the same bits as one step per pass."""
+    """Pass plans that pair steps (forced ones included) give the same bits as one step per pass."""
     from shearwave_amd.batch import Batch
 
     b = Batch(specs, two_step=False)

@@ -1,4 +1,4 @@
-"""The two-step march pass (two force-free steps per read of the state) against single steps.
+"""The two-step march pass (two steps per read of the state) against single steps.
 
 A pass evaluates each step with the same operations as the single-step
 kernel (contraction off, fused multiply-adds written out), so float32 and
@@ -79,14 +79,16 @@ def test_two_step_tile_shapes(sw, tile_y, x_chunk):
         assert torch.equal(a, b)
 
 
-def test_two_step_around_force_and_outputs(sw):
-    """Forced steps, snapshots every 3 and strain every 4 steps break the pairing; outputs are identical."""
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_two_step_around_force_and_outputs(sw, dtype):
+    """Passes over forced and ramped steps; snapshots every 3 and strain every 4 steps break the pairing."""
     shape = (18, 16, 70)
     p = _problem(shape, seed=21, n_steps=48)  # a multiple of the snapshot stride (solver.py:469-507)
     kw = dict(snapshot_stride=3, return_strain=True)
-    ref = _run_gpu(sw, p, shape, {"sw_kernel": "march", "sw_strain_stride": 4}, **kw)
-    got = _run_gpu(sw, p, shape, {"sw_kernel": "march2", "sw_strain_stride": 4}, **kw)
-    auto = _run_gpu(sw, p, shape, {"sw_strain_stride": 4}, **kw)
+    o = {"sw_strain_stride": 4, "sw_dtype": dtype}
+    ref = _run_gpu(sw, p, shape, {**o, "sw_kernel": "march"}, **kw)
+    got = _run_gpu(sw, p, shape, {**o, "sw_kernel": "march2"}, **kw)
+    auto = _run_gpu(sw, p, shape, o, **kw)
     assert len(ref) == 4  # u, v, snapshots, strain dict
     for a, b, c in zip(_flat(got), _flat(ref), _flat(auto)):
         np.testing.assert_array_equal(a, b)

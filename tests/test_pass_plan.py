@@ -10,26 +10,19 @@ def _steps(plan):
     out = []
     for e in plan:
         t, two = int(e) >> 1, int(e) & 1
-        out += [(t, two)] + ([(t + 1, two)] if two else [])
+        out += [t] + ([t + 1] if two else [])
     return out
 
 
 @pytest.mark.parametrize("n", [0, 1, 2, 7, 10])
-def test_plan_covers_every_step_once_in_order(n):
-    env = np.zeros(n)
-    env[: n // 3] = 1.0
-    plan = pass_plan(env, n)
-    steps = _steps(plan)
-    assert [t for t, _ in steps] == list(range(n))
-    for t, two in steps:
-        if two:
-            assert env[t] == 0.0  # forced steps are never paired
+@pytest.mark.parametrize("two_step", [True, False])
+def test_plan_covers_every_step_once_in_order(n, two_step):
+    plan = pass_plan(n, two_step)
+    assert _steps(plan) == list(range(n))
     assert plan.dtype == np.int32
+    assert len(plan) == ((n + 1) // 2 if two_step else n)
 
 
-def test_plan_pairs_force_free_steps():
-    env = np.array([1.0, 0.5, 0, 0, 0, 0, 0, 1.0, 0, 0, 0])
-    plan = pass_plan(env, len(env))
-    assert [(int(e) >> 1, int(e) & 1) for e in plan] == [(0, 0), (1, 0), (2, 1), (4, 1), (6, 0), (7, 0), (8, 1), (10, 0)]
-    assert len(pass_plan(None, 9)) == 5
-    assert all(int(e) & 1 == 0 for e in pass_plan(None, 9, two_step=False))
+def test_plan_pairs_steps():
+    assert [(int(e) >> 1, int(e) & 1) for e in pass_plan(5)] == [(0, 1), (2, 1), (4, 0)]
+    assert all(int(e) & 1 == 0 for e in pass_plan(9, two_step=False))
