@@ -10,8 +10,9 @@ seconds per solve, projection and strain included).
 The JSON line also carries the roofline of the dominant kernel (in float32
 the two-step pass, which advances two force-free viscous steps per launch;
 timed with HIP events on the stream it runs on; algorithmic bytes per launch
-= 56 B per voxel-step x 2 steps x voxels) and a CPU baseline (the NumPy
-oracle in float64 on a bounded sample of the same loop, one core).
+= 44 B per voxel-step, the reference's (u, u_prev) traffic of one step, x 2
+steps x voxels) and a CPU baseline (the NumPy oracle in float64 on a bounded
+sample of the same loop, one core).
 
     python bench.py [--steps K] [--warmup W] [--n 256] [--nt 2000] [--no-cpu] [--fp64]
 """
@@ -420,7 +421,7 @@ def main():
     two = args.dtype == "float32" and args.kernel in ("auto", "march2")
     spl = 2 if two else 1
     kt = time_step_kernel(s, args.kernel_steps // spl, spl)
-    per_voxel_step = W.step_bytes(item, "increment", viscous=True, forced=False)
+    per_voxel_step = W.alg_step_bytes(item, viscous=True, forced=False)  # 44 B (fp32) / 88 B (fp64)
     alg_bytes = per_voxel_step * spl * vox  # algorithmic bytes per launch
     achieved = alg_bytes / kt / 1e9
     traffic = traffic_from_profile() if (args.n == 256 and args.dtype == "float32") else None
@@ -460,6 +461,7 @@ def main():
             "alg_bytes_per_voxel_step": per_voxel_step,
             "voxel_steps_per_launch": spl * vox,
             "avg_launch_us": round(kt * 1e6, 2),
+            "traffic_GBs": None if traffic is None else round(traffic / kt / 1e9, 1),
         },
         "phases": {
             "projection_s": round(t_proj, 4),
@@ -477,7 +479,7 @@ def main():
         torch.cuda.synchronize()
         t64 = time.perf_counter() - t0
         kt64 = time_step_kernel(b64.solve, args.kernel_steps // 2, 1)
-        ab64 = W.step_bytes(8, "increment", viscous=True, forced=False) * vox
+        ab64 = W.alg_step_bytes(8, viscous=True, forced=False) * vox
         rec["fp64"] = {"value": round(vox * w["n_steps"] / t64 / 1e9, 3), "ms_per_solve": round(t64 * 1e3, 2),
                        "step_kernel_GBs": round(ab64 / kt64 / 1e9, 1), "frac": round(ab64 / kt64 / 1e9 / HBM_PEAK_GBS, 4)}
         del b64

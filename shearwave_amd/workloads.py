@@ -82,6 +82,21 @@ def dropin_256(n=256, seed=2, n_steps=2000):
 
 
 # algorithmic HBM bytes per voxel of each kernel (read + write, once each)
+def alg_step_bytes(itemsize, viscous=True, forced=False):
+    """Algorithmic bytes of one time step per voxel, in the reference's (u, u_prev) form.
+
+    Read u, u_prev (3 components each), mu/rho, eta/rho and, on a force-active
+    step, f/rho; write u_next: 44 B in float32, 88 B in float64, +12/24 B when
+    forced, -4/8 B when inviscid. This is the unit of the roofline figure
+    (bytes per voxel-step), whatever form a kernel stores.
+    """
+    vec = 3 * itemsize
+    b = 2 * vec + vec + itemsize * (2 if viscous else 1)
+    if forced:
+        b += vec
+    return b
+
+
 def step_bytes(itemsize, form="increment", viscous=True, forced=False):
     """Time step: u, b read (stencil, counted once), u' (and d') written, mu/rho (and eta/rho) read."""
     vec = 3 * itemsize
@@ -95,6 +110,7 @@ def step_bytes(itemsize, form="increment", viscous=True, forced=False):
 
 
 CG_BYTES_PER_ITER = 8 * 8  # dir: r, p_old read, p written; step: phi, r, p read, phi, r written (float64)
+CG_TEXTBOOK_BYTES_PER_ITER = 11 * 8  # the three-kernel form: A p, p.Ap, phi and r updates, r.r (float64)
 
 
 def batch_48(seed=3, n_solves=48, h=0.5e-3):
