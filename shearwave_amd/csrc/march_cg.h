@@ -262,13 +262,8 @@ __device__ inline double cg_dir_tile(const double* __restrict__ r, const double*
                                      double* __restrict__ p, const Grid& g, double beta, double qx, double qy,
                                      double qz, const CgTile& t, CgLds& lds) {
     const int r0 = threadIdx.y + 2, q0 = threadIdx.x + 2;
-#ifdef SW_CG_NT_DIR
-    auto raw = [&](int xx, uint32_t o) {
-        return CgRaw{load_once(&at_byte(plane_at(r, xx, g), o)), load_once(&at_byte(plane_at(p_old, xx, g), o))};
-    };
-#else
+    // plain loads: these lines also serve the halos of the neighbouring tiles
     auto raw = [&](int xx, uint32_t o) { return CgRaw{at_byte(plane_at(r, xx, g), o), at_byte(plane_at(p_old, xx, g), o)}; };
-#endif
     auto center = [&](int xx) { return raw(xx, t.yzb); };
     // halo loads are unconditional (lanes without a halo point read offset 0),
     // so no branch separates a load from the wait that precedes its use
@@ -300,11 +295,7 @@ __device__ inline double cg_dir_tile(const double* __restrict__ r, const double*
         if (!t.inyz) return;
         const bool in = g.interior(x, t.y, t.z);
         if (in) s += c * cg_apply(g, x, t.y, t.z, a, c, e, lds[B], r0, q0, qx, qy, qz);
-#ifdef SW_CG_NT_P
-        store_out(&at_byte(plane_at(p, x, g), t.yzb), in ? c : 0.0);
-#else
         at_byte(plane_at(p, x, g), t.yzb) = in ? c : 0.0;
-#endif
     });
     return s;
 }
@@ -317,13 +308,8 @@ __device__ inline double cg_step_tile(double* __restrict__ phi, double* __restri
                                       const Grid& g, double alpha, double qx, double qy, double qz, const CgTile& t,
                                       CgLds& lds) {
     const int r0 = threadIdx.y + 2, q0 = threadIdx.x + 2;
-#ifdef SW_CG_NT_STEP_P
-    auto center = [&](int xx) { return load_once(&at_byte(plane_at(p, xx, g), t.yzb)); };
-    auto halo = [&](int xx) { return load_once(&at_byte(plane_at(p, xx, g), t.hyzb)); };
-#else
     auto center = [&](int xx) { return at_byte(plane_at(p, xx, g), t.yzb); };
     auto halo = [&](int xx) { return at_byte(plane_at(p, xx, g), t.hyzb); };
-#endif
     auto once = [&](double* f, int xx) { return load_once(&at_byte(plane_at(f, xx, g), t.yzb)); };
     const int x0 = t.x0, x1 = t.x1;
     double pq[8];  // p of this column, slot k & 7 (planes x-2 .. x+2+D live)
