@@ -58,6 +58,7 @@ def parse():
     ap.add_argument("--serve-rates", default="20,100")
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation run")
+    ap.add_argument("--no-radforce", action="store_true", help="skip the radiation-force run")
     ap.add_argument("--config1", action="store_true", help="only run the 64^3 CPU-vs-GPU comparison and exit")
     return ap.parse_args()
 
@@ -267,6 +268,98 @@ def bench_serve(rates=(20.0, 100.0), n_requests=24, tick_passes=10):
     best = max(r["solves_per_s"] for r in out["rates"])
     out["best_served_over_plain"] = round(best / plain_rate, 3)
     return out
+
+
+def bench_radforce(shape=(160, 160, 160), nt=256, on=(64, 192), reps=5, cpu_shape=(48, 48, 48)):
+    """Row f3: the acoustic radiation force over a resident float32 pressure record (nx, ny, nz, nt).
+
+    Times the two kernels with HIP events on their stream (the auto-gate's
+    energy pass over the whole record, the force pass over the gated
+    samples) and the whole drop-in call on device tensors. Algorithmic bytes:
+    energy = the record once; force = the gated samples once, c, rho, alpha
+    read and b written once. The CPU baseline is the float32 oracle on a
+    `cpu_shape` block of the same record, one core.
+    """
+    import torch
+
+    import shearwave_amd as sw
+    from shearwave_amd import api
+    from shearwave_amd import engine as E
+    from shearwave_amd._abi import check, lib
+
+    n_vox = int(np.prod(shape))
+    dev = E.device()
+    g = torch.Generator(device=dev).manual_seed(7)
+    p = torch.randn((n_vox, nt), generator=g, device=dev, dtype=torch.float32)
+    t = torch.arange(nt, device=dev)
+    p *= torch.where((t >= on[0]) & (t < on[1]), 1.0e5, 2.0e3).to(torch.float32)
+    c = 1450.0 + 150.0 * torch.rand(shape, generator=g, device=dev, dtype=torch.float32)
+    rho = 950.0 + 150.0 * torch.rand(shape, generator=g, device=dev, dtype=torch.float32)
+    alpha = 0.3 + 0.9 * torch.rand(shape, generator=g, device=dev, dtype=torch.float32)
+    out = torch.empty(shape, dtype=torch.float32, device=dev)
+    env = torch.empty(nt, dtype=torch.float64, device=dev)
+    need = int(lib().sw_pressure_energy_workspace(n_vox, nt))
+    work = torch.empty(need // 8 + 1, dtype=torch.float64, device=dev)
+    st = E.stream_ptr()
+
+    def energy():
+        check(lib().sw_pressure_energy(0, p.data_ptr(), n_vox, nt, env.data_ptr(), work.data_ptr(), need, st))
+
+    gate = api._auto_gate(p, n_vox, nt, "float32")
+    gidx, g0, n_gate = api._gate_args(gate)
+
+    def force():
+        check(lib().sw_radiation_force(0, p.data_ptr(), n_vox, nt, None if gidx is None else gidx.data_ptr(), g0,
+                                       n_gate, c.data_ptr(), rho.data_ptr(), alpha.data_ptr(), 1.0e6, out.data_ptr(),
+                                       st))
+
+    def timed(fn):
+        fn()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        b.synchronize()
+        return a.elapsed_time(b) / 1e3 / reps
+
+    t_e = timed(energy)
+    t_f = timed(force)
+    p4 = p.view(shape + (nt,))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sw.compute_radiation_force(p4, c, rho, alpha, 1.0e6)
+    torch.cuda.synchronize()
+    t_call = (time.perf_counter() - t0) / reps
+    e_bytes = n_vox * nt * 4
+    f_bytes = n_vox * (len(gate) * 4 + 3 * 4 + 4)
+    # CPU baseline: the float32 oracle on a block of the same record
+    from oracle import radiation_force as R
+
+    cn = int(np.prod(cpu_shape))
+    pc = p[:cn].cpu().numpy().reshape(cpu_shape + (nt,))
+    cc, rc_, ac = (m.flatten()[:cn].cpu().numpy().reshape(cpu_shape) for m in (c, rho, alpha))
+    t0 = time.perf_counter()
+    R.compute_radiation_force(pc, cc, rc_, ac, 1.0e6)
+    t_cpu = time.perf_counter() - t0
+    del p, p4, work
+    torch.cuda.empty_cache()
+    return {
+        "workload": f"pressure record {shape[0]}x{shape[1]}x{shape[2]} x nt={nt} float32 ({e_bytes / 1e9:.2f} GB), "
+                    f"burst in samples {on[0]}..{on[1] - 1}, auto-gate, f0 = 1 MHz, resident in HBM",
+        "gate_samples": int(len(gate)),
+        "energy_pass": {"us": round(t_e * 1e6, 1), "GBs": round(e_bytes / t_e / 1e9, 1),
+                        "frac": round(e_bytes / t_e / 1e9 / HBM_PEAK_GBS, 4)},
+        "force_pass": {"us": round(t_f * 1e6, 1), "GBs": round(f_bytes / t_f / 1e9, 1),
+                       "frac": round(f_bytes / t_f / 1e9 / HBM_PEAK_GBS, 4)},
+        "dropin_call_ms": round(t_call * 1e3, 3),
+        "gvoxel_samples_per_s": round(n_vox * nt / t_call / 1e9, 2),
+        "cpu_baseline": {"gvoxel_samples_per_s": round(cn * nt / t_cpu / 1e9, 4), "cores": 1, "kind": "port",
+                         "sample": f"float32 oracle on {cpu_shape[0]}x{cpu_shape[1]}x{cpu_shape[2]} x nt={nt}, "
+                                   f"auto-gate; {t_cpu:.2f} s"},
+    }
 
 
 def bench_adjoint(n=128, n_steps=800, n_fd=32):
@@ -491,6 +584,8 @@ def main():
         rec["serve"] = bench_serve(tuple(float(r) for r in args.serve_rates.split(",")), args.serve_requests)
     if not args.no_adjoint:
         rec["adjoint"] = bench_adjoint()
+    if not args.no_radforce:
+        rec["radiation_force"] = bench_radforce()
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(w, args.cpu_steps)
     else:

@@ -291,6 +291,35 @@ int sw_strain_invariants_from_u(int dtype, const void* ux, const void* uy, const
                                 void* max_shear, void* von_mises, int64_t nx, int64_t ny, int64_t nz, double dx,
                                 double dy, double dz, void* stream);
 
+/* --------------------------------------------------------- radiation force */
+
+/* Pressure (and particle velocity) fields are (n_vox, nt) with time fastest,
+ * i.e. the reference's (nx, ny, nz, nt) arrays; maps are (n_vox). A gate is
+ * either `gate`, n_gate device int32 time indices in [0, nt) (repeats
+ * allowed, as NumPy fancy indexing), or, with gate == NULL, the contiguous
+ * range [gate_start, gate_start + n_gate). n_gate = 0 gives NaN, the mean of
+ * an empty gate. */
+
+/* env[t] = sum over voxels of p^2 (float64, deterministic): the input of the
+ * auto-gate of compute_radiation_force (radiation_force.py:66-72), whose
+ * mean is env / n_vox. workspace: sw_pressure_energy_workspace bytes. */
+int64_t sw_pressure_energy_workspace(int64_t n_vox, int64_t nt);
+int sw_pressure_energy(int dtype, const void* p, int64_t n_vox, int64_t nt, double* env, void* workspace,
+                       int64_t workspace_bytes, void* stream);
+
+/* compute_radiation_force (radiation_force.py:27-83): per voxel
+ * b = 2 alpha_Np mean_gate(p^2) / (rho c) / c, alpha_Np = alpha f0[MHz] / 8.685889638 * 100. */
+int sw_radiation_force(int dtype, const void* p, int64_t n_vox, int64_t nt, const int32_t* gate, int64_t gate_start,
+                       int64_t n_gate, const void* c, const void* rho, const void* alpha_coeff, double f0, void* out,
+                       void* stream);
+
+/* compute_radiation_force_velocity (radiation_force.py:86-157): per voxel and
+ * component b_i = 2 alpha_Np mean_gate(p v_i) / c. */
+int sw_radiation_force_velocity(int dtype, const void* p, const void* vx, const void* vy, const void* vz,
+                                int64_t n_vox, int64_t nt, const int32_t* gate, int64_t gate_start, int64_t n_gate,
+                                const void* c, const void* alpha_coeff, double f0, void* bx, void* by, void* bz,
+                                void* stream);
+
 /* ------------------------------------------------------------------ layout */
 
 /* (count, 3) AoS <-> three SoA arrays of `count` elements (solver.py:148 keeps

@@ -1,7 +1,8 @@
 """shearwave_amd: shearwave's shear_fdtd hot path, MI355X-native.
 
-The viscoelastic shear FDTD time loop, its Helmholtz-Hodge force projection
-and the strain post-processing of pinton-lab/shearwave, re-implemented as
+The viscoelastic shear FDTD time loop, its Helmholtz-Hodge force projection,
+the strain post-processing and the acoustic radiation force that produces the
+body force, from pinton-lab/shearwave, re-implemented as
 hand-written gfx950 HIP kernels behind a C ABI (``include/shearwave.h``) and a
 Python drop-in with the reference's API. Build the library with
 ``python -m shearwave_amd.build``.
@@ -9,6 +10,8 @@ Python drop-in with the reference's API. Build the library with
 
 from ._abi import NativeLibraryMissing, SwError, lib
 from .api import (
+    compute_radiation_force,
+    compute_radiation_force_velocity,
     compute_strain_invariants,
     compute_strain_tensor,
     divergence_center,
@@ -35,6 +38,8 @@ __all__ = [
     "Solve",
     "SwError",
     "__version__",
+    "compute_radiation_force",
+    "compute_radiation_force_velocity",
     "compute_strain_invariants",
     "compute_strain_tensor",
     "divergence_center",

@@ -154,6 +154,11 @@ SIGNATURES = {
     "sw_strain_tensor": (_i32, [_i32, _p, _p, _p, _p, _i64, _i64, _i64, _d, _d, _d, _p]),
     "sw_strain_invariants": (_i32, [_i32, _p, _i64, _p, _p, _p, _p, _p]),
     "sw_strain_invariants_from_u": (_i32, [_i32, _p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _d, _d, _d, _p]),
+    "sw_pressure_energy_workspace": (_i64, [_i64, _i64]),
+    "sw_pressure_energy": (_i32, [_i32, _p, _i64, _i64, _p, _p, _i64, _p]),
+    "sw_radiation_force": (_i32, [_i32, _p, _i64, _i64, _p, _i64, _i64, _p, _p, _p, _d, _p, _p]),
+    "sw_radiation_force_velocity": (_i32, [_i32, _p, _p, _p, _p, _i64, _i64, _p, _i64, _i64, _p, _p, _d, _p, _p, _p,
+                                           _p]),
     "sw_aos3_to_soa": (_i32, [_i32, _p, _p, _p, _p, _i64, _p]),
     "sw_soa_to_aos3": (_i32, [_i32, _p, _p, _p, _p, _i64, _p]),
     "sw_export_state": (_i32, [_i32, _i32, C.POINTER(_p), C.POINTER(_p), _d, _p, _p, _i64, _p]),

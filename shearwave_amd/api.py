@@ -292,8 +292,18 @@ def project_body_force_to_shear(bz, dX, dY, dZ, tol=5e-5, max_iters=400, **kw):
 # -------------------------------------------------------------------- stencils
 
 
+_TORCH_TO_NP = {torch.float32: np.float32, torch.float64: np.float64, torch.float16: np.float16}
+
+
+def _np_dtype(a):
+    """NumPy dtype of an array or tensor (a device tensor is not copied)."""
+    if _is_dev(a) or isinstance(a, torch.Tensor):
+        return np.dtype(_TORCH_TO_NP.get(a.dtype, np.float64))
+    return np.asarray(a).dtype
+
+
 def _field_dtype(*arrays):
-    dt = np.result_type(*[a.detach().cpu().numpy().dtype if _is_dev(a) else np.asarray(a).dtype for a in arrays])
+    dt = np.result_type(*[_np_dtype(a) for a in arrays])
     return "float64" if dt == np.float64 else ("float32" if dt == np.float32 else "float64")
 
 
@@ -397,3 +407,105 @@ def compute_strain_invariants(epsilon):
     if _is_dev(epsilon):
         return res
     return {k: _np_out(v) for k, v in res.items()}
+
+
+# ---------------------------------------------------------- radiation force
+
+
+def _gate_indices(gate, nt):
+    """The time samples a gate selects, as NumPy fancy indexing on the time axis would (repeats kept)."""
+    idx = np.arange(nt)[gate]
+    return np.ascontiguousarray(np.atleast_1d(idx).astype(np.int32))
+
+
+def _gate_args(idx):
+    """(device indices or None, start, count): a contiguous ascending gate goes as a range, without indices."""
+    n = int(idx.shape[0])
+    if n == 0 or (n > 0 and np.all(np.diff(idx) == 1)):
+        return None, (int(idx[0]) if n else 0), n
+    return torch.from_numpy(np.ascontiguousarray(idx.astype(np.int32))).to(E.device()), 0, n
+
+
+def _auto_gate(p_dev, n_vox, nt, dt):
+    """Samples whose spatial mean of p^2 reaches 10% of its peak (radiation_force.py:66-72).
+
+    The sums run on the device in float64; the mean, the threshold and the
+    comparison run on the host in the pressure's dtype, as the reference does.
+    """
+    from ._abi import check, lib
+
+    need = int(lib().sw_pressure_energy_workspace(n_vox, nt))
+    work = torch.empty(max(need, 8) // 8, dtype=torch.float64, device=E.device())
+    env = torch.empty(nt, dtype=torch.float64, device=E.device())
+    check(lib().sw_pressure_energy(E.sw_code(dt), p_dev.data_ptr(), n_vox, nt, env.data_ptr(), work.data_ptr(),
+                                   work.numel() * 8, E.stream_ptr()))
+    npdt = np.float32 if dt == "float32" else np.float64
+    mean = (env.cpu().numpy() / n_vox).astype(npdt)
+    threshold = npdt(0.1) * mean.max()
+    mask = mean >= threshold
+    return np.arange(nt) if not mask.any() else np.nonzero(mask)[0]
+
+
+def _radforce_inputs(pressure_4d, maps):
+    if pressure_4d.ndim != 4:
+        raise ValueError(f"pressure_4d must be 4-D (nx, ny, nz, nt), got ndim={pressure_4d.ndim}")
+    shape3 = tuple(int(s) for s in pressure_4d.shape[:3])
+    for m in maps:
+        if tuple(m.shape) != shape3:
+            raise ValueError(f"maps must have shape {shape3}, got {tuple(m.shape)}")
+    return shape3, int(pressure_4d.shape[3])
+
+
+def compute_radiation_force(pressure_4d, c_map, rho_map, alpha_coeff_map, f0, gate=None):
+    """Acoustic radiation force b = 2 alpha_Np Isppa / c (radiation_force.py:27-83) on the GPU.
+
+    Same arguments and gate semantics as the reference (a slice, an index
+    array or None for the 10%-of-peak auto-gate); float32 inputs give float32,
+    anything else float64. Time reductions run in float64 on the device.
+    """
+    from ._abi import check, lib
+
+    shape3, nt = _radforce_inputs(pressure_4d, (c_map, rho_map, alpha_coeff_map))
+    dt = _field_dtype(pressure_4d, c_map, rho_map, alpha_coeff_map)
+    n_vox = int(np.prod(shape3))
+    p = E.to_device(pressure_4d, dt, shape3 + (nt,))
+    maps = [E.to_device(m, dt, shape3) for m in (c_map, rho_map, alpha_coeff_map)]
+    idx = _auto_gate(p, n_vox, nt, dt) if gate is None else _gate_indices(gate, nt)
+    g, g0, n_gate = _gate_args(idx)
+    out = torch.empty(shape3, dtype=p.dtype, device=p.device)
+    check(lib().sw_radiation_force(E.sw_code(dt), p.data_ptr(), n_vox, nt, None if g is None else g.data_ptr(), g0,
+                                   n_gate, maps[0].data_ptr(), maps[1].data_ptr(), maps[2].data_ptr(), float(f0),
+                                   out.data_ptr(), E.stream_ptr()))
+    return out if _is_dev(pressure_4d) else _np_out(out)
+
+
+def compute_radiation_force_velocity(pressure_4d, velocity_u_4d, velocity_w_4d, velocity_v_4d, c_map, rho_map,
+                                     alpha_coeff_map, f0, gate=None):
+    """Vector radiation force f_i = 2 alpha_Np <p v_i> / c (radiation_force.py:86-157) on the GPU.
+
+    ``rho_map`` is accepted for interface parity and not used, as in the reference.
+    """
+    from ._abi import check, lib
+
+    shape3, nt = _radforce_inputs(pressure_4d, (c_map, rho_map, alpha_coeff_map))
+    for v in (velocity_u_4d, velocity_w_4d, velocity_v_4d):
+        if tuple(v.shape) != tuple(pressure_4d.shape):
+            raise ValueError(f"velocity fields must have shape {tuple(pressure_4d.shape)}, got {tuple(v.shape)}")
+    dt = _field_dtype(pressure_4d, velocity_u_4d, velocity_w_4d, velocity_v_4d, c_map, rho_map, alpha_coeff_map)
+    n_vox = int(np.prod(shape3))
+    full = shape3 + (nt,)
+    p = E.to_device(pressure_4d, dt, full)
+    vel = [E.to_device(v, dt, full) for v in (velocity_u_4d, velocity_w_4d, velocity_v_4d)]
+    c = E.to_device(c_map, dt, shape3)
+    alpha = E.to_device(alpha_coeff_map, dt, shape3)
+    idx = _auto_gate(p, n_vox, nt, dt) if gate is None else _gate_indices(gate, nt)
+    g, g0, n_gate = _gate_args(idx)
+    out = torch.empty((3,) + shape3, dtype=p.dtype, device=p.device)
+    check(lib().sw_radiation_force_velocity(E.sw_code(dt), p.data_ptr(), *(v.data_ptr() for v in vel), n_vox, nt,
+                                            None if g is None else g.data_ptr(), g0, n_gate, c.data_ptr(),
+                                            alpha.data_ptr(), float(f0), out[0].data_ptr(), out[1].data_ptr(),
+                                            out[2].data_ptr(), E.stream_ptr()))
+    if _is_dev(pressure_4d):
+        return out[0], out[1], out[2]
+    return tuple(_np_out(out[k]) for k in range(3))
+

@@ -21,7 +21,8 @@ INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libshearwave_hip.so")
 
-SOURCES = ("abi.hip", "stencils.hip", "project.hip", "fdtd.hip", "strain.hip", "batch.hip", "adjoint.hip")
+SOURCES = ("abi.hip", "stencils.hip", "project.hip", "fdtd.hip", "strain.hip", "batch.hip", "adjoint.hip",
+           "radforce.hip")
 
 ARCH = "gfx950"
 CFLAGS = [

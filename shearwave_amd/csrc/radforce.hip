@@ -1,0 +1,290 @@
+// Acoustic radiation force from a recorded pressure field on gfx950
+// (compute_radiation_force / compute_radiation_force_velocity,
+// radiation_force.py:27-157): a time reduction over a 4-D (voxel, time)
+// array with time fastest, bound by HBM streaming.
+//
+//  * energy: env[t] = sum_v p[v][t]^2, the input of the reference's auto-gate
+//    (radiation_force.py:66-72). Each wave walks whole voxel rows (a
+//    coalesced run along t per load), lanes keeping per-sample sums; float64
+//    partials are summed over the block's waves and then over blocks in a
+//    fixed order: deterministic.
+//  * force: S lanes (a power of two up to 16) per voxel read its gated samples
+//    (a contiguous gate is passed as a range, with no index loads), four
+//    independent partial sums per lane, then a fixed butterfly; waves walk
+//    the voxels grid-stride. The per-voxel arithmetic follows the reference's
+//    order in float64 and rounds once to the storage type.
+#include "sw_common.h"
+
+namespace sw {
+namespace {
+
+constexpr int kEnergyThreads = 256;
+
+// A wave walks whole voxel rows (grid-stride over voxels); lane l keeps the
+// sums of samples tc + l + 64 u, u < 4, of a 256-sample chunk, so one voxel's
+// chunk is four coalesced 256 B loads. The block's four waves are then summed
+// in wave order into the block's partial.
+template <typename T>
+__global__ void __launch_bounds__(kEnergyThreads) energy_partial_kernel(const T* __restrict__ p, int64_t n_vox,
+                                                                        int64_t nt, double* __restrict__ part) {
+    __shared__ double red[kEnergyThreads / kWave][4 * kWave];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int64_t n_waves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int64_t w0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    for (int64_t tc = 0; tc < nt; tc += 4 * kWave) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        bool tin[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) tin[u] = tc + lane + kWave * u < nt;
+        int64_t v = w0;
+        for (; v + n_waves < n_vox; v += 2 * n_waves) {  // two voxels: eight loads in flight
+            const T* r0 = p + v * nt + tc + lane;
+            const T* r1 = r0 + n_waves * nt;
+            double x[8];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                x[u] = tin[u] ? (double)r0[kWave * u] : 0.0;
+                x[4 + u] = tin[u] ? (double)r1[kWave * u] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc[u] += x[u] * x[u] + x[4 + u] * x[4 + u];
+        }
+        for (; v < n_vox; v += n_waves) {
+            const T* r0 = p + v * nt + tc + lane;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double x = tin[u] ? (double)r0[kWave * u] : 0.0;
+                acc[u] += x * x;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) red[wid][lane + kWave * u] = acc[u];
+        __syncthreads();
+        for (int k = threadIdx.x; k < 4 * kWave; k += blockDim.x) {
+            const int64_t t = tc + k;
+            if (t < nt) {
+                double s = 0.0;
+                for (int w = 0; w < kEnergyThreads / kWave; ++w) s += red[w][k];  // fixed order
+                part[(int64_t)blockIdx.x * nt + t] = s;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// env[t] = sum over blocks of part[b][t]: one block per sample, threads
+// stride over the partials in a fixed order, then a fixed block reduction.
+__global__ void __launch_bounds__(256) energy_sum_kernel(const double* __restrict__ part, int64_t nb, int64_t nt,
+                                                         double* __restrict__ env) {
+    __shared__ double red[16];
+    for (int64_t t = blockIdx.x; t < nt; t += gridDim.x) {
+        double s = 0.0;
+        for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) s += part[b * nt + t];
+        s = block_sum(s, red);
+        if (threadIdx.x == 0) env[t] = s;
+        __syncthreads();
+    }
+}
+
+struct EnergyShape {
+    int64_t nb;
+};
+
+EnergyShape energy_shape(int64_t n_vox, int64_t nt) {
+    (void)nt;
+    EnergyShape e;
+    // eight blocks per CU (one resident wave of blocks), at most one wave per voxel
+    const int64_t waves_per_block = kEnergyThreads / kWave;
+    int64_t nb = (int64_t)device_cu_count() * 8;
+    const int64_t need = (n_vox + waves_per_block - 1) / waves_per_block;
+    if (nb > need) nb = need;
+    e.nb = nb < 1 ? 1 : nb;
+    return e;
+}
+
+constexpr double kDbPerNeper = 8.685889638;  // radiation_force.py:79, 149
+
+// Sample k of a gate: gate[k], or g0 + k for a contiguous gate (gate == nullptr).
+__device__ __forceinline__ int64_t gate_at(const int32_t* __restrict__ gate, int64_t g0, int64_t k) {
+    return gate ? (int64_t)gate[k] : g0 + k;
+}
+
+template <typename T, bool VEL>
+__global__ void __launch_bounds__(256) radforce_kernel(const T* __restrict__ p, const T* __restrict__ vx,
+                                                       const T* __restrict__ vy, const T* __restrict__ vz,
+                                                       int64_t n_vox, int64_t nt, const int32_t* __restrict__ gate,
+                                                       int64_t g0, int64_t n_gate, int S, const T* __restrict__ c,
+                                                       const T* __restrict__ rho, const T* __restrict__ alpha,
+                                                       double f0_mhz, T* __restrict__ o0, T* __restrict__ o1,
+                                                       T* __restrict__ o2) {
+    const int lane = threadIdx.x & 63;
+    const int seg = lane / S, sl = lane % S;
+    const int per_wave = 64 / S;
+    const int64_t n_waves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const double n = (double)n_gate;
+    for (int64_t wv = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;; wv += n_waves) {
+        const int64_t vbase = wv * per_wave;
+        if (vbase >= n_vox) break;  // uniform in the wave
+        const int64_t v = vbase + seg;
+        const bool ok = v < n_vox;
+        // four independent partial sums per lane (samples sl, sl+S, sl+2S, sl+3S, ...)
+        double s0[4] = {0.0, 0.0, 0.0, 0.0}, s1[4] = {0.0, 0.0, 0.0, 0.0}, s2[4] = {0.0, 0.0, 0.0, 0.0};
+        if (ok) {
+            const int64_t base = v * nt;
+            int64_t k = sl;
+            for (; k + 3 * S < n_gate; k += 4 * S) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int64_t i = base + gate_at(gate, g0, k + u * S);
+                    const double x = (double)p[i];
+                    if (VEL) {
+                        s0[u] += x * (double)vx[i];
+                        s1[u] += x * (double)vy[i];
+                        s2[u] += x * (double)vz[i];
+                    } else {
+                        s0[u] += x * x;
+                    }
+                }
+            }
+            for (; k < n_gate; k += S) {
+                const int64_t i = base + gate_at(gate, g0, k);
+                const double x = (double)p[i];
+                if (VEL) {
+                    s0[0] += x * (double)vx[i];
+                    s1[0] += x * (double)vy[i];
+                    s2[0] += x * (double)vz[i];
+                } else {
+                    s0[0] += x * x;
+                }
+            }
+        }
+        double a0 = (s0[0] + s0[1]) + (s0[2] + s0[3]);
+        double a1 = (s1[0] + s1[1]) + (s1[2] + s1[3]);
+        double a2 = (s2[0] + s2[1]) + (s2[2] + s2[3]);
+        for (int off = S >> 1; off > 0; off >>= 1) {  // within the S-lane segment
+            a0 += __shfl_xor(a0, off, kWave);
+            if (VEL) {
+                a1 += __shfl_xor(a1, off, kWave);
+                a2 += __shfl_xor(a2, off, kWave);
+            }
+        }
+        if (!ok || sl != 0) continue;
+        const double cc = (double)c[v];
+        const double anp = ((double)alpha[v] * f0_mhz) / kDbPerNeper * 100.0;  // radiation_force.py:78-79
+        if (VEL) {
+            const double coeff = 2.0 * anp / cc;  // radiation_force.py:151-155
+            o0[v] = (T)(coeff * (a0 / n));
+            o1[v] = (T)(coeff * (a1 / n));
+            o2[v] = (T)(coeff * (a2 / n));
+        } else {
+            const double isppa = (a0 / n) / ((double)rho[v] * cc);  // radiation_force.py:74-75
+            o0[v] = (T)(2.0 * anp * isppa / cc);                    // radiation_force.py:82
+        }
+    }
+}
+
+int check_sizes(int64_t n_vox, int64_t nt) {
+    SW_REQUIRE(n_vox >= 1 && nt >= 1, SW_ERR_ARG, "n_vox (%lld) and nt (%lld) must be positive", (long long)n_vox,
+               (long long)nt);
+    SW_REQUIRE(nt < (int64_t(1) << 31), SW_ERR_ARG, "nt too large");
+    return SW_OK;
+}
+
+// Lanes per voxel: 16 (a 64 B run of float32 samples per load and four
+// voxels per wave) once the gate has that many samples, fewer below.
+int segment_lanes(int64_t n_gate) {
+    int s = 1;
+    while (s < n_gate && s < 16) s <<= 1;
+    return s;
+}
+
+template <typename T, bool VEL>
+void launch_force(const T* p, const T* vx, const T* vy, const T* vz, int64_t n_vox, int64_t nt, const int32_t* gate,
+                  int64_t g0, int64_t n_gate, const T* c, const T* rho, const T* alpha, double f0, T* o0, T* o1,
+                  T* o2, hipStream_t s) {
+    const int S = segment_lanes(n_gate);
+    const int64_t waves = (n_vox + (64 / S) - 1) / (64 / S);
+    int64_t blocks = (waves + 3) / 4;
+    const int64_t cap = (int64_t)device_cu_count() * 32;  // grid-stride beyond eight waves per SIMD
+    if (blocks > cap) blocks = cap;
+    hipLaunchKernelGGL((radforce_kernel<T, VEL>), dim3((unsigned)blocks), dim3(256), 0, s, p, vx, vy, vz, n_vox, nt,
+                       gate, g0, n_gate, S, c, rho, alpha, f0 / 1e6, o0, o1, o2);
+}
+
+}  // namespace
+}  // namespace sw
+
+using namespace sw;
+
+extern "C" {
+
+int64_t sw_pressure_energy_workspace(int64_t n_vox, int64_t nt) {
+    if (n_vox < 1 || nt < 1) return -1;
+    const EnergyShape e = energy_shape(n_vox, nt);
+    return e.nb * nt * (int64_t)sizeof(double);
+}
+
+int sw_pressure_energy(int dtype, const void* p, int64_t n_vox, int64_t nt, double* env, void* workspace,
+                       int64_t workspace_bytes, void* stream) {
+    int rc = check_dtype(dtype);
+    if (rc) return rc;
+    if ((rc = check_sizes(n_vox, nt))) return rc;
+    SW_REQUIRE(p && env, SW_ERR_ARG, "null pointer");
+    const EnergyShape e = energy_shape(n_vox, nt);
+    SW_REQUIRE(workspace && workspace_bytes >= e.nb * nt * (int64_t)sizeof(double), SW_ERR_WORKSPACE,
+               "workspace of %lld bytes is smaller than the %lld needed", (long long)workspace_bytes,
+               (long long)(e.nb * nt * (int64_t)sizeof(double)));
+    hipStream_t s = (hipStream_t)stream;
+    double* part = (double*)workspace;
+    SW_DISPATCH(dtype, T,
+                hipLaunchKernelGGL(energy_partial_kernel<T>, dim3((unsigned)e.nb), dim3(kEnergyThreads), 0, s,
+                                   (const T*)p, n_vox, nt, part));
+    SW_LAUNCH_CHECK();
+    const int64_t sum_blocks = nt < 65536 ? nt : 65536;
+    hipLaunchKernelGGL(energy_sum_kernel, dim3((unsigned)sum_blocks), dim3(256), 0, s, part, e.nb, nt, env);
+    SW_LAUNCH_CHECK();
+    return SW_OK;
+}
+
+int sw_radiation_force(int dtype, const void* p, int64_t n_vox, int64_t nt, const int32_t* gate, int64_t gate_start,
+                       int64_t n_gate, const void* c, const void* rho, const void* alpha_coeff, double f0, void* out,
+                       void* stream) {
+    int rc = check_dtype(dtype);
+    if (rc) return rc;
+    if ((rc = check_sizes(n_vox, nt))) return rc;
+    SW_REQUIRE(p && c && rho && alpha_coeff && out, SW_ERR_ARG, "null pointer");
+    SW_REQUIRE(n_gate >= 0, SW_ERR_ARG, "n_gate must be >= 0");
+    SW_REQUIRE(gate || (gate_start >= 0 && gate_start + n_gate <= nt), SW_ERR_ARG,
+               "gate range [%lld, %lld) outside [0, %lld)", (long long)gate_start, (long long)(gate_start + n_gate),
+               (long long)nt);
+    hipStream_t s = (hipStream_t)stream;
+    SW_DISPATCH(dtype, T,
+                launch_force<T, false>((const T*)p, nullptr, nullptr, nullptr, n_vox, nt, gate, gate_start, n_gate,
+                                       (const T*)c, (const T*)rho, (const T*)alpha_coeff, f0, (T*)out, nullptr,
+                                       nullptr, s));
+    SW_LAUNCH_CHECK();
+    return SW_OK;
+}
+
+int sw_radiation_force_velocity(int dtype, const void* p, const void* vx, const void* vy, const void* vz,
+                                int64_t n_vox, int64_t nt, const int32_t* gate, int64_t gate_start, int64_t n_gate,
+                                const void* c, const void* alpha_coeff, double f0, void* bx, void* by, void* bz,
+                                void* stream) {
+    int rc = check_dtype(dtype);
+    if (rc) return rc;
+    if ((rc = check_sizes(n_vox, nt))) return rc;
+    SW_REQUIRE(p && vx && vy && vz && c && alpha_coeff && bx && by && bz, SW_ERR_ARG, "null pointer");
+    SW_REQUIRE(n_gate >= 0, SW_ERR_ARG, "n_gate must be >= 0");
+    SW_REQUIRE(gate || (gate_start >= 0 && gate_start + n_gate <= nt), SW_ERR_ARG,
+               "gate range [%lld, %lld) outside [0, %lld)", (long long)gate_start, (long long)(gate_start + n_gate),
+               (long long)nt);
+    hipStream_t s = (hipStream_t)stream;
+    SW_DISPATCH(dtype, T,
+                launch_force<T, true>((const T*)p, (const T*)vx, (const T*)vy, (const T*)vz, n_vox, nt, gate,
+                                      gate_start, n_gate, (const T*)c, nullptr, (const T*)alpha_coeff, f0, (T*)bx,
+                                      (T*)by, (T*)bz, s));
+    SW_LAUNCH_CHECK();
+    return SW_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,107 @@
+"""GPU parity of the acoustic radiation force (compute_radiation_force[_velocity]) against the oracle.
+
+Gates: float64 inputs within 1e-12 relative L2 of the float64 oracle; float32
+inputs within 1e-6 of the oracle run on the same float32 inputs (the GPU sums
+in float64 and rounds once, NumPy sums in float32). The auto-gate's time
+indices must be identical (the test signals keep every sample's mean p^2 away
+from the 10% threshold).
+"""
+
+import warnings
+
+import numpy as np
+import pytest
+
+from oracle import radiation_force as R
+from tests.test_oracle_radiation_force import burst, maps
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def field(shape, nt, seed, dtype, on=None):
+    rng = np.random.default_rng(seed)
+    p = rng.standard_normal(shape + (nt,)) * 1e5
+    if on is not None:
+        t = np.arange(nt)
+        p *= np.where((t >= on[0]) & (t < on[1]), 1.0, 0.05)
+    return p.astype(dtype)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("shape,nt", [((7, 5, 6), 5), ((12, 9, 10), 40), ((16, 12, 20), 300)])
+def test_auto_gate_and_force_match_oracle(sw, dtype, shape, nt):
+    from shearwave_amd import api
+
+    on = (nt // 5, nt - nt // 4) if nt > 5 else None
+    p = field(shape, nt, 3, dtype, on)
+    c, rho, alpha = maps(shape, dtype=dtype)
+    import torch
+
+    got_gate = api._auto_gate(torch.from_numpy(p).cuda(), int(np.prod(shape)), nt,
+                              "float32" if dtype == np.float32 else "float64")
+    want_gate = R.auto_gate(p)
+    want_gate = np.arange(nt)[want_gate]
+    np.testing.assert_array_equal(got_gate, want_gate)
+    b = sw.compute_radiation_force(p, c, rho, alpha, 2.0e6)
+    ref = R.compute_radiation_force(p, c, rho, alpha, 2.0e6)
+    assert b.dtype == ref.dtype and b.shape == shape
+    assert rel(b, ref) < (1e-12 if dtype == np.float64 else 1e-6), rel(b, ref)
+
+
+@pytest.mark.parametrize("gate", [slice(3, 17), slice(None, None, 3), np.array([4, 4, 9, 1, 30]),
+                                  np.arange(40) % 3 == 0, slice(-10, None)])
+def test_explicit_gates(sw, gate):
+    shape, nt = (9, 8, 11), 40
+    p = field(shape, nt, 4, np.float64)
+    c, rho, alpha = maps(shape)
+    b = sw.compute_radiation_force(p, c, rho, alpha, 1.0e6, gate)
+    ref = R.compute_radiation_force(p, c, rho, alpha, 1.0e6, gate)
+    assert rel(b, ref) < 1e-12
+
+
+def test_empty_gate_is_nan_like_numpy(sw):
+    shape, nt = (4, 4, 4), 10
+    p = field(shape, nt, 5, np.float64)
+    c, rho, alpha = maps(shape)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", RuntimeWarning)
+        ref = R.compute_radiation_force(p, c, rho, alpha, 1e6, slice(5, 5))
+    b = sw.compute_radiation_force(p, c, rho, alpha, 1e6, slice(5, 5))
+    assert np.isnan(ref).all() and np.isnan(b).all()
+
+
+def test_closed_form_burst_on_gpu(sw):
+    a, (p,) = burst(shape=(10, 9, 12), nt=48, on=(8, 40))
+    c, rho, alpha = maps(a.shape)
+    b = sw.compute_radiation_force(p, c, rho, alpha, 2.5e6)
+    want = 2.0 * (alpha * 2.5) / R.DB_PER_NEPER * 100.0 * (a * a / 2.0) / (rho * c) / c
+    assert rel(b, want) < 1e-12
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_velocity_force_matches_oracle(sw, dtype):
+    shape, nt = (11, 10, 13), 64
+    p = field(shape, nt, 6, dtype, (10, 50))
+    rng = np.random.default_rng(7)
+    v = [(p * s + rng.standard_normal(p.shape).astype(dtype) * 1e3).astype(dtype) * 1e-6 for s in (1.0, -0.5, 0.2)]
+    c, rho, alpha = maps(shape, dtype=dtype)
+    got = sw.compute_radiation_force_velocity(p, *v, c, rho, alpha, 1.5e6)
+    ref = R.compute_radiation_force_velocity(p, *v, c, rho, alpha, 1.5e6)
+    for g, r in zip(got, ref):
+        assert g.dtype == r.dtype
+        assert rel(g, r) < (1e-12 if dtype == np.float64 else 1e-6), rel(g, r)
+
+
+def test_shape_errors(sw):
+    p = np.zeros((3, 3, 3, 4))
+    c = np.ones((3, 3, 3))
+    with pytest.raises(ValueError):
+        sw.compute_radiation_force(p[..., 0], c, c, c, 1e6)
+    with pytest.raises(ValueError):
+        sw.compute_radiation_force(p, c[:2], c, c, 1e6)
