@@ -271,7 +271,7 @@ def bench_serve(rates=(20.0, 100.0), n_requests=24, tick_passes=10):
 
 
 def bench_radforce(shape=(160, 160, 160), nt=256, on=(64, 192), reps=5, cpu_shape=(48, 48, 48)):
-    """Row f3: the acoustic radiation force over a resident float32 pressure record (nx, ny, nz, nt).
+    """Rows f3 and f4: radiation force and acoustic strain over a resident float32 pressure record (nx, ny, nz, nt).
 
     Times the two kernels with HIP events on their stream (the auto-gate's
     energy pass over the whole record, the force pass over the gated
@@ -326,6 +326,42 @@ def bench_radforce(shape=(160, 160, 160), nt=256, on=(64, 192), reps=5, cpu_shap
 
     t_e = timed(energy)
     t_f = timed(force)
+    # acoustic strain kernels on the same record and gate (acoustic_strain.py)
+    n_gate = len(gate)
+    g_ptr = None if gidx is None else gidx.data_ptr()
+    stats = torch.empty((3, n_vox), dtype=torch.float32, device=dev)
+    gstats = torch.empty((5, n_vox), dtype=torch.float64, device=dev)
+    vol = torch.empty((n_vox, n_gate), dtype=torch.float32, device=dev)
+
+    def strain_peak():
+        check(lib().sw_acoustic_strain_peak(0, p.data_ptr(), n_vox, nt, g_ptr, g0, n_gate, c.data_ptr(),
+                                            rho.data_ptr(), stats[0].data_ptr(), stats[1].data_ptr(),
+                                            stats[2].data_ptr(), st))
+
+    def gradient_peak():
+        check(lib().sw_acoustic_strain_gradient_peak(0, p.data_ptr(), *shape, nt, g_ptr, g0, n_gate, c.data_ptr(),
+                                                     rho.data_ptr(), 0.3e-3, 0.3e-3, 0.3e-3,
+                                                     *(gstats[j].data_ptr() for j in range(5)), st))
+
+    def volumetric():
+        check(lib().sw_acoustic_strain_volumetric(0, p.data_ptr(), n_vox, nt, g_ptr, g0, n_gate, c.data_ptr(),
+                                                  rho.data_ptr(), vol.data_ptr(), st))
+
+    t_sp = timed(strain_peak)
+    t_gp = timed(gradient_peak)
+    t_vol = timed(volumetric)
+    del vol
+    grad = torch.empty((3, n_vox, n_gate), dtype=torch.float32, device=dev)
+
+    def gradient():
+        check(lib().sw_acoustic_strain_gradient(0, p.data_ptr(), *shape, nt, g_ptr, g0, n_gate, c.data_ptr(),
+                                                rho.data_ptr(), 0.3e-3, 0.3e-3, 0.3e-3, grad[0].data_ptr(),
+                                                grad[1].data_ptr(), grad[2].data_ptr(), st))
+
+    t_grad = timed(gradient)
+    del grad, stats, gstats
+    torch.cuda.empty_cache()
+    gated = n_vox * n_gate * 4  # the gated samples, read once
     p4 = p.view(shape + (nt,))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -354,6 +390,18 @@ def bench_radforce(shape=(160, 160, 160), nt=256, on=(64, 192), reps=5, cpu_shap
                         "frac": round(e_bytes / t_e / 1e9 / HBM_PEAK_GBS, 4)},
         "force_pass": {"us": round(t_f * 1e6, 1), "GBs": round(f_bytes / t_f / 1e9, 1),
                        "frac": round(f_bytes / t_f / 1e9 / HBM_PEAK_GBS, 4)},
+        "acoustic_strain": {
+            "peak_rms_mean": {"us": round(t_sp * 1e6, 1), "GBs": round((gated + n_vox * 20) / t_sp / 1e9, 1),
+                              "frac": round((gated + n_vox * 20) / t_sp / 1e9 / HBM_PEAK_GBS, 4)},
+            "gradient_statistics": {"us": round(t_gp * 1e6, 1),
+                                    "GBs": round((gated + n_vox * (8 + 40)) / t_gp / 1e9, 1),
+                                    "frac": round((gated + n_vox * 48) / t_gp / 1e9 / HBM_PEAK_GBS, 4)},
+            "volumetric": {"us": round(t_vol * 1e6, 1), "GBs": round((2 * gated + n_vox * 8) / t_vol / 1e9, 1),
+                           "frac": round((2 * gated + n_vox * 8) / t_vol / 1e9 / HBM_PEAK_GBS, 4)},
+            "gradient": {"us": round(t_grad * 1e6, 1), "GBs": round((4 * gated + n_vox * 8) / t_grad / 1e9, 1),
+                         "frac": round((4 * gated + n_vox * 8) / t_grad / 1e9 / HBM_PEAK_GBS, 4)},
+            "bytes": "gated samples read once; maps (c, rho) read and outputs written once",
+        },
         "dropin_call_ms": round(t_call * 1e3, 3),
         "gvoxel_samples_per_s": round(n_vox * nt / t_call / 1e9, 2),
         "cpu_baseline": {"gvoxel_samples_per_s": round(cn * nt / t_cpu / 1e9, 4), "cores": 1, "kind": "port",

@@ -320,6 +320,36 @@ int sw_radiation_force_velocity(int dtype, const void* p, const void* vx, const 
                                 const void* c, const void* alpha_coeff, double f0, void* bx, void* by, void* bz,
                                 void* stream);
 
+/* --------------------------------------------------------- acoustic strain */
+
+/* Records, maps and gates as for the radiation force; K = rho c^2.
+ * compute_acoustic_strain_volumetric (acoustic_strain.py:75-104):
+ * out (n_vox, n_gate) = -p[:, gate] / K. */
+int sw_acoustic_strain_volumetric(int dtype, const void* p, int64_t n_vox, int64_t nt, const int32_t* gate,
+                                  int64_t gate_start, int64_t n_gate, const void* c, const void* rho, void* out,
+                                  void* stream);
+
+/* compute_acoustic_strain_peak (acoustic_strain.py:107-151): per voxel
+ * peak = max|p| / K, rms = sqrt(mean p^2) / K, mean = -mean(p) / K. */
+int sw_acoustic_strain_peak(int dtype, const void* p, int64_t n_vox, int64_t nt, const int32_t* gate,
+                            int64_t gate_start, int64_t n_gate, const void* c, const void* rho, void* peak, void* rms,
+                            void* mean, void* stream);
+
+/* compute_acoustic_strain_gradient (acoustic_strain.py:154-203): gradient_center
+ * of -p_t / K for every gated sample; gx, gy, gz are (nx, ny, nz, n_gate). */
+int sw_acoustic_strain_gradient(int dtype, const void* p, int64_t nx, int64_t ny, int64_t nz, int64_t nt,
+                                const int32_t* gate, int64_t gate_start, int64_t n_gate, const void* c,
+                                const void* rho, double dx, double dy, double dz, void* gx, void* gy, void* gz,
+                                void* stream);
+
+/* compute_acoustic_strain_gradient_peak (acoustic_strain.py:206-277): float64
+ * running max of |grad|, |d/dx|, |d/dy|, |d/dz| and sqrt(mean |grad|^2). */
+int sw_acoustic_strain_gradient_peak(int dtype, const void* p, int64_t nx, int64_t ny, int64_t nz, int64_t nt,
+                                     const int32_t* gate, int64_t gate_start, int64_t n_gate, const void* c,
+                                     const void* rho, double dx, double dy, double dz, double* peak_mag,
+                                     double* peak_gx, double* peak_gy, double* peak_gz, double* rms_mag,
+                                     void* stream);
+
 /* ------------------------------------------------------------------ layout */
 
 /* (count, 3) AoS <-> three SoA arrays of `count` elements (solver.py:148 keeps

@@ -1,8 +1,9 @@
 """shearwave_amd: shearwave's shear_fdtd hot path, MI355X-native.
 
 The viscoelastic shear FDTD time loop, its Helmholtz-Hodge force projection,
-the strain post-processing and the acoustic radiation force that produces the
-body force, from pinton-lab/shearwave, re-implemented as
+the strain post-processing, the acoustic radiation force that produces the
+body force and the acoustic strain statistics, from pinton-lab/shearwave,
+re-implemented as
 hand-written gfx950 HIP kernels behind a C ABI (``include/shearwave.h``) and a
 Python drop-in with the reference's API. Build the library with
 ``python -m shearwave_amd.build``.
@@ -10,6 +11,10 @@ Python drop-in with the reference's API. Build the library with
 
 from ._abi import NativeLibraryMissing, SwError, lib
 from .api import (
+    compute_acoustic_strain_gradient,
+    compute_acoustic_strain_gradient_peak,
+    compute_acoustic_strain_peak,
+    compute_acoustic_strain_volumetric,
     compute_radiation_force,
     compute_radiation_force_velocity,
     compute_strain_invariants,
@@ -38,6 +43,10 @@ __all__ = [
     "Solve",
     "SwError",
     "__version__",
+    "compute_acoustic_strain_gradient",
+    "compute_acoustic_strain_gradient_peak",
+    "compute_acoustic_strain_peak",
+    "compute_acoustic_strain_volumetric",
     "compute_radiation_force",
     "compute_radiation_force_velocity",
     "compute_strain_invariants",

@@ -159,6 +159,12 @@ SIGNATURES = {
     "sw_radiation_force": (_i32, [_i32, _p, _i64, _i64, _p, _i64, _i64, _p, _p, _p, _d, _p, _p]),
     "sw_radiation_force_velocity": (_i32, [_i32, _p, _p, _p, _p, _i64, _i64, _p, _i64, _i64, _p, _p, _d, _p, _p, _p,
                                            _p]),
+    "sw_acoustic_strain_volumetric": (_i32, [_i32, _p, _i64, _i64, _p, _i64, _i64, _p, _p, _p, _p]),
+    "sw_acoustic_strain_peak": (_i32, [_i32, _p, _i64, _i64, _p, _i64, _i64, _p, _p, _p, _p, _p, _p]),
+    "sw_acoustic_strain_gradient": (_i32, [_i32, _p, _i64, _i64, _i64, _i64, _p, _i64, _i64, _p, _p, _d, _d, _d, _p,
+                                           _p, _p, _p]),
+    "sw_acoustic_strain_gradient_peak": (_i32, [_i32, _p, _i64, _i64, _i64, _i64, _p, _i64, _i64, _p, _p, _d, _d, _d,
+                                                _p, _p, _p, _p, _p, _p]),
     "sw_aos3_to_soa": (_i32, [_i32, _p, _p, _p, _p, _i64, _p]),
     "sw_soa_to_aos3": (_i32, [_i32, _p, _p, _p, _p, _i64, _p]),
     "sw_export_state": (_i32, [_i32, _i32, C.POINTER(_p), C.POINTER(_p), _d, _p, _p, _i64, _p]),

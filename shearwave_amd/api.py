@@ -509,3 +509,78 @@ def compute_radiation_force_velocity(pressure_4d, velocity_u_4d, velocity_w_4d, 
         return out[0], out[1], out[2]
     return tuple(_np_out(out[k]) for k in range(3))
 
+
+
+# ---------------------------------------------------------- acoustic strain
+
+
+def _acoustic_inputs(pressure_4d, c_map, rho_map, gate):
+    """Device record and maps in the result dtype, and the gate as (indices or None, start, count)."""
+    shape3, nt = _radforce_inputs(pressure_4d, (c_map, rho_map))
+    dt = _field_dtype(pressure_4d, c_map, rho_map)
+    n_vox = int(np.prod(shape3))
+    p = E.to_device(pressure_4d, dt, shape3 + (nt,))
+    c = E.to_device(c_map, dt, shape3)
+    rho = E.to_device(rho_map, dt, shape3)
+    idx = _auto_gate(p, n_vox, nt, dt) if gate is None else _gate_indices(gate, nt)
+    g, g0, n_gate = _gate_args(idx)
+    return shape3, nt, dt, n_vox, p, c, rho, (None if g is None else g.data_ptr(), g0, n_gate, g)
+
+
+def _ret(t, like):
+    return t if _is_dev(like) else _np_out(t)
+
+
+def compute_acoustic_strain_volumetric(pressure_4d, c_map, rho_map, gate=None):
+    """epsilon = -p / K over the gated samples, (nx, ny, nz, nt_gated) (acoustic_strain.py:75-104), on the GPU."""
+    from ._abi import check, lib
+
+    shape3, nt, dt, n_vox, p, c, rho, (gp, g0, n_gate, _keep) = _acoustic_inputs(pressure_4d, c_map, rho_map, gate)
+    out = torch.empty(shape3 + (n_gate,), dtype=p.dtype, device=p.device)
+    check(lib().sw_acoustic_strain_volumetric(E.sw_code(dt), p.data_ptr(), n_vox, nt, gp, g0, n_gate, c.data_ptr(),
+                                              rho.data_ptr(), out.data_ptr(), E.stream_ptr()))
+    return _ret(out, pressure_4d)
+
+
+def compute_acoustic_strain_peak(pressure_4d, c_map, rho_map, gate=None):
+    """{"peak", "rms", "mean"} of the acoustic strain over the gated samples (acoustic_strain.py:107-151)."""
+    from ._abi import check, lib
+
+    shape3, nt, dt, n_vox, p, c, rho, (gp, g0, n_gate, _keep) = _acoustic_inputs(pressure_4d, c_map, rho_map, gate)
+    out = torch.empty((3,) + shape3, dtype=p.dtype, device=p.device)
+    check(lib().sw_acoustic_strain_peak(E.sw_code(dt), p.data_ptr(), n_vox, nt, gp, g0, n_gate, c.data_ptr(),
+                                        rho.data_ptr(), out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr(),
+                                        E.stream_ptr()))
+    return {k: _ret(out[j], pressure_4d) for j, k in enumerate(("peak", "rms", "mean"))}
+
+
+def compute_acoustic_strain_gradient(pressure_4d, c_map, rho_map, dX, dY, dZ, gate=None):
+    """Spatial gradient of -p_t / K for every gated sample (acoustic_strain.py:154-203), on the GPU.
+
+    Each component is (nx, ny, nz, nt_gated) in the pressure's dtype, zero on the boundary layer.
+    """
+    from ._abi import check, lib
+
+    shape3, nt, dt, n_vox, p, c, rho, (gp, g0, n_gate, _keep) = _acoustic_inputs(pressure_4d, c_map, rho_map, gate)
+    out = torch.empty((3,) + shape3 + (n_gate,), dtype=p.dtype, device=p.device)
+    check(lib().sw_acoustic_strain_gradient(E.sw_code(dt), p.data_ptr(), *shape3, nt, gp, g0, n_gate, c.data_ptr(),
+                                            rho.data_ptr(), float(dX), float(dY), float(dZ), out[0].data_ptr(),
+                                            out[1].data_ptr(), out[2].data_ptr(), E.stream_ptr()))
+    p_dtype = _np_dtype(pressure_4d)
+    res = [out[j] for j in range(3)]
+    if _is_dev(pressure_4d):
+        return tuple(res)
+    return tuple(_np_out(r).astype(p_dtype, copy=False) for r in res)
+
+
+def compute_acoustic_strain_gradient_peak(pressure_4d, c_map, rho_map, dX, dY, dZ, gate=None):
+    """Float64 running statistics of the strain gradient over the gated samples (acoustic_strain.py:206-277)."""
+    from ._abi import check, lib
+
+    shape3, nt, dt, n_vox, p, c, rho, (gp, g0, n_gate, _keep) = _acoustic_inputs(pressure_4d, c_map, rho_map, gate)
+    out = torch.empty((5,) + shape3, dtype=torch.float64, device=p.device)
+    check(lib().sw_acoustic_strain_gradient_peak(E.sw_code(dt), p.data_ptr(), *shape3, nt, gp, g0, n_gate,
+                                                 c.data_ptr(), rho.data_ptr(), float(dX), float(dY), float(dZ),
+                                                 *(out[j].data_ptr() for j in range(5)), E.stream_ptr()))
+    keys = ("peak_magnitude", "peak_grad_x", "peak_grad_y", "peak_grad_z", "rms_magnitude")
+    return {k: _ret(out[j], pressure_4d) for j, k in enumerate(keys)}

@@ -1,0 +1,84 @@
+"""GPU parity of the acoustic strain statistics (acoustic_strain.py) against the oracle.
+
+Gates: the volumetric strain, the gradient, the peak and the float64 gradient
+statistics are bit-identical to the oracle in float32 and float64 (same
+operations, contraction off, the gradient statistics summed in sample order).
+rms and mean use float64 sums where NumPy sums in the storage type: they are
+checked against the oracle run on float64 copies of the same inputs, to 1e-6
+(float32) or 1e-12 (float64) of the rms scale.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import acoustic_strain as A
+
+pytestmark = pytest.mark.gpu
+
+H = (0.3e-3, 0.25e-3, 0.35e-3)
+
+
+def record(shape, nt, seed, dtype, burst=True):
+    rng = np.random.default_rng(seed)
+    p = rng.standard_normal(shape + (nt,)) * 1e5
+    if burst:
+        t = np.arange(nt)
+        p *= np.where((t >= nt // 5) & (t < nt - nt // 4), 1.0, 0.05)
+    c = rng.uniform(1450, 1600, shape)
+    rho = rng.uniform(950, 1100, shape)
+    return p.astype(dtype), c.astype(dtype), rho.astype(dtype)
+
+
+GATES = [None, slice(3, 17), np.array([5, 5, 2, 11]), slice(None, None, 4)]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("gate", GATES)
+def test_volumetric_and_gradient_bit_exact(sw, dtype, gate):
+    p, c, rho = record((9, 8, 11), 24, 1, dtype)
+    eps = sw.compute_acoustic_strain_volumetric(p, c, rho, gate)
+    ref = A.compute_acoustic_strain_volumetric(p, c, rho, gate)
+    assert eps.dtype == ref.dtype and eps.shape == ref.shape
+    np.testing.assert_array_equal(eps, ref)
+    got = sw.compute_acoustic_strain_gradient(p, c, rho, *H, gate)
+    want = A.compute_acoustic_strain_gradient(p, c, rho, *H, gate)
+    for g, w in zip(got, want):
+        assert g.dtype == w.dtype and g.shape == w.shape
+        np.testing.assert_array_equal(g, w)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("gate", GATES)
+def test_gradient_statistics_bit_exact(sw, dtype, gate):
+    p, c, rho = record((10, 7, 12), 30, 2, dtype)
+    got = sw.compute_acoustic_strain_gradient_peak(p, c, rho, *H, gate)
+    want = A.compute_acoustic_strain_gradient_peak(p, c, rho, *H, gate)
+    assert set(got) == set(want)
+    for k in want:
+        assert got[k].dtype == np.float64
+        np.testing.assert_array_equal(got[k], want[k])
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float32, 1e-6), (np.float64, 1e-12)])
+@pytest.mark.parametrize("gate", GATES)
+def test_peak_rms_mean(sw, dtype, tol, gate):
+    p, c, rho = record((12, 9, 10), 40, 3, dtype)
+    got = sw.compute_acoustic_strain_peak(p, c, rho, gate)
+    ref = A.compute_acoustic_strain_peak(p, c, rho, gate)
+    np.testing.assert_array_equal(got["peak"], ref["peak"])
+    gate64 = A.resolve_gate(p, gate)  # the gate the storage-type record selects
+    ref64 = A.compute_acoustic_strain_peak(p.astype(np.float64), c.astype(np.float64), rho.astype(np.float64), gate64)
+    scale = np.abs(ref64["rms"]).max()
+    for k in ("rms", "mean"):
+        assert got[k].dtype == dtype
+        assert np.abs(got[k].astype(np.float64) - ref64[k]).max() <= tol * scale, k
+
+
+def test_device_tensors_in_and_out(sw):
+    import torch
+
+    p, c, rho = record((6, 6, 6), 16, 4, np.float32)
+    tp, tc, tr = (torch.from_numpy(a).cuda() for a in (p, c, rho))
+    eps = sw.compute_acoustic_strain_volumetric(tp, tc, tr, slice(2, 9))
+    assert isinstance(eps, torch.Tensor) and eps.is_cuda
+    np.testing.assert_array_equal(eps.cpu().numpy(), A.compute_acoustic_strain_volumetric(p, c, rho, slice(2, 9)))
