@@ -350,6 +350,33 @@ int sw_acoustic_strain_gradient_peak(int dtype, const void* p, int64_t nx, int64
                                      double* peak_gx, double* peak_gy, double* peak_gz, double* rms_mag,
                                      void* stream);
 
+/* ------------------------------------------------------- static elasticity */
+
+typedef struct sw_static_info {
+    int64_t iters;   /* completed CG iterations (u updates) */
+    float relres;    /* last sqrt(r.r / rs0), NaN before the first iteration */
+    int32_t reason;  /* SW_CG_CONVERGED, SW_CG_DENOM (p.Ap <= 0) or SW_CG_MAX_ITERS */
+    int32_t pad;
+} sw_static_info;
+
+/* Bytes of device workspace sw_elastic_solve_static needs for an nx*ny*nz grid. */
+int64_t sw_static_workspace_bytes(int64_t nx, int64_t ny, int64_t nz);
+
+/* elastic_solve_static (solver.py:569-625): CG on (-mu lap) u = bz with a zero
+ * Dirichlet boundary, float32 throughout as the reference. bz, mu and u are
+ * float32 device arrays of nx*ny*nz voxels; mu == NULL uses mu_scalar
+ * (rounded to float32) everywhere. The dot products are summed in NumPy's
+ * float32 order (8192-element buffers, pairwise within), so u, every scalar
+ * and the iteration count equal the reference's. tol_is_f32 != 0 compares
+ * rel < tol in float32 (a Python float tol), 0 in float64 (a float64 NumPy
+ * scalar). history, if non-NULL, is a float32 device array of 2*max_iters
+ * receiving (r.r, rel) per iteration. `info` is a host pointer, filled after
+ * the call synchronises `stream`. */
+int sw_elastic_solve_static(const float* bz, const float* mu, double mu_scalar, int64_t nx, int64_t ny, int64_t nz,
+                            double dx, double dy, double dz, double tol, int tol_is_f32, int64_t max_iters,
+                            float* u, void* workspace, int64_t workspace_bytes, float* history,
+                            sw_static_info* info, void* stream);
+
 /* ------------------------------------------------------------------ layout */
 
 /* (count, 3) AoS <-> three SoA arrays of `count` elements (solver.py:148 keeps

@@ -2,7 +2,7 @@
 
 The viscoelastic shear FDTD time loop, its Helmholtz-Hodge force projection,
 the strain post-processing, the acoustic radiation force that produces the
-body force and the acoustic strain statistics, from pinton-lab/shearwave,
+body force, the acoustic strain statistics and the static elasticity solve, from pinton-lab/shearwave,
 re-implemented as
 hand-written gfx950 HIP kernels behind a C ABI (``include/shearwave.h``) and a
 Python drop-in with the reference's API. Build the library with
@@ -20,6 +20,7 @@ from .api import (
     compute_strain_invariants,
     compute_strain_tensor,
     divergence_center,
+    elastic_solve_static,
     gradient_center,
     hann1d,
     laplacian_center,
@@ -52,6 +53,7 @@ __all__ = [
     "compute_strain_invariants",
     "compute_strain_tensor",
     "divergence_center",
+    "elastic_solve_static",
     "gradient_center",
     "hann1d",
     "laplacian_center",

@@ -43,6 +43,10 @@ class CgInfo(C.Structure):
     _fields_ = [("iters", _i64), ("relres", _d), ("max_div", _d), ("reason", C.c_int32), ("pad", C.c_int32)]
 
 
+class StaticInfo(C.Structure):
+    _fields_ = [("iters", _i64), ("relres", C.c_float), ("reason", C.c_int32), ("pad", C.c_int32)]
+
+
 class FdtdState(C.Structure):
     _fields_ = [
         ("dtype", C.c_int32),
@@ -165,6 +169,9 @@ SIGNATURES = {
                                            _p, _p, _p]),
     "sw_acoustic_strain_gradient_peak": (_i32, [_i32, _p, _i64, _i64, _i64, _i64, _p, _i64, _i64, _p, _p, _d, _d, _d,
                                                 _p, _p, _p, _p, _p, _p]),
+    "sw_static_workspace_bytes": (_i64, [_i64, _i64, _i64]),
+    "sw_elastic_solve_static": (_i32, [_p, _p, _d, _i64, _i64, _i64, _d, _d, _d, _d, _i32, _i64, _p, _p, _i64, _p,
+                                       C.POINTER(StaticInfo), _p]),
     "sw_aos3_to_soa": (_i32, [_i32, _p, _p, _p, _p, _i64, _p]),
     "sw_soa_to_aos3": (_i32, [_i32, _p, _p, _p, _p, _i64, _p]),
     "sw_export_state": (_i32, [_i32, _i32, C.POINTER(_p), C.POINTER(_p), _d, _p, _p, _i64, _p]),

@@ -289,6 +289,64 @@ def project_body_force_to_shear(bz, dX, dY, dZ, tol=5e-5, max_iters=400, **kw):
     return project_vector_field(z, z, bz, dX, dY, dZ, tol, max_iters, **kw)
 
 
+# ---------------------------------------------------------- static elasticity
+
+
+def _tol_is_f32(tol):
+    """Whether ``rel < tol``, rel a float32 NumPy scalar, compares in float32.
+
+    Under NumPy 2's promotion a Python float (or int) adapts to float32; a
+    NumPy scalar keeps its own type, so a float64 one compares in float64.
+    """
+    if isinstance(tol, np.generic):
+        return np.result_type(np.float32, tol.dtype) == np.float32
+    return True
+
+
+def elastic_solve_static(bz, dX, dY, dZ, mu, tol=1e-6, max_iters=1500, *, verbose=False, return_device=False,
+                         info=None):
+    """Solve mu * laplace(u) = -bz with zero Dirichlet boundary via CG (solver.py:569-625), on the GPU.
+
+    As the reference: CG on the SPD form (-mu lap) u = bz, float32 vectors and
+    scalars, the masked updates in the same order, and every dot product
+    summed in NumPy's float32 order, so the result (and the iteration count)
+    equals the reference's. Returns float32 u (a device tensor for device
+    input or ``return_device``); ``info`` (a dict) receives iters, relres and
+    the stop reason. ``verbose`` prints the reference's progress lines once
+    the solve has finished.
+    """
+    import ctypes as C
+
+    from ._abi import CG_REASONS, StaticInfo, check, lib
+
+    nx, ny, nz = (int(s) for s in bz.shape)
+    shape = (nx, ny, nz)
+    mu_src = _prepare_material(mu, nx, ny, nz)
+    rhs = E.to_device(bz, "float32", shape)
+    mu_t = None if isinstance(mu_src, float) else E.to_device(mu_src, "float32", shape)
+    n_it = max(int(max_iters), 0)  # range(max_iters)
+    u = torch.empty(shape, dtype=torch.float32, device=E.device())
+    ws = E.static_workspace(shape)
+    hist = torch.empty(max(2 * n_it, 2), dtype=torch.float32, device=E.device()) if verbose else None
+    rep = StaticInfo()
+    check(
+        lib().sw_elastic_solve_static(
+            rhs.data_ptr(), None if mu_t is None else mu_t.data_ptr(), mu_src if mu_t is None else 0.0, nx, ny, nz,
+            float(dX), float(dY), float(dZ), float(tol), int(_tol_is_f32(tol)), n_it, u.data_ptr(), ws.data_ptr(),
+            ws.numel(), None if hist is None else hist.data_ptr(), C.byref(rep), E.stream_ptr(),
+        )
+    )
+    iters = int(rep.iters)
+    if verbose and iters:
+        for it, (rs_new, rel) in enumerate(hist[: 2 * iters].view(-1, 2).cpu().numpy()):
+            if it == 0 or (it + 1) % 25 == 0 or rel < tol:
+                print(f"CG (GPU): it={it + 1}/{max_iters}, |r|={np.sqrt(rs_new):.3e}, rel={rel:.3e}")
+    if info is not None:
+        info.update(iters=iters, relres=float(rep.relres) if iters else None,
+                    reason=CG_REASONS.get(int(rep.reason), str(rep.reason)))
+    return u if (return_device or _is_dev(bz)) else _np_out(u)
+
+
 # -------------------------------------------------------------------- stencils
 
 

@@ -22,7 +22,7 @@ BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libshearwave_hip.so")
 
 SOURCES = ("abi.hip", "stencils.hip", "project.hip", "fdtd.hip", "strain.hip", "batch.hip", "adjoint.hip",
-           "radforce.hip", "acoustic.hip")
+           "radforce.hip", "acoustic.hip", "static.hip")
 
 ARCH = "gfx950"
 CFLAGS = [

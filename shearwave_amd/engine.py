@@ -78,17 +78,27 @@ def to_device(x, dtype, shape=None) -> torch.Tensor:
 _WORKSPACES: dict = {}
 
 
-def project_workspace(shape) -> torch.Tensor:
-    """Cached CG workspace for a grid shape (one per device and shape)."""
-    key = (torch.cuda.current_device(), tuple(shape))
+def _workspace(kind, shape, nbytes_fn) -> torch.Tensor:
+    """Cached device workspace of one kind for a grid shape (one per device and shape)."""
+    key = (kind, torch.cuda.current_device(), tuple(shape))
     ws = _WORKSPACES.get(key)
     if ws is None:
-        nbytes = int(lib().sw_project_workspace_bytes(*shape))
+        nbytes = int(nbytes_fn(*shape))
         if nbytes <= 0:
             raise ValueError(f"bad grid {shape}")
         ws = torch.empty(nbytes, dtype=torch.uint8, device=device())
         _WORKSPACES[key] = ws
     return ws
+
+
+def project_workspace(shape) -> torch.Tensor:
+    """Cached CG workspace of the projection for a grid shape."""
+    return _workspace("project", shape, lib().sw_project_workspace_bytes)
+
+
+def static_workspace(shape) -> torch.Tensor:
+    """Cached workspace of the static elasticity solve for a grid shape."""
+    return _workspace("static", shape, lib().sw_static_workspace_bytes)
 
 
 def project_inplace(f, spacing, tol, max_iters, *, scale_out=None, inv_rho=None, scale=1.0) -> dict:
