@@ -59,6 +59,7 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation run")
     ap.add_argument("--no-radforce", action="store_true", help="skip the radiation-force run")
+    ap.add_argument("--no-static", action="store_true", help="skip the static elasticity solve run")
     ap.add_argument("--config1", action="store_true", help="only run the 64^3 CPU-vs-GPU comparison and exit")
     return ap.parse_args()
 
@@ -410,6 +411,95 @@ def bench_radforce(shape=(160, 160, 160), nt=256, on=(64, 192), reps=5, cpu_shap
     }
 
 
+def bench_static(n=256, k1=10, k2=60, cpu_iters=40):
+    """Row f4: the static elasticity solve (solver.py:569-625) on a resident n^3 float32 force.
+
+    Per-iteration time from two calls that differ only in their iteration cap
+    (tol = 0 never stops early): (T(k2) - T(k1)) / (k2 - k1), timed with HIP
+    events on the call's stream, so launches and the host's stop-flag polls
+    are included and the set-up is not. Algorithmic bytes per voxel and
+    iteration: the dir kernel reads r, p_old, mu and writes p (16 B), the step
+    kernel reads p, u, r, mu and writes u, r (24 B): 40 B. Also the
+    reference's quasi-static validation solve (72x48x72, tol 1e-6,
+    validate_quasistatic.py:24-33, 83-85) end to end on device tensors, and
+    the float32 oracle's rate over `cpu_iters` iterations of that case on one
+    core.
+    """
+    import torch
+
+    import shearwave_amd as sw
+    from shearwave_amd import engine as E
+
+    dev = E.device()
+    g = torch.Generator(device=dev).manual_seed(11)
+    shape = (n, n, n)
+    w = torch.sin(torch.pi * (torch.arange(n, device=dev, dtype=torch.float32) + 0.5) / n)
+    bz = 0.02 * torch.randn(shape, generator=g, device=dev, dtype=torch.float32) * w[:, None, None] * w[None, :, None]
+    bz = (bz * w[None, None, :]).contiguous()
+    mu = 2000.0 + 4000.0 * torch.rand(shape, generator=g, device=dev, dtype=torch.float32)
+    d = 2.0e-4
+
+    def call(k):
+        return sw.elastic_solve_static(bz, d, d, d, mu, tol=0.0, max_iters=k)
+
+    def timed(k, reps=2):
+        call(k)
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            call(k)
+        b.record()
+        b.synchronize()
+        return a.elapsed_time(b) / 1e3 / reps
+
+    t1, t2 = timed(k1), timed(k2)
+    per_it = (t2 - t1) / (k2 - k1)
+    vox = n**3
+    alg = 40 * vox
+    # the reference's validation solve
+    vshape, vd = (72, 48, 72), 2.0e-4
+    xg, yg, zg = np.meshgrid(*(np.arange(m) - m // 2 for m in vshape), indexing="ij")
+    G = np.exp(-0.5 * ((xg / 8.0) ** 2 + (yg / 8.0) ** 2 + (zg / 6.0) ** 2)).astype(np.float32)
+    G /= np.max(G)
+    taper = np.ones(vshape, np.float32)
+    for ax, m in enumerate(vshape):
+        taper = taper * np.hanning(m).astype(np.float32).reshape([-1 if q == ax else 1 for q in range(3)])
+    # the force of the validation run: its tapered Gaussian, projected to the shear part
+    _, _, vbz = sw.project_body_force_to_shear((0.02 * G * taper).astype(np.float32), vd, vd, vd, tol=5e-5,
+                                               max_iters=400)
+    vbz_t = torch.from_numpy(vbz).to(dev)
+    mu_v = 1000.0 * 2.0**2
+    info = {}
+    sw.elastic_solve_static(vbz_t, vd, vd, vd, mu_v, tol=1e-6, max_iters=1500, info=info)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sw.elastic_solve_static(vbz_t, vd, vd, vd, mu_v, tol=1e-6, max_iters=1500)
+    torch.cuda.synchronize()
+    t_val = time.perf_counter() - t0
+    from oracle import static_solve as S
+
+    t0 = time.perf_counter()
+    S.elastic_solve_static(vbz, vd, vd, vd, mu_v, tol=0.0, max_iters=cpu_iters)
+    t_cpu = (time.perf_counter() - t0) / cpu_iters
+    vvox = int(np.prod(vshape))
+    del bz, mu
+    torch.cuda.empty_cache()
+    return {
+        "workload": f"{n}^3 float32 force and mu field resident in HBM, h = {d}, CG iterations timed",
+        "us_per_iteration": round(per_it * 1e6, 1),
+        "gvoxel_iters_per_s": round(vox / per_it / 1e9, 2),
+        "alg_bytes_per_voxel_iter": 40,
+        "GBs": round(alg / per_it / 1e9, 1),
+        "frac": round(alg / per_it / 1e9 / HBM_PEAK_GBS, 4),
+        "validation_solve": {"shape": list(vshape), "iters": info.get("iters"), "reason": info.get("reason"),
+                             "ms": round(t_val * 1e3, 2)},
+        "cpu_baseline": {"gvoxel_iters_per_s": round(vvox / t_cpu / 1e9, 4), "cores": 1, "kind": "port",
+                         "sample": f"float32 oracle, {cpu_iters} iterations of the 72x48x72 validation solve, "
+                                   f"{t_cpu * 1e3:.1f} ms per iteration"},
+    }
+
+
 def bench_adjoint(n=128, n_steps=800, n_fd=32):
     """Config 5: gradient of a sensor-plane misfit w.r.t. mu and eta (128^3, nT=800)."""
     import torch
@@ -634,6 +724,8 @@ def main():
         rec["adjoint"] = bench_adjoint()
     if not args.no_radforce:
         rec["radiation_force"] = bench_radforce()
+    if not args.no_static:
+        rec["static_solve"] = bench_static()
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(w, args.cpu_steps)
     else:
