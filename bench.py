@@ -61,6 +61,9 @@ def parse():
     ap.add_argument("--no-radforce", action="store_true", help="skip the radiation-force run")
     ap.add_argument("--no-static", action="store_true", help="skip the static elasticity solve run")
     ap.add_argument("--config1", action="store_true", help="only run the 64^3 CPU-vs-GPU comparison and exit")
+    ap.add_argument("--only", choices=["radforce", "static"],
+                    help="only run that section (radiation force + acoustic strain, or the static solve), print "
+                         "its record and exit")
     return ap.parse_args()
 
 
@@ -623,6 +626,9 @@ def main():
 
     if args.config1:
         print(json.dumps(config1_cpu_vs_gpu()), flush=True)
+        return
+    if args.only:
+        print(json.dumps({"radforce": bench_radforce, "static": bench_static}[args.only]()), flush=True)
         return
     w = W.dropin_256(n=args.n, n_steps=args.nt)
     item = 4 if args.dtype == "float32" else 8
