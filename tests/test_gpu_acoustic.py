@@ -82,3 +82,13 @@ def test_device_tensors_in_and_out(sw):
     eps = sw.compute_acoustic_strain_volumetric(tp, tc, tr, slice(2, 9))
     assert isinstance(eps, torch.Tensor) and eps.is_cuda
     np.testing.assert_array_equal(eps.cpu().numpy(), A.compute_acoustic_strain_volumetric(p, c, rho, slice(2, 9)))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_gradient_statistics_bit_exact_large(sw, dtype):
+    """Millions of quotients: the kernel divides by K and 2h through a correction step from the reciprocal."""
+    p, c, rho = record((40, 36, 44), 64, 9, dtype, burst=False)
+    got = sw.compute_acoustic_strain_gradient_peak(p, c, rho, *H, slice(None))
+    want = A.compute_acoustic_strain_gradient_peak(p, c, rho, *H, slice(None))
+    for k in want:
+        np.testing.assert_array_equal(got[k], want[k])
