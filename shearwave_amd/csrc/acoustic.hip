@@ -55,6 +55,58 @@ __device__ __forceinline__ Seg seg_of(int S) {
     return q;
 }
 
+// Visiting order of the two kernels that read a voxel's six neighbours: 32 x
+// 32 (y, z) tiles, each walked one x plane at a time (the tile's 1024 columns
+// of plane x, then plane x + 1). A voxel's neighbours are then visited within
+// about one tile plane of it, except across tile edges, so their records are
+// still in L2 (a record of n_gate samples is far too long for the natural
+// order, where the x neighbours are a whole plane of records apart). Each XCD
+// (blocks b with b % 8 equal, for a grid of a multiple of 8 blocks) walks a
+// contiguous range of the order. Positions past the grid in y or z are skipped.
+constexpr int kOrderTile = 32;
+
+struct Order {
+    int ntz;
+    int64_t groups;                // voxel groups of 64/S positions
+    int64_t first, stride, end;  // this wave's walk over the groups
+};
+
+__device__ __forceinline__ Order order_walk(const Grid& g, int S) {
+    Order w;
+    const int nty = (g.ny + kOrderTile - 1) / kOrderTile;
+    w.ntz = (g.nz + kOrderTile - 1) / kOrderTile;
+    w.groups = (int64_t)nty * w.ntz * g.nx * (kOrderTile * kOrderTile / (64 / S));
+    const unsigned nb = gridDim.x, wpb = blockDim.x >> 6, b = blockIdx.x, wave = threadIdx.x >> 6;
+    if ((nb & 7u) == 0u) {
+        const unsigned xcd = b & 7u;
+        w.first = w.groups * xcd / 8 + (int64_t)(b >> 3) * wpb + wave;
+        w.stride = (int64_t)(nb >> 3) * wpb;
+        w.end = w.groups * (xcd + 1) / 8;
+    } else {
+        w.first = (int64_t)b * wpb + wave;
+        w.stride = (int64_t)nb * wpb;
+        w.end = w.groups;
+    }
+    return w;
+}
+
+// Position o of the order: its voxel, and whether it lies in the grid.
+__device__ __forceinline__ bool order_voxel(int64_t o, const Grid& g, int ntz, int& x, int& y, int& z) {
+    constexpr int T2 = kOrderTile * kOrderTile;
+    const int i = (int)(o % T2);
+    const int64_t rest = o / T2;
+    x = (int)(rest % g.nx);
+    const int64_t tile = rest / g.nx;
+    y = (int)(tile / ntz) * kOrderTile + i / kOrderTile;
+    z = (int)(tile % ntz) * kOrderTile + i % kOrderTile;
+    return y < g.ny && z < g.nz;
+}
+
+inline int64_t order_positions(const Grid& g) {
+    const int64_t nty = (g.ny + kOrderTile - 1) / kOrderTile, ntz = (g.nz + kOrderTile - 1) / kOrderTile;
+    return nty * ntz * g.nx * kOrderTile * kOrderTile;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) volumetric_kernel(const T* __restrict__ p, int64_t n_vox, int64_t nt,
                                                          const int32_t* __restrict__ gate, int64_t g0, int64_t n_gate,
@@ -135,15 +187,13 @@ __global__ void __launch_bounds__(256) strain_gradient_kernel(const T* __restric
                                                               const T* __restrict__ rho, T h2x, T h2y, T h2z,
                                                               T* __restrict__ gx, T* __restrict__ gy,
                                                               T* __restrict__ gz) {
-    const int64_t n_vox = g.size();
     const int64_t P = g.plane(), R = g.nz;
     const Seg q = seg_of(S);
-    for (int64_t wv = q.first;; wv += q.stride) {
-        const int64_t vbase = wv * (64 / S);
-        if (vbase >= n_vox) break;
-        const int64_t v = vbase + q.seg;
-        if (v >= n_vox) continue;
-        const int x = (int)(v / P), y = (int)((v / R) % g.ny), z = (int)(v % R);
+    const Order w = order_walk(g, S);
+    for (int64_t wv = w.first; wv < w.end; wv += w.stride) {
+        int x, y, z;
+        if (!order_voxel(wv * (64 / S) + q.seg, g, w.ntz, x, y, z)) continue;
+        const int64_t v = g.at(x, y, z);
         const int64_t o = v * n_gate;
         if (!g.interior(x, y, z)) {
             for (int64_t k = q.sl; k < n_gate; k += S) gx[o + k] = gy[o + k] = gz[o + k] = T(0);
@@ -166,73 +216,149 @@ __global__ void __launch_bounds__(256) strain_gradient_kernel(const T* __restric
 
 __device__ __forceinline__ double nan_max(double m, double x) { return (x > m || x != x) ? x : m; }
 
-// S = 16 lanes per voxel. The maxima reduce across the segment (order-free);
-// the sum of |grad|^2 is taken in sample order through the segment's lanes
-// (shuffles), as the reference's running sum.
+// Gradient statistics as an x march. A block owns an 8 x 32 (y, z) tile and an
+// x chunk of kGXC planes; one thread per column. The gated samples are taken
+// in windows of kGW, in order; for each window the block marches x over its chunk,
+// staging every plane's tile plus a one-voxel halo in LDS (a ring of three
+// planes), with the strain -p / K of each staged voxel and sample computed
+// once, in float64 as the reference. A thread then reads its voxel's six
+// neighbour strains from LDS. The maxima and the in-order sum of |grad|^2 of
+// the thread's kGXC voxels stay in registers across the windows, so the sum keeps
+// the reference's sample order. Each record is read about 1.6 times (halo and
+// the two planes before each chunk) instead of once per neighbour.
+#ifndef SW_GP_WINDOW
+#define SW_GP_WINDOW 8
+#endif
+#ifndef SW_GP_XCHUNK
+#define SW_GP_XCHUNK 8
+#endif
+constexpr int kGTY = 8, kGTZ = 32, kGW = SW_GP_WINDOW, kGXC = SW_GP_XCHUNK;
+constexpr int kGHY = kGTY + 2, kGHZ = kGTZ + 2, kGHV = kGHY * kGHZ;
+constexpr int kGThreads = kGTY * kGTZ;
+constexpr int kGFetch = (kGHV * kGW + kGThreads - 1) / kGThreads;  // staged elements per thread
+static_assert(kGThreads == 256, "gradient-statistics block");
+
+struct GpLds {
+    double s[3][kGHV][kGW + 1];  // strain of (plane slot, halo'd tile voxel, sample); +1 spreads the banks
+};
+
 template <typename T>
-__global__ void __launch_bounds__(256) strain_gradient_peak_kernel(
+__global__ void __launch_bounds__(kGThreads) strain_gradient_peak_kernel(
     const T* __restrict__ p, Grid g, int64_t nt, const int32_t* __restrict__ gate, int64_t g0, int64_t n_gate,
-    int S, const T* __restrict__ c, const T* __restrict__ rho, double h2x, double h2y, double h2z,
+    const T* __restrict__ c, const T* __restrict__ rho, double h2x, double h2y, double h2z, int nty, int ntz,
     double* __restrict__ peak_mag, double* __restrict__ peak_gx, double* __restrict__ peak_gy,
     double* __restrict__ peak_gz, double* __restrict__ rms_mag) {
-    const int64_t n_vox = g.size();
-    const int64_t P = g.plane(), R = g.nz;
-    const Seg q = seg_of(S);
-    const int base_lane = (threadIdx.x & 63) - q.sl;  // first lane of this segment
-    for (int64_t wv = q.first;; wv += q.stride) {
-        const int64_t vbase = wv * (64 / S);
-        if (vbase >= n_vox) break;  // uniform in the wave
-        const int64_t v = vbase + q.seg;
-        const bool ok = v < n_vox;
-        int x = 0, y = 0, z = 0;
-        if (ok) {
-            x = (int)(v / P);
-            y = (int)((v / R) % g.ny);
-            z = (int)(v % R);
+    __shared__ GpLds L;
+    // tile of this block: z tiles fastest, then y, then x chunk; XCD-aware, so
+    // the tiles one XCD runs at once are neighbours and share their halos in L2
+    const unsigned nb = gridDim.x;
+    unsigned b = blockIdx.x;
+    if ((nb & 7u) == 0u) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    const int tzi = (int)(b % (unsigned)ntz);
+    const unsigned rest = b / (unsigned)ntz;
+    const int tyi = (int)(rest % (unsigned)nty);
+    const int x0 = (int)(rest / (unsigned)nty) * kGXC, x1 = min(x0 + kGXC, g.nx);
+    const int y0 = tyi * kGTY, z0 = tzi * kGTZ;
+    const int tid = threadIdx.x, y = y0 + tid / kGTZ, z = z0 + tid % kGTZ;
+    const bool col = y < g.ny && z < g.nz;
+    const int64_t P = g.plane();
+    const int hv = (tid / kGTZ + 1) * kGHZ + tid % kGTZ + 1;  // this column in the halo'd tile
+    auto slot = [&](int x) { return (x - x0 + 3) % 3; };
+    // Staging of plane xs (tile + halo, samples k0 .. k0 + kGW - 1) in two
+    // halves: fetch issues every load of the thread's elements, commit divides
+    // and writes LDS, so the loads of plane x + 2 are in flight while plane x
+    // is computed. Element i of this thread (e = tid + i * kGThreads) is voxel
+    // e / kGW of the halo'd tile and sample tid % kGW of the window (the same
+    // sample for all i, as kGThreads is a multiple of kGW); eoff is its voxel's
+    // offset in a plane, -1 outside the grid (such an element reads voxel 0 and
+    // is not committed).
+    int32_t eoff[kGFetch];
+#pragma unroll
+    for (int i = 0; i < kGFetch; ++i) {
+        const int e = tid + i * kGThreads, h = e / kGW;
+        const int yy = y0 - 1 + h / kGHZ, zz = z0 - 1 + h % kGHZ;
+        const bool ok = e < kGHV * kGW && yy >= 0 && yy < g.ny && zz >= 0 && zz < g.nz;
+        eoff[i] = ok ? yy * g.nz + zz : -1;  // planes hold fewer than 2^31 voxels (checked on the host)
+    }
+    const int u_el = tid % kGW;
+    struct Fetch {
+        T p[kGFetch], r[kGFetch], c[kGFetch];
+    };
+    auto fetch = [&](int xs, int64_t t_el, Fetch& f) {
+        if (xs < 0 || xs >= g.nx) return;  // block-uniform
+        int64_t base = (int64_t)xs * P;
+        // opaque to the optimiser: otherwise it hoists every plane's element
+        // addresses out of the window loop and runs out of registers
+        asm volatile("" : "+s"(base));
+#pragma unroll
+        for (int i = 0; i < kGFetch; ++i) {
+            const bool ok = eoff[i] >= 0 && t_el >= 0;
+            const int64_t v = ok ? base + eoff[i] : 0;
+            f.p[i] = p[v * nt + (ok ? t_el : 0)];
+            f.r[i] = rho[v];
+            f.c[i] = c[v];
         }
-        const bool in = ok && g.interior(x, y, z);
-        double kxp = 1.0, kxm = 1.0, kyp = 1.0, kym = 1.0, kzp = 1.0, kzm = 1.0;
-        if (in) {  // K promoted to float64, as float64 / float32 in NumPy (acoustic_strain.py:262)
-            kxp = (double)bulk(rho, c, v + P);
-            kxm = (double)bulk(rho, c, v - P);
-            kyp = (double)bulk(rho, c, v + R);
-            kym = (double)bulk(rho, c, v - R);
-            kzp = (double)bulk(rho, c, v + 1);
-            kzm = (double)bulk(rho, c, v - 1);
+    };
+    auto commit = [&](int xs, int64_t t_el, const Fetch& f) {
+        if (xs < 0 || xs >= g.nx) return;  // block-uniform
+        double(*dst)[kGW + 1] = L.s[slot(xs)];
+#pragma unroll
+        for (int i = 0; i < kGFetch; ++i) {
+            if (eoff[i] < 0 || t_el < 0) continue;
+            // -p / K in float64, K = rho c^2 promoted as float64 / float32 in NumPy (acoustic_strain.py:258-262)
+            dst[(tid + i * kGThreads) / kGW][u_el] = -(double)f.p[i] / (double)(f.r[i] * (f.c[i] * f.c[i]));
         }
-        double pm = 0.0, px = 0.0, py = 0.0, pz = 0.0, sum = 0.0;
-        // rounds of S samples; every lane of the segment runs every round (shuffles)
-        for (int64_t k0 = 0; k0 < n_gate; k0 += S) {
-            const int64_t k = k0 + q.sl;
-            double m2 = 0.0;
-            if (in && k < n_gate) {
-                const int64_t t = gate_sample(gate, g0, k);
-                auto s = [&](int64_t u, double K) { return -(double)p[u * nt + t] / K; };
-                const double ax = (s(v + P, kxp) - s(v - P, kxm)) / h2x;
-                const double ay = (s(v + R, kyp) - s(v - R, kym)) / h2y;
-                const double az = (s(v + 1, kzp) - s(v - 1, kzm)) / h2z;
-                const double m = std::sqrt(ax * ax + ay * ay + az * az);  // acoustic_strain.py:265
-                pm = nan_max(pm, m);
-                px = nan_max(px, std::fabs(ax));
-                py = nan_max(py, std::fabs(ay));
-                pz = nan_max(pz, std::fabs(az));
-                m2 = m * m;
+    };
+    double pm[kGXC], px[kGXC], py[kGXC], pz[kGXC], sum[kGXC];
+#pragma unroll
+    for (int j = 0; j < kGXC; ++j) pm[j] = px[j] = py[j] = pz[j] = sum[j] = 0.0;
+    Fetch f;
+    for (int64_t k0 = 0; k0 < n_gate; k0 += kGW) {
+        const int nw = (int)min<int64_t>(kGW, n_gate - k0);
+        const int64_t t_el = k0 + u_el < n_gate ? gate_sample(gate, g0, k0 + u_el) : -1;  // this thread's sample
+        fetch(x0 - 1, t_el, f);
+        commit(x0 - 1, t_el, f);
+        fetch(x0, t_el, f);
+        commit(x0, t_el, f);
+        fetch(x0 + 1, t_el, f);
+#pragma unroll
+        for (int j = 0; j < kGXC; ++j) {
+            const int x = x0 + j;
+            if (x >= x1) break;  // block-uniform
+            commit(x + 1, t_el, f);
+            if (j + 1 < kGXC && x + 1 < x1) fetch(x + 2, t_el, f);
+            __syncthreads();
+            if (col && g.interior(x, y, z)) {
+                const double(*sp)[kGW + 1] = L.s[slot(x + 1)];
+                const double(*sc)[kGW + 1] = L.s[slot(x)];
+                const double(*sm)[kGW + 1] = L.s[slot(x - 1)];
+                for (int u = 0; u < nw; ++u) {
+                    // gradient_center of the strain (solver.py:726-728), acoustic_strain.py:262-270
+                    const double ax = (sp[hv][u] - sm[hv][u]) / h2x;
+                    const double ay = (sc[hv + kGHZ][u] - sc[hv - kGHZ][u]) / h2y;
+                    const double az = (sc[hv + 1][u] - sc[hv - 1][u]) / h2z;
+                    const double m = std::sqrt(ax * ax + ay * ay + az * az);
+                    pm[j] = nan_max(pm[j], m);
+                    px[j] = nan_max(px[j], std::fabs(ax));
+                    py[j] = nan_max(py[j], std::fabs(ay));
+                    pz[j] = nan_max(pz[j], std::fabs(az));
+                    sum[j] += m * m;
+                }
             }
-            const int cnt = (int)min<int64_t>(S, n_gate - k0);
-            for (int j = 0; j < cnt; ++j) sum += __shfl(m2, base_lane + j, kWave);  // sample order
+            __syncthreads();  // the next commit overwrites the slot of plane x - 1
         }
-        for (int off = S >> 1; off > 0; off >>= 1) {
-            pm = nan_max(pm, __shfl_xor(pm, off, kWave));
-            px = nan_max(px, __shfl_xor(px, off, kWave));
-            py = nan_max(py, __shfl_xor(py, off, kWave));
-            pz = nan_max(pz, __shfl_xor(pz, off, kWave));
-        }
-        if (!ok || q.sl != 0) continue;
-        peak_mag[v] = pm;
-        peak_gx[v] = px;
-        peak_gy[v] = py;
-        peak_gz[v] = pz;
-        rms_mag[v] = std::sqrt(sum / (double)(n_gate > 0 ? n_gate : 1));  // acoustic_strain.py:272
+    }
+    if (!col) return;
+#pragma unroll
+    for (int j = 0; j < kGXC; ++j) {
+        const int x = x0 + j;
+        if (x >= x1) break;
+        const int64_t v = g.at(x, y, z);
+        peak_mag[v] = pm[j];
+        peak_gx[v] = px[j];
+        peak_gy[v] = py[j];
+        peak_gz[v] = pz[j];
+        rms_mag[v] = std::sqrt(sum[j] / (double)(n_gate > 0 ? n_gate : 1));  // acoustic_strain.py:272
     }
 }
 
@@ -315,7 +441,7 @@ int sw_acoustic_strain_gradient(int dtype, const void* p, int64_t nx, int64_t ny
     // 2.0 * h in float64, then the storage type: NumPy 2 casts the Python scalar to the array's dtype
     const int S = gate_lanes(n_gate, 64);
     SW_DISPATCH(dtype, T,
-                hipLaunchKernelGGL(strain_gradient_kernel<T>, dim3(segment_blocks(g.size(), S)), dim3(256), 0, s,
+                hipLaunchKernelGGL(strain_gradient_kernel<T>, dim3(segment_blocks(order_positions(g), S)), dim3(256), 0, s,
                                    (const T*)p, g, nt, gate, gate_start, n_gate, S, (const T*)c, (const T*)rho,
                                    (T)(2.0 * dx), (T)(2.0 * dy), (T)(2.0 * dz), (T*)gx, (T*)gy, (T*)gz));
     SW_LAUNCH_CHECK();
@@ -334,11 +460,12 @@ int sw_acoustic_strain_gradient_peak(int dtype, const void* p, int64_t nx, int64
     SW_REQUIRE(p && c && rho && peak_mag && peak_gx && peak_gy && peak_gz && rms_mag, SW_ERR_ARG, "null pointer");
     hipStream_t s = (hipStream_t)stream;
     const Grid g{(int)nx, (int)ny, (int)nz};
-    const int S = gate_lanes(n_gate, 16);
+    const int nty = (g.ny + kGTY - 1) / kGTY, ntz = (g.nz + kGTZ - 1) / kGTZ, ntx = (g.nx + kGXC - 1) / kGXC;
     SW_DISPATCH(dtype, T,
-                hipLaunchKernelGGL(strain_gradient_peak_kernel<T>, dim3(segment_blocks(g.size(), S)), dim3(256), 0, s,
-                                   (const T*)p, g, nt, gate, gate_start, n_gate, S, (const T*)c, (const T*)rho,
-                                   2.0 * dx, 2.0 * dy, 2.0 * dz, peak_mag, peak_gx, peak_gy, peak_gz, rms_mag));
+                hipLaunchKernelGGL(strain_gradient_peak_kernel<T>, dim3((unsigned)(nty * ntz * ntx)), dim3(kGThreads),
+                                   0, s, (const T*)p, g, nt, gate, gate_start, n_gate, (const T*)c, (const T*)rho,
+                                   2.0 * dx, 2.0 * dy, 2.0 * dz, nty, ntz, peak_mag, peak_gx, peak_gy, peak_gz,
+                                   rms_mag));
     SW_LAUNCH_CHECK();
     return SW_OK;
 }
