@@ -2,15 +2,17 @@
 // (acoustic_strain.py:75-277): epsilon = -p / K, K = rho c^2, over the gated
 // samples of an (n_vox, nt) record with time fastest.
 //
-//  * every kernel gives S lanes per voxel over the gated samples, so loads
-//    and stores are coalesced runs along t; voxel coordinates and the bulk
-//    modulus of the voxel and its neighbours are computed once per voxel.
+//  * volumetric, peak / rms / mean and gradient give S lanes per voxel over
+//    the gated samples, so loads and stores are coalesced runs along t; voxel
+//    coordinates and the bulk moduli are computed once per voxel.
 //  * volumetric and gradient write (voxel, gated sample) outputs; the
-//    gradient reads the +-1 neighbour records along x, y, z.
+//    gradient reads the +-1 neighbour records along x, y, z (in a tiled
+//    visiting order that keeps neighbours close in time).
 //  * peak / rms / mean: max |p| and float64 sums of p^2 and p, four samples
 //    in flight per lane, then a butterfly over the segment.
-//  * gradient statistics: float64, maxima reduced over the segment, the sum
-//    of |grad|^2 taken in sample order, as the reference's running sum.
+//  * gradient statistics: an x march over (y, z) tiles that stages the
+//    float64 strain of three planes in LDS, one window of samples at a time;
+//    a thread per voxel keeps the maxima and the in-order sum of |grad|^2.
 //
 // Contraction is off and each expression keeps the reference's order, so
 // the volumetric strain, the gradient, the peak and the gradient statistics
