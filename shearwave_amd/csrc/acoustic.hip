@@ -109,21 +109,39 @@ inline int64_t order_positions(const Grid& g) {
     return nty * ntz * g.nx * kOrderTile * kOrderTile;
 }
 
+#ifndef SW_VOL_U
+#define SW_VOL_U 4
+#endif
+constexpr int kVolU = SW_VOL_U;  // voxel groups per wave iteration
+
 template <typename T>
 __global__ void __launch_bounds__(256) volumetric_kernel(const T* __restrict__ p, int64_t n_vox, int64_t nt,
                                                          const int32_t* __restrict__ gate, int64_t g0, int64_t n_gate,
                                                          int S, const T* __restrict__ c, const T* __restrict__ rho,
                                                          T* __restrict__ out) {
+    // kVolU voxel groups per iteration, so each lane has kVolU loads in flight
     const Seg q = seg_of(S);
-    for (int64_t wv = q.first;; wv += q.stride) {
-        const int64_t vbase = wv * (64 / S);
-        if (vbase >= n_vox) break;
-        const int64_t v = vbase + q.seg;
-        if (v >= n_vox) continue;
-        const T K = bulk(rho, c, v);
-        const T* pv = p + v * nt;
-        T* ov = out + v * n_gate;
-        for (int64_t k = q.sl; k < n_gate; k += S) ov[k] = -pv[gate_sample(gate, g0, k)] / K;  // acoustic_strain.py:104
+    const int64_t gsz = 64 / S;
+    for (int64_t wv = q.first; wv * gsz < n_vox; wv += kVolU * q.stride) {  // uniform in the wave
+        int64_t v[kVolU];
+        bool ok[kVolU];
+        T K[kVolU];
+#pragma unroll
+        for (int u = 0; u < kVolU; ++u) {
+            v[u] = (wv + u * q.stride) * gsz + q.seg;
+            ok[u] = v[u] < n_vox;
+            if (!ok[u]) v[u] = 0;  // read voxel 0, store nothing
+            K[u] = bulk(rho, c, v[u]);
+        }
+        for (int64_t k = q.sl; k < n_gate; k += S) {
+            const int64_t t = gate_sample(gate, g0, k);
+            T x[kVolU];
+#pragma unroll
+            for (int u = 0; u < kVolU; ++u) x[u] = p[v[u] * nt + t];
+#pragma unroll
+            for (int u = 0; u < kVolU; ++u)
+                if (ok[u]) out[v[u] * n_gate + k] = -x[u] / K[u];  // acoustic_strain.py:104
+        }
     }
 }
 
@@ -182,6 +200,11 @@ __global__ void __launch_bounds__(256) strain_peak_kernel(const T* __restrict__ 
     }
 }
 
+#ifndef SW_GRAD_U
+#define SW_GRAD_U 2
+#endif
+constexpr int kGradU = SW_GRAD_U;  // voxel groups per wave iteration
+
 template <typename T>
 __global__ void __launch_bounds__(256) strain_gradient_kernel(const T* __restrict__ p, Grid g, int64_t nt,
                                                               const int32_t* __restrict__ gate, int64_t g0,
@@ -190,28 +213,49 @@ __global__ void __launch_bounds__(256) strain_gradient_kernel(const T* __restric
                                                               T* __restrict__ gx, T* __restrict__ gy,
                                                               T* __restrict__ gz) {
     const int64_t P = g.plane(), R = g.nz;
+    // kGradU voxel groups per iteration, so each lane has 6 kGradU loads in flight
     const Seg q = seg_of(S);
     const Order w = order_walk(g, S);
-    for (int64_t wv = w.first; wv < w.end; wv += w.stride) {
-        int x, y, z;
-        if (!order_voxel(wv * (64 / S) + q.seg, g, w.ntz, x, y, z)) continue;
-        const int64_t v = g.at(x, y, z);
-        const int64_t o = v * n_gate;
-        if (!g.interior(x, y, z)) {
-            for (int64_t k = q.sl; k < n_gate; k += S) gx[o + k] = gy[o + k] = gz[o + k] = T(0);
-            continue;
+    for (int64_t wv = w.first; wv < w.end; wv += kGradU * w.stride) {  // uniform in the wave
+        int64_t v[kGradU];
+        bool ok[kGradU], in[kGradU];
+        T K[kGradU][6];
+        const T* nb[kGradU][6];
+#pragma unroll
+        for (int u = 0; u < kGradU; ++u) {
+            int x = 0, y = 0, z = 0;
+            ok[u] = wv + u * w.stride < w.end && order_voxel((wv + u * w.stride) * (64 / S) + q.seg, g, w.ntz, x, y, z);
+            v[u] = ok[u] ? g.at(x, y, z) : 0;
+            in[u] = ok[u] && g.interior(x, y, z);
+            // the six neighbours (x+, x-, y+, y-, z+, z-); off the interior the voxel itself (unused)
+            const int64_t off[6] = {in[u] ? P : 0, in[u] ? -P : 0, in[u] ? R : 0,
+                                    in[u] ? -R : 0, in[u] ? 1 : 0, in[u] ? -1 : 0};
+#pragma unroll
+            for (int n = 0; n < 6; ++n) {
+                K[u][n] = bulk(rho, c, v[u] + off[n]);
+                nb[u][n] = p + (v[u] + off[n]) * nt;
+            }
         }
-        const T kxp = bulk(rho, c, v + P), kxm = bulk(rho, c, v - P);
-        const T kyp = bulk(rho, c, v + R), kym = bulk(rho, c, v - R);
-        const T kzp = bulk(rho, c, v + 1), kzm = bulk(rho, c, v - 1);
-        const T *pxp = p + (v + P) * nt, *pxm = p + (v - P) * nt, *pyp = p + (v + R) * nt;
-        const T *pym = p + (v - R) * nt, *pzp = p + (v + 1) * nt, *pzm = p + (v - 1) * nt;
         for (int64_t k = q.sl; k < n_gate; k += S) {
             const int64_t t = gate_sample(gate, g0, k);
-            // strain -p / K at the neighbours (acoustic_strain.py:196), gradient_center (solver.py:726-728)
-            gx[o + k] = (-pxp[t] / kxp - -pxm[t] / kxm) / h2x;
-            gy[o + k] = (-pyp[t] / kyp - -pym[t] / kym) / h2y;
-            gz[o + k] = (-pzp[t] / kzp - -pzm[t] / kzm) / h2z;
+            T pv[kGradU][6];
+#pragma unroll
+            for (int u = 0; u < kGradU; ++u)
+#pragma unroll
+                for (int n = 0; n < 6; ++n) pv[u][n] = nb[u][n][t];
+#pragma unroll
+            for (int u = 0; u < kGradU; ++u) {
+                if (!ok[u]) continue;
+                const int64_t o = v[u] * n_gate + k;
+                if (!in[u]) {
+                    gx[o] = gy[o] = gz[o] = T(0);
+                    continue;
+                }
+                // strain -p / K at the neighbours (acoustic_strain.py:196), gradient_center (solver.py:726-728)
+                gx[o] = (-pv[u][0] / K[u][0] - -pv[u][1] / K[u][1]) / h2x;
+                gy[o] = (-pv[u][2] / K[u][2] - -pv[u][3] / K[u][3]) / h2y;
+                gz[o] = (-pv[u][4] / K[u][4] - -pv[u][5] / K[u][5]) / h2z;
+            }
         }
     }
 }
