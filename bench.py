@@ -55,7 +55,7 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the batching comparison")
     ap.add_argument("--batch-solves", type=int, default=48)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
-    ap.add_argument("--serve-rates", default="20,100")
+    ap.add_argument("--serve-rates", default="5,10,20,50,100,200")
     ap.add_argument("--serve-requests", type=int, default=24)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation run")
     ap.add_argument("--no-radforce", action="store_true", help="skip the radiation-force run")
@@ -123,20 +123,14 @@ def time_step_kernel(solve, n_launch, steps_per_launch):
     return start.elapsed_time(stop) / 1e3 / n_launch
 
 
-def cpu_baseline(w, n_steps):
-    """The float64 oracle on `n_steps` force-free steps of the same 256^3 loop (one core)."""
+def _oracle_steps(shape, h, m, e, dt, n_steps, seed=0):
+    """Seconds for `n_steps` force-free viscous steps of the float64 oracle loop (after one warm-up step)."""
     from oracle import solver as O
 
-    shape = w["shape"]
-    h = w["h"]
-    rho = w["rho"]
-    m = w["mu"] / rho
-    e = w["eta"] / rho
-    dt = w["dt"]
-    rng = np.random.default_rng(0)
+    rng = np.random.default_rng(seed)
     mask = O.interior_mask(shape, np.float64)
-    u_prev = 1e-9 * rng.standard_normal(shape + (3,)) * mask[..., None]
-    u = 1e-9 * rng.standard_normal(shape + (3,)) * mask[..., None]
+    u_prev = 1e-9 * rng.standard_normal(tuple(shape) + (3,)) * mask[..., None]
+    u = 1e-9 * rng.standard_normal(tuple(shape) + (3,)) * mask[..., None]
 
     def one():
         nonlocal u_prev, u
@@ -152,7 +146,13 @@ def cpu_baseline(w, n_steps):
     t0 = time.perf_counter()
     for _ in range(n_steps):
         one()
-    t = time.perf_counter() - t0
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(w, n_steps):
+    """The float64 oracle on `n_steps` force-free steps of the same 256^3 loop (one core)."""
+    shape = w["shape"]
+    t = _oracle_steps(shape, w["h"], w["mu"] / w["rho"], w["eta"] / w["rho"], w["dt"], n_steps)
     vox = int(np.prod(shape))
     return {
         "value": vox * n_steps / t / 1e9,
@@ -165,6 +165,35 @@ def cpu_baseline(w, n_steps):
         "numpy": np.__version__,
         "host": platform.processor() or platform.machine(),
         "os_cpu_count": os.cpu_count(),
+    }
+
+
+def _cpu_worker(args):
+    shape, n_steps, seed = args
+    t = _oracle_steps(shape, 5e-4, 2.5, 1.0e-3, 2e-5, n_steps, seed)
+    return int(np.prod(shape)) * n_steps / t
+
+
+def cpu_parallel_throughput(shape=(128, 128, 160), n_steps=2, max_procs=16):
+    """Independent float64 oracle loops, one process per core (at most max_procs): aggregate voxel-steps/s.
+
+    The throughput a CPU node gets on the batching and serving workloads by
+    running one solve per core. Loop only (the projection is not timed),
+    which flatters the CPU. Run before the GPU is initialised: the workers are
+    spawned processes.
+    """
+    import multiprocessing as mp
+
+    n = max(1, min(os.cpu_count() or 1, max_procs))
+    t0 = time.perf_counter()
+    with mp.get_context("spawn").Pool(n) as pool:
+        rates = pool.map(_cpu_worker, [(shape, n_steps, s) for s in range(n)])
+    return {
+        "processes": n,
+        "gvox_steps_per_s": round(sum(rates) / 1e9, 4),
+        "per_process_gvox_steps_per_s": round(min(rates) / 1e9, 5),
+        "sample": f"{n} processes, each {n_steps} force-free steps of a {shape[0]}x{shape[1]}x{shape[2]} float64 "
+                  f"oracle loop after a warm-up step; loop only, projection excluded; {time.perf_counter() - t0:.1f} s",
     }
 
 
@@ -620,6 +649,10 @@ def main():
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported by this build "
               "(a multi-GPU layout is described in DESIGN.md, not implemented)", file=sys.stderr)
         sys.exit(2)
+    # before anything touches the GPU: the CPU workers are spawned processes
+    cpu_par = None
+    if not args.no_cpu and not (args.no_batch and args.no_serve) and not args.only and not args.config1:
+        cpu_par = cpu_parallel_throughput()
     import torch
 
     from shearwave_amd import workloads as W
@@ -724,8 +757,16 @@ def main():
         del bench
         torch.cuda.empty_cache()
         rec["batch"] = bench_batch(args.batch_solves)
+        if cpu_par:
+            per_solve = rec["batch"]["voxel_steps"] / args.batch_solves
+            rec["batch"]["cpu_parallel"] = {**cpu_par, "solves_per_s_est": round(cpu_par["gvox_steps_per_s"] * 1e9 /
+                                                                                   per_solve, 5)}
     if not args.no_serve:
         rec["serve"] = bench_serve(tuple(float(r) for r in args.serve_rates.split(",")), args.serve_requests)
+        if cpu_par:
+            per_req = rec["serve"]["rates"][0]["voxel_steps"] / args.serve_requests
+            rec["serve"]["cpu_parallel"] = {**cpu_par, "solves_per_s_est": round(cpu_par["gvox_steps_per_s"] * 1e9 /
+                                                                                   per_req, 5)}
     if not args.no_adjoint:
         rec["adjoint"] = bench_adjoint()
     if not args.no_radforce:
