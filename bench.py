@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--nt", type=int, default=2000)
     ap.add_argument("--dtype", default="float32", choices=["float32", "float64"])
-    ap.add_argument("--fp64", action="store_true", help="also time one float64 solve")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the float64 solve")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--kernel-steps", type=int, default=200)
@@ -149,12 +149,41 @@ def _oracle_steps(shape, h, m, e, dt, n_steps, seed=0):
     return time.perf_counter() - t0
 
 
-def cpu_baseline(w, n_steps):
-    """The float64 oracle on `n_steps` force-free steps of the same 256^3 loop (one core)."""
+def cpu_projection_sample(w, iters, full_iters):
+    """The float64 oracle projection of the same force with the CG capped at `iters` iterations (one core).
+
+    Returns seconds for that call, its CG iterations, and the projection time
+    extrapolated to `full_iters` iterations (divergence and gradient-subtract
+    once, each CG iteration at the sampled per-iteration cost).
+    """
+    from oracle import solver as O
+
+    z = np.zeros(w["shape"])
+    bz = np.asarray(w["bz"], dtype=np.float64)
+    h = w["h"]
+    t0 = time.perf_counter()
+    O.project_vector_field(z, z, bz, h, h, h, 0.0, 1, dtype=np.float64)
+    t1 = time.perf_counter() - t0
+    info = {}
+    t0 = time.perf_counter()
+    O.project_vector_field(z, z, bz, h, h, h, 0.0, iters, dtype=np.float64, info=info)
+    tn = time.perf_counter() - t0
+    per_iter = (tn - t1) / max(iters - 1, 1)
+    return {"seconds": round(tn, 3), "cg_iters": info.get("iters", iters), "per_iter_s": round(per_iter, 4),
+            "extrapolated_s": round(t1 + per_iter * (full_iters - 1), 1), "full_iters": full_iters}
+
+
+def cpu_baseline(w, n_steps, proj_iters=0, full_iters=0):
+    """The float64 oracle on `n_steps` force-free steps of the same 256^3 loop (one core).
+
+    With proj_iters > 0 also a sample of the projection (setup), timed apart.
+    """
     shape = w["shape"]
     t = _oracle_steps(shape, w["h"], w["mu"] / w["rho"], w["eta"] / w["rho"], w["dt"], n_steps)
     vox = int(np.prod(shape))
+    proj = cpu_projection_sample(w, proj_iters, full_iters) if proj_iters > 0 else None
     return {
+        "projection_sample": proj,
         "value": vox * n_steps / t / 1e9,
         "unit": "Gvoxel-steps/s",
         "cores": 1,
@@ -740,7 +769,7 @@ def main():
             "loop_gvox_steps_per_s": round(spl * vox / kt / 1e9, 2),
         },
     }
-    if args.fp64:
+    if not args.no_fp64:
         b64 = DropIn(w, "float64", args.kernel, 0, 0)
         b64.run()
         torch.cuda.synchronize()
@@ -774,7 +803,7 @@ def main():
     if not args.no_static:
         rec["static_solve"] = bench_static()
     if not args.no_cpu:
-        rec["cpu_baseline"] = cpu_baseline(w, args.cpu_steps)
+        rec["cpu_baseline"] = cpu_baseline(w, args.cpu_steps, proj_iters=3, full_iters=int(rec["phases"]["cg_iters"]))
     else:
         rec["cpu_baseline"] = None
     print(json.dumps(rec), flush=True)
