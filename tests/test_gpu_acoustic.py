@@ -92,3 +92,23 @@ def test_gradient_statistics_bit_exact_large(sw, dtype):
     want = A.compute_acoustic_strain_gradient_peak(p, c, rho, *H, slice(None))
     for k in want:
         np.testing.assert_array_equal(got[k], want[k])
+
+
+def test_example_call_pattern(sw):
+    """As tips_transcranial_shear_3d.py:1136-1160 calls it: a transposed record view, gate=slice(None),
+    spacings positional, and the returned dicts' arrays masked in place."""
+    rng = np.random.default_rng(13)
+    shape, nt = (9, 10, 8), 20
+    p = (rng.standard_normal((nt,) + shape) * 1e5).astype(np.float32).transpose(1, 2, 3, 0)
+    _, c, rho = record(shape, 2, 14, np.float32)
+    st = sw.compute_acoustic_strain_peak(p, c, rho, gate=slice(None))
+    np.testing.assert_array_equal(st["peak"], A.compute_acoustic_strain_peak(p, c, rho, gate=slice(None))["peak"])
+    mask = np.zeros(shape, bool)
+    mask[0] = True
+    for key in ("peak", "rms", "mean"):
+        st[key][mask] = 0.0
+    gp = sw.compute_acoustic_strain_gradient_peak(p, c, rho, *H, gate=slice(None))
+    want = A.compute_acoustic_strain_gradient_peak(p, c, rho, *H, gate=slice(None))
+    for key in want:
+        np.testing.assert_array_equal(gp[key], want[key])
+        gp[key][mask] = 0.0

@@ -105,3 +105,28 @@ def test_shape_errors(sw):
         sw.compute_radiation_force(p[..., 0], c, c, c, 1e6)
     with pytest.raises(ValueError):
         sw.compute_radiation_force(p, c[:2], c, c, 1e6)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_example_call_pattern(sw, dtype):
+    """As tips_transcranial_shear_3d.py:1079-1110 calls it: (nt, nx, ny, nz) maps transposed to
+    (nx, ny, nz, nt) views, gate=slice(None), and the returned arrays scaled and masked in place."""
+    shape, nt = (10, 8, 9), 24
+    rng = np.random.default_rng(11)
+    c, rho, alpha = maps(shape, 12)
+    c, rho, alpha = (m.astype(dtype) for m in (c, rho, alpha))
+    p_t = (rng.standard_normal((nt,) + shape) * 1e5).astype(dtype)
+    vel = [(rng.standard_normal((nt,) + shape) * 0.1).astype(dtype) for _ in range(3)]
+    p4 = p_t.transpose(1, 2, 3, 0)
+    v4 = [v.transpose(1, 2, 3, 0) for v in vel]
+    assert not p4.flags["C_CONTIGUOUS"]
+    tol = 1e-12 if dtype == np.float64 else 1e-6
+    got = sw.compute_radiation_force_velocity(p4, *v4, c, rho, alpha, 1.0e6, gate=slice(None))
+    want = R.compute_radiation_force_velocity(p4, *v4, c, rho, alpha, 1.0e6, gate=slice(None))
+    for g, w in zip(got, want):
+        assert rel(g, w) < tol
+        g *= 2.0  # writable, as the example scales them
+        g[0] = 0.0
+    b0 = sw.compute_radiation_force(p4, c, rho, alpha, 1.0e6, gate=slice(None))
+    assert rel(b0, R.compute_radiation_force(p4, c, rho, alpha, 1.0e6, gate=slice(None))) < tol
+    b0[b0 > 0] = 0.0
