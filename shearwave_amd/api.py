@@ -605,6 +605,8 @@ def compute_acoustic_strain_peak(pressure_4d, c_map, rho_map, gate=None):
     from ._abi import check, lib
 
     shape3, nt, dt, n_vox, p, c, rho, (gp, g0, n_gate, _keep) = _acoustic_inputs(pressure_4d, c_map, rho_map, gate)
+    if n_gate == 0:  # np.max over the empty sample axis (acoustic_strain.py:143)
+        raise ValueError("zero-size array to reduction operation maximum which has no identity")
     out = torch.empty((3,) + shape3, dtype=p.dtype, device=p.device)
     check(lib().sw_acoustic_strain_peak(E.sw_code(dt), p.data_ptr(), n_vox, nt, gp, g0, n_gate, c.data_ptr(),
                                         rho.data_ptr(), out[0].data_ptr(), out[1].data_ptr(), out[2].data_ptr(),

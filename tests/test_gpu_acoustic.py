@@ -112,3 +112,21 @@ def test_example_call_pattern(sw):
     for key in want:
         np.testing.assert_array_equal(gp[key], want[key])
         gp[key][mask] = 0.0
+
+
+def test_empty_gate_as_numpy(sw):
+    """An empty gate: empty sample axes, zero gradient statistics, and NumPy's error for the peak."""
+    p, c, rho = record((6, 7, 8), 12, 15, np.float32)
+    gate = np.array([], dtype=np.int64)
+    eps = sw.compute_acoustic_strain_volumetric(p, c, rho, gate)
+    assert eps.shape == (6, 7, 8, 0)
+    for g in sw.compute_acoustic_strain_gradient(p, c, rho, *H, gate):
+        assert g.shape == (6, 7, 8, 0)
+    got = sw.compute_acoustic_strain_gradient_peak(p, c, rho, *H, gate)
+    want = A.compute_acoustic_strain_gradient_peak(p, c, rho, *H, gate)
+    for k in want:
+        np.testing.assert_array_equal(got[k], want[k])
+    with pytest.raises(ValueError):
+        A.compute_acoustic_strain_peak(p, c, rho, gate)
+    with pytest.raises(ValueError, match="zero-size array"):
+        sw.compute_acoustic_strain_peak(p, c, rho, gate)
