@@ -51,7 +51,7 @@ def loss_and_grad(prob, mu, eta, target):
     tr = SensorTraces.apply(mu, eta, prob)
     loss = 0.5 * ((tr.to(torch.float64) - target) ** 2).sum()
     loss.backward()
-    return float(loss), mu.grad, eta.grad
+    return float(loss.detach()), mu.grad, eta.grad
 
 
 def loss_only(prob, mu, eta, target):
