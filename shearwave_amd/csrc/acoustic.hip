@@ -333,8 +333,10 @@ __global__ void __launch_bounds__(kGThreads) strain_gradient_peak_kernel(
     auto fetch = [&](int xs, int64_t t_el, Fetch& f) {
         if (xs < 0 || xs >= g.nx) return;  // block-uniform
         int64_t base = (int64_t)xs * P;
-        // opaque to the optimiser: otherwise it hoists every plane's element
-        // addresses out of the window loop and runs out of registers
+        // Opaque to the optimiser (an empty asm, no instruction): otherwise it
+        // hoists each unrolled plane's element addresses out of the window loop,
+        // the kernel needs 256 VGPRs and one wave per SIMD instead of 252 and two,
+        // and runs at 5.0 ms instead of 3.0 ms on a 160^3 x 256 record.
         asm volatile("" : "+s"(base));
 #pragma unroll
         for (int i = 0; i < kGFetch; ++i) {
@@ -504,6 +506,7 @@ int sw_acoustic_strain_gradient_peak(int dtype, const void* p, int64_t nx, int64
     if ((rc = check_grid(nx, ny, nz))) return rc;
     if ((rc = check_record(nx * ny * nz, nt, gate, gate_start, n_gate))) return rc;
     SW_REQUIRE(p && c && rho && peak_mag && peak_gx && peak_gy && peak_gz && rms_mag, SW_ERR_ARG, "null pointer");
+    SW_REQUIRE(ny * nz < (int64_t(1) << 31), SW_ERR_ARG, "a plane of %lld voxels is too large", (long long)(ny * nz));
     hipStream_t s = (hipStream_t)stream;
     const Grid g{(int)nx, (int)ny, (int)nz};
     const int nty = (g.ny + kGTY - 1) / kGTY, ntz = (g.nz + kGTZ - 1) / kGTZ, ntx = (g.nx + kGXC - 1) / kGXC;
