@@ -51,10 +51,11 @@ def test_c_struct_layouts_match_ctypes(tmp_path):
         #include <stddef.h>
         #include "shearwave.h"
         int main(void) {
-            printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(sw_fdtd_state), sizeof(sw_fdtd_outputs),
+            printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(sw_fdtd_state), sizeof(sw_fdtd_outputs),
                    sizeof(sw_cg_info), sizeof(sw_batch_solve), offsetof(sw_batch_solve, u),
                    offsetof(sw_batch_solve, env), sizeof(sw_batch_cg), offsetof(sw_batch_cg, part),
-                   offsetof(sw_batch_cg, item0));
+                   offsetof(sw_batch_cg, item0), sizeof(sw_static_info), offsetof(sw_static_info, relres),
+                   offsetof(sw_static_info, reason));
             return 0;
         }
     """))
@@ -66,5 +67,5 @@ def test_c_struct_layouts_match_ctypes(tmp_path):
     got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     want = [C.sizeof(A.FdtdState), C.sizeof(A.FdtdOutputs), C.sizeof(A.CgInfo), C.sizeof(A.BatchSolve),
             A.BatchSolve.u.offset, A.BatchSolve.env.offset, C.sizeof(A.BatchCg), A.BatchCg.part.offset,
-            A.BatchCg.item0.offset]
+            A.BatchCg.item0.offset, C.sizeof(A.StaticInfo), A.StaticInfo.relres.offset, A.StaticInfo.reason.offset]
     assert got == want
