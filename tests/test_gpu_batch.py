@@ -127,3 +127,23 @@ def test_two_step_passes_equal_single_steps(sw, specs, mixed):
         u, v = b.result(k)
         np.testing.assert_array_equal(u, mixed[k][0])
         np.testing.assert_array_equal(v, mixed[k][1])
+
+
+def test_zero_force_solve_in_a_batch(sw, specs, mixed):
+    """A solve with no force projects by the early exit (max|div| < 1e-12) and stays zero; its
+    neighbour in the batch is unchanged, bit for bit."""
+    from shearwave_amd.batch import Batch, SolveSpec
+
+    sp = specs[0]
+    zero = SolveSpec(shape=sp.shape, h=sp.h, dt=sp.dt, rho=sp.rho, mu=sp.mu, eta=sp.eta,
+                     force=(None, None, np.zeros(sp.shape)), env=sp.env, n_steps=sp.n_steps, vel_damp=sp.vel_damp,
+                     tol=sp.tol, max_iters=sp.max_iters)
+    b = Batch([zero, specs[1]])
+    info = b.project()
+    assert info[0]["reason"] == "early_exit" and info[0]["iters"] == 0
+    b.run()
+    u0, v0 = b.result(0)
+    assert not np.any(u0) and not np.any(v0)
+    u1, v1 = b.result(1)
+    np.testing.assert_array_equal(u1, mixed[1][0])
+    np.testing.assert_array_equal(v1, mixed[1][1])

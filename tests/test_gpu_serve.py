@@ -64,3 +64,21 @@ def test_served_results_match_oracle(served):
         u, v = O.shear_fdtd(None, None, sp.force[2].astype(np.float64), sp.rho.astype(np.float64),
                             sp.mu.astype(np.float64), *sp.h, sp.dt, sp.n_steps, opts=opts, dtype=np.float64)
         assert rel(r.u, u) < 1e-5 and rel(r.v, v) < 1e-5
+
+
+def test_tight_admission_serves_everything_identically(sw, trace, served):
+    """With at most two active solves and a memory budget of about one solve (with its CG vectors),
+    requests queue instead of being dropped, start in arrival order, and give the same results."""
+    from shearwave_amd.batch import solve_bytes
+    from shearwave_amd.serve import Server
+
+    budget = max(solve_bytes(r.spec, with_cg=True) for r in trace) + 1
+    srv = Server(tick_passes=7, max_active=2, budget_bytes=budget)
+    reqs, summary = srv.run(copy.deepcopy(trace))
+    assert summary["requests"] == len(trace)
+    starts = [r.start_pass for r in sorted(reqs, key=lambda r: r.arrival)]
+    assert starts == sorted(starts)
+    ref = {r.arrival: r for r in served[0]}
+    for r in reqs:
+        np.testing.assert_array_equal(r.u, ref[r.arrival].u)
+        np.testing.assert_array_equal(r.v, ref[r.arrival].v)
