@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define SW_ABI_VERSION 1
+#define SW_ABI_VERSION 2
 
 /* status codes */
 #define SW_OK 0
@@ -91,15 +91,17 @@ int64_t sw_project_workspace_bytes(int64_t nx, int64_t ny, int64_t nz);
 
 /* project_vector_field (solver.py:644-683) with poisson_cg (solver.py:1010-1050).
  * Overwrites fx, fy, fz (dtype storage) with mask*(mask*f - grad_c phi), phi
- * solving -div_c grad_c phi = -div_c(mask*f) by CG on the interior. The CG
- * vectors, dot products and residual norm are float64 in both storage types.
- * If scale_out is non-NULL (three arrays of dtype), it also receives
- * scale * f_S * inv_rho (inv_rho: dtype array, or NULL for 1). `info` is a
- * host pointer, filled after the call synchronises `stream`. */
-int sw_project_vector_field(int dtype, void* fx, void* fy, void* fz, int64_t nx, int64_t ny, int64_t nz,
-                            double dx, double dy, double dz, double tol, int64_t max_iters, void* workspace,
-                            int64_t workspace_bytes, void* const* scale_out, const void* inv_rho, double scale,
-                            sw_cg_info* info, void* stream);
+ * solving -div_c grad_c phi = -div_c(mask*f) by CG on the interior. phi, the
+ * dot products and the residual norm are float64; the CG vectors r and p are
+ * stored in vec_dtype: SW_F64 as poisson_cg keeps them, or SW_F32 (p rounded
+ * as it is formed, and every use of p reads that stored value, so phi and r
+ * stay consistent). If scale_out is non-NULL (three arrays of dtype), it also
+ * receives scale * f_S * inv_rho (inv_rho: dtype array, or NULL for 1).
+ * `info` is a host pointer, filled after the call synchronises `stream`. */
+int sw_project_vector_field(int dtype, int vec_dtype, void* fx, void* fy, void* fz, int64_t nx, int64_t ny,
+                            int64_t nz, double dx, double dy, double dz, double tol, int64_t max_iters,
+                            void* workspace, int64_t workspace_bytes, void* const* scale_out, const void* inv_rho,
+                            double scale, sw_cg_info* info, void* stream);
 
 /* ---------------------------------------------------------------- time loop */
 
@@ -237,7 +239,8 @@ typedef struct sw_batch_cg {
     float* out[3];                   /* scale * f_S * inv_rho, or NULL */
     const float* inv_rho;            /* or NULL for 1 */
     double scale;
-    double *phi, *r, *p0, *p1;       /* float64 CG vectors, nx*ny*nz each */
+    double* phi;                     /* float64 potential, nx*ny*nz */
+    float *r, *p0, *p1;              /* float32 CG vectors, nx*ny*nz each (dot products stay float64) */
     double* part;                    /* 2 * n_items float64 partial slots */
     int32_t item0, n_items;          /* this solve's contiguous range in the CG item list */
 } sw_batch_cg;

@@ -150,7 +150,7 @@ SIGNATURES = {
     "sw_project_workspace_bytes": (_i64, [_i64, _i64, _i64]),
     "sw_project_vector_field": (
         _i32,
-        [_i32, _p, _p, _p, _i64, _i64, _i64, _d, _d, _d, _d, _i64, _p, _i64, C.POINTER(_p), _p, _d,
+        [_i32, _i32, _p, _p, _p, _i64, _i64, _i64, _d, _d, _d, _d, _i64, _p, _i64, C.POINTER(_p), _p, _d,
          C.POINTER(CgInfo), _p],
     ),
     "sw_prepare_coefficients": (_i32, [_i32, _p, _p, _p, _p, _p, _p, _i64, _p]),

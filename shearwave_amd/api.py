@@ -13,6 +13,10 @@ Build-only options (namespaced ``sw_``):
     output falls between them), ``"march"`` (one step per pass) or ``"direct"``
     time-step kernel; ``"auto"`` is ``"march2"`` on grids the march tiles fit;
   * ``sw_tile_y``, ``sw_tile_z``, ``sw_x_chunk``: march-kernel tile shape (0 = automatic);
+  * ``sw_cg_vectors``: storage of the projection's CG vectors r and p,
+    ``"float32"`` or ``"float64"`` (default: ``sw_dtype``); phi, the dot
+    products and the residual norm are float64 in both, as poisson_cg keeps
+    them (solver.py:1020-1041);
   * ``sw_strain_stride``: with ``shear_fdtd(..., return_strain=True)``, the
     four strain invariants every that many steps, computed in-loop.
 """
@@ -172,7 +176,7 @@ def shear_fdtd(  # noqa: PLR0912, PLR0915
     solve = E.Solve(shape, (dX, dY, dZ), float(dT), rho_f, mu_f, eta, dtype=dtype, form=form,
                     vel_damp=float(vel_damp), kernel=opts.get("sw_kernel", "auto"),
                     tile_y=int(opts.get("sw_tile_y", 0)), x_chunk=int(opts.get("sw_x_chunk", 0)),
-                    tile_z=int(opts.get("sw_tile_z", 0)))
+                    tile_z=int(opts.get("sw_tile_z", 0)), cg_vectors=opts.get("sw_cg_vectors"))
     if initial_u is not None or initial_v is not None:
         solve.set_initial(initial_u, initial_v)
 
@@ -263,19 +267,22 @@ def shear_fdtd_jax(bx, by, bz, rho, mu, dX, dY, dZ, dT, n_steps, opts=None, call
 # ------------------------------------------------------------------ projection
 
 
-def project_vector_field(fx, fy, fz, dX, dY, dZ, tol, max_iters, *, dtype="float32", return_device=False, info=None):
+def project_vector_field(fx, fy, fz, dX, dY, dZ, tol, max_iters, *, dtype="float32", return_device=False, info=None,
+                         cg_vectors=None):
     """Divergence-free projection (solver.py:644-683) on the GPU.
 
     Returns float32 arrays like the reference (``dtype="float64"`` keeps
-    float64). The CG runs in float64 on the device; ``info`` (a dict) receives
-    iters, relres, max_div and the stop reason.
+    float64). phi, the CG dot products and the residual norm are float64 on
+    the device; the CG vectors r and p are stored in ``cg_vectors`` (default:
+    ``dtype``). ``info`` (a dict) receives iters, relres, max_div and the stop
+    reason.
     """
     shape = tuple(int(s) for s in fx.shape)
     dev_in = _is_dev(fx)
     f = torch.empty((3,) + shape, dtype=E.torch_dtype(dtype), device=E.device())
     for c, comp in enumerate((fx, fy, fz)):
         f[c].copy_(E.to_device(comp, dtype, shape))
-    rep = E.project_inplace(f, (dX, dY, dZ), tol, max_iters)
+    rep = E.project_inplace(f, (dX, dY, dZ), tol, max_iters, vectors=cg_vectors)
     if info is not None:
         info.update(rep)
     if return_device or dev_in:

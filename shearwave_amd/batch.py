@@ -122,11 +122,12 @@ def pass_plan(n_steps, two_step=True):
 
 
 def solve_bytes(spec: SolveSpec, with_cg=True) -> int:
-    """HBM footprint of one solve: fields, state, force, and (during projection) the CG vectors."""
+    """HBM footprint of one solve: fields, state, force, and (during projection) the CG
+    vectors (phi in float64; r and two p buffers in float32)."""
     n = spec.voxels
     b = n * 4 * (3 + 12 + 6) + spec.n_steps * 8
     if with_cg:
-        b += n * 8 * 4
+        b += n * (8 + 3 * 4)
     return b
 
 
@@ -208,9 +209,10 @@ class Batch:
                 for ty in range(math.ceil(ny / CG_BY)):
                     for tz in range(math.ceil(nz / 64)):
                         items.append((k, tz, ty, tx))
-            vec = torch.empty((4,) + s.shape, dtype=torch.float64, device=dev)
+            phi = torch.empty(s.shape, dtype=torch.float64, device=dev)
+            vec = torch.empty((3,) + s.shape, dtype=torch.float32, device=dev)  # r, p0, p1
             part = torch.empty(2 * (len(items) - n_items0), dtype=torch.float64, device=dev)
-            keep += [vec, part]
+            keep += [phi, vec, part]
             c = cgs[k]
             c.nx, c.ny, c.nz = s.shape
             c.xchunk = xc
@@ -222,7 +224,8 @@ class Batch:
                 c.out[j] = s.force[j].data_ptr()
             c.inv_rho = s.inv_rho.data_ptr()
             c.scale = float(sp.dt) ** 2
-            c.phi, c.r, c.p0, c.p1 = (vec[j].data_ptr() for j in range(4))
+            c.phi = phi.data_ptr()
+            c.r, c.p0, c.p1 = (vec[j].data_ptr() for j in range(3))
             c.part = part.data_ptr()
             c.item0 = n_items0
             c.n_items = len(items) - n_items0

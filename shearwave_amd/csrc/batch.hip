@@ -151,8 +151,8 @@ __global__ void __launch_bounds__(64 * kCgBY) batch_cg_init_kernel(const sw_batc
         const Grid g{S.nx, S.ny, S.nz};
         const CgTile t = batch_cg_tile(S, it);
         double s = 0.0, mx = 0.0;
-        cg_init_tile<float>(S.f[0], S.f[1], S.f[2], S.phi, S.r, S.p0, g, 0.5 / S.dx, 0.5 / S.dy, 0.5 / S.dz, t, s,
-                            mx);
+        cg_init_tile<float, float>(S.f[0], S.f[1], S.f[2], S.phi, S.r, S.p0, g, 0.5 / S.dx, 0.5 / S.dy, 0.5 / S.dz,
+                                   t, s, mx);
         s = block_sum(s, red);
         mx = block_max(mx, red);
         if (publish_slot(S.part, S.part + S.n_items, (int)(k - S.item0), s, mx, &st->counter, S.n_items)) {
@@ -171,10 +171,11 @@ __global__ void __launch_bounds__(64 * kCgBY) batch_cg_init_kernel(const sw_batc
     }
 }
 
-// p ping-pongs between p0 and p1 by the solve's iteration parity.
+// p ping-pongs between p0 and p1 by the solve's iteration parity. The batch
+// keeps r and p in float32 (phi, dot products and residual norm in float64).
 __global__ void __launch_bounds__(64 * kCgBY) batch_cg_dir_kernel(const sw_batch_cg* __restrict__ solves, CgState* states,
                                                                   const int4* __restrict__ items, int64_t n_items) {
-    __shared__ CgLds lds;
+    __shared__ CgLdsT<float> lds;
     __shared__ double red[16];
     const ItemWalk w = item_walk(n_items);
     for (int64_t k = w.first; k < w.hi; k += w.stride) {
@@ -185,9 +186,9 @@ __global__ void __launch_bounds__(64 * kCgBY) batch_cg_dir_kernel(const sw_batch
         const Grid g{S.nx, S.ny, S.nz};
         const CgTile t = batch_cg_tile(S, it);
         const bool odd = st->iters & 1;
-        const double* p_old = odd ? S.p1 : S.p0;
-        double* p_new = odd ? S.p0 : S.p1;
-        double s = cg_dir_tile(S.r, p_old, p_new, g, st->beta, 0.25 / (S.dx * S.dx), 0.25 / (S.dy * S.dy),
+        const float* p_old = odd ? S.p1 : S.p0;
+        float* p_new = odd ? S.p0 : S.p1;
+        double s = cg_dir_tile<float>(S.r, p_old, p_new, g, st->beta, 0.25 / (S.dx * S.dx), 0.25 / (S.dy * S.dy),
                                0.25 / (S.dz * S.dz), t, lds);
         s = block_sum(s, red);
         if (publish_slot(S.part, nullptr, (int)(k - S.item0), s, 0.0, &st->counter, S.n_items)) {
@@ -201,7 +202,7 @@ __global__ void __launch_bounds__(64 * kCgBY) batch_cg_dir_kernel(const sw_batch
 
 __global__ void __launch_bounds__(64 * kCgBY) batch_cg_step_kernel(const sw_batch_cg* __restrict__ solves, CgState* states,
                                                                    const int4* __restrict__ items, int64_t n_items) {
-    __shared__ CgLds lds;
+    __shared__ CgLdsT<float> lds;
     __shared__ double red[16];
     const ItemWalk w = item_walk(n_items);
     for (int64_t k = w.first; k < w.hi; k += w.stride) {
@@ -212,8 +213,8 @@ __global__ void __launch_bounds__(64 * kCgBY) batch_cg_step_kernel(const sw_batc
         const Grid g{S.nx, S.ny, S.nz};
         const CgTile t = batch_cg_tile(S, it);
         const bool odd = st->iters & 1;
-        const double* p = odd ? S.p0 : S.p1;
-        double s = cg_step_tile(S.phi, S.r, p, g, st->alpha, 0.25 / (S.dx * S.dx), 0.25 / (S.dy * S.dy),
+        const float* p = odd ? S.p0 : S.p1;
+        double s = cg_step_tile<float>(S.phi, S.r, p, g, st->alpha, 0.25 / (S.dx * S.dx), 0.25 / (S.dy * S.dy),
                                 0.25 / (S.dz * S.dz), t, lds);
         s = block_sum(s, red);
         if (publish_slot(S.part, nullptr, (int)(k - S.item0), s, 0.0, &st->counter, S.n_items)) {

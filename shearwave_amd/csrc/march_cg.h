@@ -132,6 +132,20 @@ __device__ inline void cg_reset(CgState* st, double tol, long long max_iters) {
 // p_old, p and phi are each read from HBM about once per iteration. The
 // queue's roles rotate over a loop unrolled by six, so no prefetched register
 // is moved.
+//
+// Vector storage V: r, p_old and p are double (as poisson_cg keeps them) or
+// float; phi, every product, the dot products and the residual norm are
+// double in both. With float vectors, p is rounded to float as it is formed
+// and A p, p.Ap, phi += alpha p and r -= alpha A p all use that stored value,
+// so phi and r stay consistent (b - A phi - r at double rounding level) and
+// only the search directions carry float rounding; r is rounded as stored and
+// r.r is summed over the stored values.
+
+// v rounded to the vector storage type V and widened back (the identity for double).
+template <typename V>
+__device__ __forceinline__ double as_stored(double v) {
+    return (double)(V)v;
+}
 
 #ifndef SW_CG_BY
 #define SW_CG_BY 16
@@ -139,7 +153,8 @@ __device__ inline void cg_reset(CgState* st, double tol, long long max_iters) {
 constexpr int kCgBY = SW_CG_BY;  // tile rows in y (block of 64 x kCgBY threads)
 constexpr int kCgLZ = 68;  // 64 + 2 + 2
 
-using CgLds = double[2][kCgBY + 4][kCgLZ];
+template <typename V>
+using CgLdsT = V[2][kCgBY + 4][kCgLZ];
 
 // Offsets inside a plane are 32-bit (planes of fewer than kMaxMarchPlane
 // voxels, which the launchers check): a load is then a uniform plane pointer
@@ -148,11 +163,17 @@ struct CgTile {
     int x0, x1, y, z;
     bool inyz;
     uint32_t yz;   // element offset of (y, z) in a plane
-    uint32_t yzb;  // the same in bytes of a double
+    uint32_t yzb;  // the same in bytes of a double (vb<V>() gives it for V)
     bool halo_thread, halo_in;
     int ly, lz;
-    uint32_t hyzb;  // byte offset of this thread's halo point
+    uint32_t hyzb;  // byte offset of this thread's halo point (double)
 };
+
+// A double's byte offset b8 as the byte offset of the same element of a V array.
+template <typename V>
+__device__ __forceinline__ uint32_t vb(uint32_t b8) {
+    return sizeof(V) == 8 ? b8 : b8 >> 1;
+}
 
 // Plane xx of f, clamped into the grid: a uniform pointer. Values read from a
 // clamped plane only feed terms that the boundary masks drop.
@@ -196,11 +217,13 @@ __device__ inline CgTile cg_tile(const Grid& g, int x0, int x1, int y0, int z0) 
 // A p at an interior voxel: x terms from the queue (a = x-2, c = x, e = x+2),
 // y and z terms from the LDS plane; rows next to the wall drop G on the
 // boundary layer (solver.py:1024-1026 with gradient_center zero there).
+template <typename V>
 __device__ inline double cg_apply(const Grid& g, int x, int y, int z, double a, double c, double e,
-                                  const double (*L)[kCgLZ], int r, int q, double qx, double qy, double qz) {
+                                  const V (*L)[kCgLZ], int r, int q, double qx, double qy, double qz) {
     // the four LDS reads are unconditional (always inside the tile's halo), so
     // they issue together and the masks below are selects, not branches
-    const double yp = L[r + 2][q], ym = L[r - 2][q], zp = L[r][q + 2], zm = L[r][q - 2];
+    const double yp = (double)L[r + 2][q], ym = (double)L[r - 2][q], zp = (double)L[r][q + 2],
+                 zm = (double)L[r][q - 2];
     const double tx = ((x + 1 < g.nx - 1) ? (e - c) : 0.0) - ((x - 1 > 0) ? (c - a) : 0.0);
     const double ty = ((y + 1 < g.ny - 1) ? (yp - c) : 0.0) - ((y - 1 > 0) ? (c - ym) : 0.0);
     const double tz = ((z + 1 < g.nz - 1) ? (zp - c) : 0.0) - ((z - 1 > 0) ? (c - zm) : 0.0);
@@ -252,28 +275,34 @@ __device__ __forceinline__ void march8(int x0, int x1, F&& body) {
 }
 
 // r and p_old of one point, as loaded.
+template <typename V>
 struct CgRaw {
-    double r, q;
+    V r, q;
 };
 
-// p = r + beta p_old (one fma, so every recomputation is bit-identical), A p.
-// Writes p over the tile; returns this thread's share of p.Ap.
-__device__ inline double cg_dir_tile(const double* __restrict__ r, const double* __restrict__ p_old,
-                                     double* __restrict__ p, const Grid& g, double beta, double qx, double qy,
-                                     double qz, const CgTile& t, CgLds& lds) {
+// p = r + beta p_old (one fma, rounded to V, so every recomputation is
+// bit-identical to the stored p), A p. Writes p over the tile; returns this
+// thread's share of p.Ap.
+template <typename V>
+__device__ inline double cg_dir_tile(const V* __restrict__ r, const V* __restrict__ p_old, V* __restrict__ p,
+                                     const Grid& g, double beta, double qx, double qy, double qz, const CgTile& t,
+                                     CgLdsT<V>& lds) {
     const int r0 = threadIdx.y + 2, q0 = threadIdx.x + 2;
+    const uint32_t cb = vb<V>(t.yzb), hb = vb<V>(t.hyzb);
     // plain loads: these lines also serve the halos of the neighbouring tiles
-    auto raw = [&](int xx, uint32_t o) { return CgRaw{at_byte(plane_at(r, xx, g), o), at_byte(plane_at(p_old, xx, g), o)}; };
-    auto center = [&](int xx) { return raw(xx, t.yzb); };
+    auto raw = [&](int xx, uint32_t o) {
+        return CgRaw<V>{at_byte(plane_at(r, xx, g), o), at_byte(plane_at(p_old, xx, g), o)};
+    };
+    auto center = [&](int xx) { return raw(xx, cb); };
     // halo loads are unconditional (lanes without a halo point read offset 0),
     // so no branch separates a load from the wait that precedes its use
-    auto halo = [&](int xx) { return raw(xx, t.hyzb); };
-    auto pv = [&](const CgRaw& v) { return fma(beta, v.q, v.r); };
+    auto halo = [&](int xx) { return raw(xx, hb); };
+    auto pv = [&](const CgRaw<V>& v) { return as_stored<V>(fma(beta, (double)v.q, (double)v.r)); };
     const int x0 = t.x0, x1 = t.x1;
     constexpr int D = kCgDistDir;
-    double pq[8];  // p of this column at plane x0 + k in slot k & 7 (planes x-2 .. x+2 live)
-    CgRaw rq[8];   // raw loads of this column in flight, slot k & 7 (planes x+3 .. x+2+D)
-    CgRaw hq[8];   // raw loads of this thread's halo point, slot k & 7 (planes x .. x+D)
+    double pq[8];    // p of this column at plane x0 + k in slot k & 7 (planes x-2 .. x+2 live)
+    CgRaw<V> rq[8];  // raw loads of this column in flight, slot k & 7 (planes x+3 .. x+2+D)
+    CgRaw<V> hq[8];  // raw loads of this thread's halo point, slot k & 7 (planes x .. x+D)
 #pragma unroll
     for (int k = -2; k < 2; ++k) pq[k & 7] = pv(center(x0 + k));
 #pragma unroll
@@ -287,70 +316,75 @@ __device__ inline double cg_dir_tile(const double* __restrict__ r, const double*
         const double e = pv(rq[(K + 2) & 7]);  // plane x+2, loaded D iterations ago
         pq[(K + 2) & 7] = e;
         const double a = pq[(K + 6) & 7], c = pq[K & 7];
-        lds[B][r0][q0] = c;
-        if (t.halo_thread) lds[B][t.ly][t.lz] = pv(hq[K & 7]);
+        lds[B][r0][q0] = (V)c;
+        if (t.halo_thread) lds[B][t.ly][t.lz] = (V)pv(hq[K & 7]);
         rq[(K + 2 + D) & 7] = center(x + 2 + D);
         hq[(K + D) & 7] = halo(x + D);
         __syncthreads();
         if (!t.inyz) return;
         const bool in = g.interior(x, t.y, t.z);
-        if (in) s += c * cg_apply(g, x, t.y, t.z, a, c, e, lds[B], r0, q0, qx, qy, qz);
-        at_byte(plane_at(p, x, g), t.yzb) = in ? c : 0.0;
+        if (in) s += c * cg_apply<V>(g, x, t.y, t.z, a, c, e, lds[B], r0, q0, qx, qy, qz);
+        at_byte(plane_at(p, x, g), cb) = in ? (V)c : (V)0;
     });
     return s;
 }
 
 // phi += alpha p, r -= alpha A p (A p recomputed from p) over the tile;
-// returns this thread's share of r.r. r and phi are read and written once
-// here (non-temporal); p keeps plain loads, its lines serve the halos of the
-// neighbouring tiles.
-__device__ inline double cg_step_tile(double* __restrict__ phi, double* __restrict__ r, const double* __restrict__ p,
+// returns this thread's share of r.r (over r as stored). r and phi are read
+// and written once here (non-temporal); p keeps plain loads, its lines serve
+// the halos of the neighbouring tiles. Loaded values stay in V until the
+// iteration that consumes them.
+template <typename V>
+__device__ inline double cg_step_tile(double* __restrict__ phi, V* __restrict__ r, const V* __restrict__ p,
                                       const Grid& g, double alpha, double qx, double qy, double qz, const CgTile& t,
-                                      CgLds& lds) {
+                                      CgLdsT<V>& lds) {
     const int r0 = threadIdx.y + 2, q0 = threadIdx.x + 2;
-    auto center = [&](int xx) { return at_byte(plane_at(p, xx, g), t.yzb); };
-    auto halo = [&](int xx) { return at_byte(plane_at(p, xx, g), t.hyzb); };
-    auto once = [&](double* f, int xx) { return load_once(&at_byte(plane_at(f, xx, g), t.yzb)); };
+    const uint32_t cb = vb<V>(t.yzb), hb = vb<V>(t.hyzb);
+    auto center = [&](int xx) { return at_byte(plane_at(p, xx, g), cb); };
+    auto halo = [&](int xx) { return at_byte(plane_at(p, xx, g), hb); };
+    auto once_r = [&](int xx) { return load_once(&at_byte(plane_at(r, xx, g), cb)); };
+    auto once_phi = [&](int xx) { return load_once(&at_byte(plane_at(phi, xx, g), t.yzb)); };
     const int x0 = t.x0, x1 = t.x1;
-    double pq[8];  // p of this column, slot k & 7 (planes x-2 .. x+2+D live)
-    double hq[4];  // p at this thread's halo point, slot k & 3 (planes x .. x+D)
-    double rq[4];  // r of this column, slot k & 3
+    V pq[8];      // p of this column, slot k & 7 (planes x-2 .. x+2+D live)
+    V hq[4];      // p at this thread's halo point, slot k & 3 (planes x .. x+D)
+    V rq[4];      // r of this column, slot k & 3
     double fq[4];  // phi of this column, slot k & 3
 #pragma unroll
     for (int k = -2; k < 2 + kCgDist; ++k) pq[k & 7] = center(x0 + k);
 #pragma unroll
     for (int k = 0; k < kCgDist; ++k) {
         hq[k & 3] = halo(x0 + k);
-        rq[k & 3] = once(r, x0 + k);
-        fq[k & 3] = once(phi, x0 + k);
+        rq[k & 3] = once_r(x0 + k);
+        fq[k & 3] = once_phi(x0 + k);
     }
     double s = 0.0;
     march8(x0, x1, [&](auto kc, int x) {
         constexpr int K = decltype(kc)::value;
         constexpr int B = K & 1;
-        const double a = pq[(K + 6) & 7], c = pq[K & 7], e = pq[(K + 2) & 7];
-        lds[B][r0][q0] = c;
+        const V cv = pq[K & 7];
+        const double a = (double)pq[(K + 6) & 7], c = (double)cv, e = (double)pq[(K + 2) & 7];
+        lds[B][r0][q0] = cv;
         if (t.halo_thread) lds[B][t.ly][t.lz] = hq[K & 3];
         pq[(K + 2 + kCgDist) & 7] = center(x + 2 + kCgDist);
         hq[(K + kCgDist) & 3] = halo(x + kCgDist);
-        rq[(K + kCgDist) & 3] = once(r, x + kCgDist);
-        fq[(K + kCgDist) & 3] = once(phi, x + kCgDist);
+        rq[(K + kCgDist) & 3] = once_r(x + kCgDist);
+        fq[(K + kCgDist) & 3] = once_phi(x + kCgDist);
         __syncthreads();
         if (!t.inyz || !g.interior(x, t.y, t.z)) return;
-        const double ap = cg_apply(g, x, t.y, t.z, a, c, e, lds[B], r0, q0, qx, qy, qz);
+        const double ap = cg_apply<V>(g, x, t.y, t.z, a, c, e, lds[B], r0, q0, qx, qy, qz);
         store_out(&at_byte(plane_at(phi, x, g), t.yzb), fq[K & 3] + alpha * c);
-        const double rv = rq[K & 3] - alpha * ap;
-        store_out(&at_byte(plane_at(r, x, g), t.yzb), rv);
-        s += rv * rv;
+        const V rv = (V)((double)rq[K & 3] - alpha * ap);
+        store_out(&at_byte(plane_at(r, x, g), cb), rv);
+        s += (double)rv * (double)rv;
     });
     return s;
 }
 
 // r0 = -div_c(mask f) over a tile (no LDS: each f component is read at +-1
 // along its own axis only), p_old = 0, phi = 0. Returns r.r and max|div|.
-template <typename T>
+template <typename T, typename V>
 __device__ inline void cg_init_tile(const T* __restrict__ fx, const T* __restrict__ fy, const T* __restrict__ fz,
-                                    double* __restrict__ phi, double* __restrict__ r, double* __restrict__ p_old,
+                                    double* __restrict__ phi, V* __restrict__ r, V* __restrict__ p_old,
                                     const Grid& g, double i2hx, double i2hy, double i2hz, const CgTile& t, double& s,
                                     double& mx) {
     if (!t.inyz) return;
@@ -368,12 +402,12 @@ __device__ inline void cg_init_tile(const T* __restrict__ fx, const T* __restric
             const double zp = (t.z + 1 < g.nz - 1) ? (double)fz[i + 1] : 0.0;
             const double zm = (t.z - 1 > 0) ? (double)fz[i - 1] : 0.0;
             const double div = (xp - xm) * i2hx + (yp - ym) * i2hy + (zp - zm) * i2hz;
-            rv = -div;
+            rv = as_stored<V>(-div);
             s += rv * rv;
             mx = fmax(mx, fabs(div));
         }
-        r[i] = rv;
-        p_old[i] = 0.0;
+        r[i] = (V)rv;
+        p_old[i] = (V)0;
         phi[i] = 0.0;
     }
 }

@@ -12,6 +12,8 @@
 //                                                     (reads phi, r, p; writes phi, r)
 // i.e. 64 B per voxel per iteration in float64 instead of the textbook 88 B,
 // with bit-identical arithmetic (Ap is recomputed by the same operations).
+// With float32 vectors (r, p_old, p in float; phi in double) it is 40 B; the
+// products, dot products and residual norm are float64 either way.
 // Scalars (alpha, beta, rs, stop flag) live in device memory; the last block
 // of each kernel reduces the per-block partials in a fixed order (so results
 // are deterministic) and applies the stopping rules. The host only polls the
@@ -31,10 +33,10 @@ constexpr int kMaxRedBlocks = 2048;
 __global__ void cg_reset_kernel(CgState* st, double tol, long long max_iters) { cg_reset(st, tol, max_iters); }
 
 // r = -div_c(mask f) on the interior, p_old = 0, phi = 0; reduces r.r and max|div|.
-template <typename T>
+template <typename T, typename V>
 __global__ void __launch_bounds__(256) cg_init_kernel(const T* __restrict__ fx, const T* __restrict__ fy,
                                                       const T* __restrict__ fz, double* __restrict__ phi,
-                                                      double* __restrict__ r, double* __restrict__ p_old, Grid g,
+                                                      V* __restrict__ r, V* __restrict__ p_old, Grid g,
                                                       double i2hx, double i2hy, double i2hz, double* part_sum,
                                                       double* part_max, CgState* st) {
     __shared__ double red[16];
@@ -52,12 +54,12 @@ __global__ void __launch_bounds__(256) cg_init_kernel(const T* __restrict__ fx, 
             const double zp = (z + 1 < g.nz - 1) ? (double)fz[i + 1] : 0.0;
             const double zm = (z - 1 > 0) ? (double)fz[i - 1] : 0.0;
             const double div = (xp - xm) * i2hx + (yp - ym) * i2hy + (zp - zm) * i2hz;
-            rv = -div;
+            rv = as_stored<V>(-div);
             s += rv * rv;
             mx = fmax(mx, fabs(div));
         }
-        r[i] = rv;
-        p_old[i] = 0.0;
+        r[i] = (V)rv;
+        p_old[i] = (V)0;
         phi[i] = 0.0;
     });
     s = block_sum(s, red);
@@ -75,9 +77,10 @@ __global__ void __launch_bounds__(256) cg_init_kernel(const T* __restrict__ fx, 
     ((((c) + 1 < (n) - 1) ? ((v)(2) - (v)(0)) : 0.0) - (((c) - 1 > 0) ? ((v)(0) - (v)(-2)) : 0.0)) * (q)
 
 // Direction update fused with A p and p.Ap.
-__global__ void __launch_bounds__(256) cg_dir_kernel(const double* __restrict__ r, const double* __restrict__ p_old,
-                                                     double* __restrict__ p, Grid g, double qx, double qy,
-                                                     double qz, double* part_sum, CgState* st) {
+template <typename V>
+__global__ void __launch_bounds__(256) cg_dir_kernel(const V* __restrict__ r, const V* __restrict__ p_old,
+                                                     V* __restrict__ p, Grid g, double qx, double qy, double qz,
+                                                     double* part_sum, CgState* st) {
     if (st->stop) return;
     __shared__ double red[16];
     const double beta = st->beta;
@@ -87,7 +90,7 @@ __global__ void __launch_bounds__(256) cg_dir_kernel(const double* __restrict__ 
     SW_FOR_EACH_VOXEL(g, {
         double pv = 0.0;
         if (g.interior(x, y, z)) {
-            auto pn = [&](int64_t k) { return fma(beta, p_old[i + k], r[i + k]); };
+            auto pn = [&](int64_t k) { return as_stored<V>(fma(beta, (double)p_old[i + k], (double)r[i + k])); };
             pv = pn(0);
             auto px = [&](int k) { return k == 0 ? pv : pn(k * P); };
             auto py = [&](int k) { return k == 0 ? pv : pn(k * R); };
@@ -96,7 +99,7 @@ __global__ void __launch_bounds__(256) cg_dir_kernel(const double* __restrict__ 
                                 SW_AXIS_TERM(pz, z, g.nz, qz));
             s += pv * ap;
         }
-        p[i] = pv;
+        p[i] = (V)pv;
     });
     s = block_sum(s, red);
     if (publish_slot(part_sum, nullptr, blockIdx.x, s, 0.0, &st->counter, gridDim.x)) {
@@ -107,8 +110,9 @@ __global__ void __launch_bounds__(256) cg_dir_kernel(const double* __restrict__ 
 }
 
 // phi += alpha p, r -= alpha A p (A p recomputed), r.r and the stopping rules.
-__global__ void __launch_bounds__(256) cg_step_kernel(double* __restrict__ phi, double* __restrict__ r,
-                                                      const double* __restrict__ p, Grid g, double qx, double qy,
+template <typename V>
+__global__ void __launch_bounds__(256) cg_step_kernel(double* __restrict__ phi, V* __restrict__ r,
+                                                      const V* __restrict__ p, Grid g, double qx, double qy,
                                                       double qz, double* part_sum, CgState* st) {
     if (st->stop) return;
     __shared__ double red[16];
@@ -118,16 +122,16 @@ __global__ void __launch_bounds__(256) cg_step_kernel(double* __restrict__ phi, 
     double s = 0.0;
     SW_FOR_EACH_VOXEL(g, {
         if (g.interior(x, y, z)) {
-            const double pv = p[i];
-            auto px = [&](int k) { return k == 0 ? pv : p[i + k * P]; };
-            auto py = [&](int k) { return k == 0 ? pv : p[i + k * R]; };
-            auto pz = [&](int k) { return k == 0 ? pv : p[i + k]; };
+            const double pv = (double)p[i];
+            auto px = [&](int k) { return k == 0 ? pv : (double)p[i + k * P]; };
+            auto py = [&](int k) { return k == 0 ? pv : (double)p[i + k * R]; };
+            auto pz = [&](int k) { return k == 0 ? pv : (double)p[i + k]; };
             const double ap = -(SW_AXIS_TERM(px, x, g.nx, qx) + SW_AXIS_TERM(py, y, g.ny, qy) +
                                 SW_AXIS_TERM(pz, z, g.nz, qz));
             phi[i] = phi[i] + alpha * pv;
-            const double rv = r[i] - alpha * ap;
+            const V rv = (V)((double)r[i] - alpha * ap);
             r[i] = rv;
-            s += rv * rv;
+            s += (double)rv * (double)rv;
         }
     });
     s = block_sum(s, red);
@@ -152,16 +156,16 @@ __device__ inline CgTile cg_block_tile(const Grid& g, int xchunk, int nty, int n
     return cg_tile(g, x0, min(x0 + xchunk, g.nx), tyi * kCgBY, tzi * 64);
 }
 
-__global__ void __launch_bounds__(64 * kCgBY) cg_dir_march_kernel(const double* __restrict__ r,
-                                                                  const double* __restrict__ p_old,
-                                                                  double* __restrict__ p, Grid g, double qx,
-                                                                  double qy, double qz, int xchunk, int nty, int ntz,
+template <typename V>
+__global__ void __launch_bounds__(64 * kCgBY) cg_dir_march_kernel(const V* __restrict__ r, const V* __restrict__ p_old,
+                                                                  V* __restrict__ p, Grid g, double qx, double qy,
+                                                                  double qz, int xchunk, int nty, int ntz,
                                                                   double* part_sum, CgState* st) {
     if (st->stop) return;
-    __shared__ CgLds lds;
+    __shared__ CgLdsT<V> lds;
     __shared__ double red[16];
     const CgTile t = cg_block_tile(g, xchunk, nty, ntz);
-    double s = cg_dir_tile(r, p_old, p, g, st->beta, qx, qy, qz, t, lds);
+    double s = cg_dir_tile<V>(r, p_old, p, g, st->beta, qx, qy, qz, t, lds);
     s = block_sum(s, red);
     if (publish_slot(part_sum, nullptr, blockIdx.x, s, 0.0, &st->counter, gridDim.x)) {
         double denom, unused;
@@ -170,15 +174,16 @@ __global__ void __launch_bounds__(64 * kCgBY) cg_dir_march_kernel(const double* 
     }
 }
 
-__global__ void __launch_bounds__(64 * kCgBY) cg_step_march_kernel(double* __restrict__ phi, double* __restrict__ r,
-                                                                   const double* __restrict__ p, Grid g, double qx,
+template <typename V>
+__global__ void __launch_bounds__(64 * kCgBY) cg_step_march_kernel(double* __restrict__ phi, V* __restrict__ r,
+                                                                   const V* __restrict__ p, Grid g, double qx,
                                                                    double qy, double qz, int xchunk, int nty, int ntz,
                                                                    double* part_sum, CgState* st) {
     if (st->stop) return;
-    __shared__ CgLds lds;
+    __shared__ CgLdsT<V> lds;
     __shared__ double red[16];
     const CgTile t = cg_block_tile(g, xchunk, nty, ntz);
-    double s = cg_step_tile(phi, r, p, g, st->alpha, qx, qy, qz, t, lds);
+    double s = cg_step_tile<V>(phi, r, p, g, st->alpha, qx, qy, qz, t, lds);
     s = block_sum(s, red);
     if (publish_slot(part_sum, nullptr, blockIdx.x, s, 0.0, &st->counter, gridDim.x)) {
         double rs_new, unused;
@@ -247,8 +252,12 @@ __global__ void __launch_bounds__(256) grad_sub_kernel(T* __restrict__ fx, T* __
     });
 }
 
+// phi is float64; r, p0 and p1 hold float64 or float32 vectors (each slot is
+// sized for float64, so one workspace size serves both).
 struct Workspace {
-    double *phi, *r, *p0, *p1, *part_sum, *part_max;
+    double* phi;
+    void *r, *p0, *p1;
+    double *part_sum, *part_max;
     CgState* st;
 };
 
@@ -271,11 +280,11 @@ Workspace carve(void* base, int64_t nx, int64_t ny, int64_t nz) {
     Workspace w;
     w.phi = (double*)p;
     p += align256(n * 8);
-    w.r = (double*)p;
+    w.r = p;
     p += align256(n * 8);
-    w.p0 = (double*)p;
+    w.p0 = p;
     p += align256(n * 8);
-    w.p1 = (double*)p;
+    w.p1 = p;
     p += align256(n * 8);
     w.part_sum = (double*)p;
     p += align256(pc * 8);
@@ -291,6 +300,46 @@ unsigned red_grid(const Grid& g) {
     return row_grid(g, cap);
 }
 
+// Launch the CG from r0 to its stop on the stream; returns the final state
+// in hs (host). V is the storage of r and p.
+template <typename V>
+int run_cg(const Workspace& w, const Grid& g, bool march, const CgShape& cs, unsigned nb, double qx, double qy,
+           double qz, int64_t max_iters, CgState& hs, hipStream_t s) {
+    const dim3 blk = row_block();
+    SW_TRY_HIP(hipMemcpyAsync(&hs, w.st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+    SW_TRY_HIP(hipStreamSynchronize(s));
+    int64_t launched = 0;
+    int64_t chunk = 8;
+    V* r = (V*)w.r;
+    V* p_old = (V*)w.p0;
+    V* p_new = (V*)w.p1;
+    while (!hs.stop && launched < max_iters) {
+        const int64_t k = (max_iters - launched < chunk) ? (max_iters - launched) : chunk;
+        for (int64_t j = 0; j < k; ++j) {
+            if (march) {
+                hipLaunchKernelGGL(cg_dir_march_kernel<V>, dim3(cs.blocks()), dim3(64, kCgBY), 0, s, r, p_old, p_new,
+                                   g, qx, qy, qz, cs.xchunk, cs.nty, cs.ntz, w.part_sum, w.st);
+                hipLaunchKernelGGL(cg_step_march_kernel<V>, dim3(cs.blocks()), dim3(64, kCgBY), 0, s, w.phi, r, p_new,
+                                   g, qx, qy, qz, cs.xchunk, cs.nty, cs.ntz, w.part_sum, w.st);
+            } else {
+                hipLaunchKernelGGL(cg_dir_kernel<V>, dim3(nb), blk, 0, s, r, p_old, p_new, g, qx, qy, qz, w.part_sum,
+                                   w.st);
+                hipLaunchKernelGGL(cg_step_kernel<V>, dim3(nb), blk, 0, s, w.phi, r, p_new, g, qx, qy, qz, w.part_sum,
+                                   w.st);
+            }
+            V* t = p_old;
+            p_old = p_new;
+            p_new = t;
+        }
+        SW_LAUNCH_CHECK();
+        launched += k;
+        SW_TRY_HIP(hipMemcpyAsync(&hs, w.st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+        SW_TRY_HIP(hipStreamSynchronize(s));
+        if (chunk < 64) chunk *= 2;
+    }
+    return SW_OK;
+}
+
 }  // namespace
 }  // namespace sw
 
@@ -303,12 +352,13 @@ int64_t sw_project_workspace_bytes(int64_t nx, int64_t ny, int64_t nz) {
     return workspace_bytes(nx, ny, nz);
 }
 
-int sw_project_vector_field(int dtype, void* fx, void* fy, void* fz, int64_t nx, int64_t ny, int64_t nz, double dx,
-                            double dy, double dz, double tol, int64_t max_iters, void* workspace,
-                            int64_t workspace_bytes_given, void* const* scale_out, const void* inv_rho, double scale,
-                            sw_cg_info* info, void* stream) {
+int sw_project_vector_field(int dtype, int vec_dtype, void* fx, void* fy, void* fz, int64_t nx, int64_t ny,
+                            int64_t nz, double dx, double dy, double dz, double tol, int64_t max_iters,
+                            void* workspace, int64_t workspace_bytes_given, void* const* scale_out,
+                            const void* inv_rho, double scale, sw_cg_info* info, void* stream) {
     int rc = check_dtype(dtype);
     if (rc) return rc;
+    if ((rc = check_dtype(vec_dtype))) return rc;
     if ((rc = check_grid(nx, ny, nz))) return rc;
     SW_REQUIRE(fx && fy && fz, SW_ERR_ARG, "null force component");
     SW_REQUIRE(max_iters >= 0, SW_ERR_ARG, "max_iters must be >= 0");
@@ -334,41 +384,21 @@ int sw_project_vector_field(int dtype, void* fx, void* fy, void* fz, int64_t nx,
 
     hipLaunchKernelGGL(cg_reset_kernel, dim3(1), dim3(1), 0, s, w.st, tol, (long long)max_iters);
     SW_LAUNCH_CHECK();
-    SW_DISPATCH(dtype, T,
-                hipLaunchKernelGGL(cg_init_kernel<T>, dim3(nb), blk, 0, s, (const T*)fx, (const T*)fy, (const T*)fz,
-                                   w.phi, w.r, w.p0, g, i2hx, i2hy, i2hz, w.part_sum, w.part_max, w.st));
-    SW_LAUNCH_CHECK();
-
     CgState hs;
-    SW_TRY_HIP(hipMemcpyAsync(&hs, w.st, sizeof(CgState), hipMemcpyDeviceToHost, s));
-    SW_TRY_HIP(hipStreamSynchronize(s));
-    int64_t launched = 0;
-    int64_t chunk = 8;
-    double* p_old = w.p0;
-    double* p_new = w.p1;
-    while (!hs.stop && launched < max_iters) {
-        const int64_t k = (max_iters - launched < chunk) ? (max_iters - launched) : chunk;
-        for (int64_t j = 0; j < k; ++j) {
-            if (march) {
-                hipLaunchKernelGGL(cg_dir_march_kernel, dim3(cs.blocks()), dim3(64, kCgBY), 0, s, w.r, p_old, p_new,
-                                   g, qx, qy, qz, cs.xchunk, cs.nty, cs.ntz, w.part_sum, w.st);
-                hipLaunchKernelGGL(cg_step_march_kernel, dim3(cs.blocks()), dim3(64, kCgBY), 0, s, w.phi, w.r, p_new,
-                                   g, qx, qy, qz, cs.xchunk, cs.nty, cs.ntz, w.part_sum, w.st);
-            } else {
-                hipLaunchKernelGGL(cg_dir_kernel, dim3(nb), blk, 0, s, w.r, p_old, p_new, g, qx, qy, qz, w.part_sum,
-                                   w.st);
-                hipLaunchKernelGGL(cg_step_kernel, dim3(nb), blk, 0, s, w.phi, w.r, p_new, g, qx, qy, qz, w.part_sum,
-                                   w.st);
-            }
-            double* t = p_old;
-            p_old = p_new;
-            p_new = t;
-        }
+    if (vec_dtype == SW_F32) {
+        SW_DISPATCH(dtype, T,
+                    hipLaunchKernelGGL((cg_init_kernel<T, float>), dim3(nb), blk, 0, s, (const T*)fx, (const T*)fy,
+                                       (const T*)fz, w.phi, (float*)w.r, (float*)w.p0, g, i2hx, i2hy, i2hz,
+                                       w.part_sum, w.part_max, w.st));
         SW_LAUNCH_CHECK();
-        launched += k;
-        SW_TRY_HIP(hipMemcpyAsync(&hs, w.st, sizeof(CgState), hipMemcpyDeviceToHost, s));
-        SW_TRY_HIP(hipStreamSynchronize(s));
-        if (chunk < 64) chunk *= 2;
+        if ((rc = run_cg<float>(w, g, march, cs, nb, qx, qy, qz, max_iters, hs, s))) return rc;
+    } else {
+        SW_DISPATCH(dtype, T,
+                    hipLaunchKernelGGL((cg_init_kernel<T, double>), dim3(nb), blk, 0, s, (const T*)fx, (const T*)fy,
+                                       (const T*)fz, w.phi, (double*)w.r, (double*)w.p0, g, i2hx, i2hy, i2hz,
+                                       w.part_sum, w.part_max, w.st));
+        SW_LAUNCH_CHECK();
+        if ((rc = run_cg<double>(w, g, march, cs, nb, qx, qy, qz, max_iters, hs, s))) return rc;
     }
 
     SW_DISPATCH(dtype, T,

@@ -101,19 +101,23 @@ def static_workspace(shape) -> torch.Tensor:
     return _workspace("static", shape, lib().sw_static_workspace_bytes)
 
 
-def project_inplace(f, spacing, tol, max_iters, *, scale_out=None, inv_rho=None, scale=1.0) -> dict:
+def project_inplace(f, spacing, tol, max_iters, *, scale_out=None, inv_rho=None, scale=1.0, vectors=None) -> dict:
     """Helmholtz-Hodge projection of f (3, nx, ny, nz) in place on the device.
 
     Optionally writes scale * f_S * inv_rho into scale_out (3, nx, ny, nz).
-    Returns the CG report: iters, relres, max_div, reason.
+    ``vectors`` is the storage of the CG vectors r and p (``"float32"`` or
+    ``"float64"``; default: f's dtype); phi, the dot products and the residual
+    norm are float64 either way. Returns the CG report: iters, relres,
+    max_div, reason.
     """
     shape = tuple(f.shape[1:])
     ws = project_workspace(shape)
     info = _abi.CgInfo()
     so = ptrs(*scale_out) if scale_out is not None else None
+    vec = sw_code(f.dtype if vectors is None else vectors)
     check(
         lib().sw_project_vector_field(
-            sw_code(f.dtype), f[0].data_ptr(), f[1].data_ptr(), f[2].data_ptr(), *shape, *spacing, float(tol),
+            sw_code(f.dtype), vec, f[0].data_ptr(), f[1].data_ptr(), f[2].data_ptr(), *shape, *spacing, float(tol),
             int(max_iters), ws.data_ptr(), ws.numel(), so, None if inv_rho is None else inv_rho.data_ptr(),
             float(scale), C.byref(info), stream_ptr(),
         )
@@ -137,12 +141,14 @@ class Solve:
     """
 
     def __init__(self, shape, spacing, dt, rho, mu, eta=None, *, dtype="float32", form="increment",
-                 vel_damp=0.0, kernel="auto", tile_y=0, x_chunk=0, tile_z=0):
+                 vel_damp=0.0, kernel="auto", tile_y=0, x_chunk=0, tile_z=0, cg_vectors=None):
         self.shape = tuple(int(s) for s in shape)
         self.spacing = tuple(float(h) for h in spacing)
         self.dt = float(dt)
         self.dtype = dtype_name(dtype)
         self.code = sw_code(self.dtype)
+        # storage of the projection's CG vectors r and p (phi and the reductions stay float64)
+        self.cg_vectors = self.dtype if cg_vectors is None else dtype_name(cg_vectors)
         self.form = form
         self.n = int(np.prod(self.shape))
         tdt = torch_dtype(self.dtype)
@@ -200,7 +206,7 @@ class Solve:
             for c in range(3):
                 self.state.force[c] = self.force[c].data_ptr()
         info = project_inplace(f, self.spacing, tol, max_iters, scale_out=self.force, inv_rho=self.inv_rho,
-                               scale=self.dt * self.dt)
+                               scale=self.dt * self.dt, vectors=self.cg_vectors)
         return info, f
 
     def set_initial(self, u0=None, v0=None):
@@ -284,8 +290,7 @@ class Solve:
             check(lib().sw_combine3(self.code, ptrs(*u), ptrs(*b), 1.0, -1.0, ptrs(*prev), self.n, stream_ptr()))
         else:
             prev = torch.stack(b)
-        infos = [project_inplace(uvec, self.spacing, tol, max_iters), project_inplace(prev, self.spacing, tol,
-                                                                                        max_iters)]
+        infos = [project_inplace(v, self.spacing, tol, max_iters, vectors=self.cg_vectors) for v in (uvec, prev)]
         for c in range(3):
             u[c].copy_(uvec[c])
         if self.form == "increment":

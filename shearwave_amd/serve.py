@@ -2,7 +2,7 @@
 
 A request (one solve) arrives at some time; the server admits it as soon as
 its HBM footprint fits (fields, state, force and, during its projection, the
-float64 CG vectors) and a slot is free. Admission (upload, coefficients, and
+CG vectors: phi in float64, r and p in float32) and a slot is free. Admission (upload, coefficients, and
 one batched projection of every request admitted together) runs on a second
 stream, so the solves already on the GPU keep stepping meanwhile. Admitted
 solves join the running batch at the next tick: every tick is `tick_passes`

@@ -106,6 +106,38 @@ def test_projection_fp32_is_divergence_free(sw):
         assert rel(a, b) < 1e-6
 
 
+@pytest.mark.parametrize("variant", ["march", "direct"])
+def test_projection_float32_cg_vectors(sw, variant, monkeypatch):
+    """r and p stored in float32 (phi, dot products and residual norm in float64).
+
+    Same stopping rule and iteration count as float64 vectors to within 2%;
+    projected force within 1e-6 (rel L2) of the float64 oracle's, and its
+    divergence reduced below 1e-6 of the input's.
+    """
+    if variant == "direct":
+        monkeypatch.setenv("SW_CG_KERNEL", "direct")
+    rng = np.random.default_rng(11)
+    shape = (40, 21, 70)  # partial y and z tiles
+    f = [(rng.standard_normal(shape) * taper(shape)).astype(np.float32) for _ in range(2)]
+    f.append((300.0 * gauss(shape, 4.0) + rng.standard_normal(shape) * taper(shape)).astype(np.float32))
+    runs = {}
+    for vec in ("float32", "float64"):
+        info = {}
+        out = sw.project_vector_field(*f, *H, 1e-9, 5000, info=info, cg_vectors=vec)
+        assert out[0].dtype == np.float32 and info["reason"] == "converged", (vec, info)
+        runs[vec] = (out, info)
+    k32, k64 = runs["float32"][1]["iters"], runs["float64"][1]["iters"]
+    assert abs(k32 - k64) <= 2 + 0.02 * k64, (k32, k64)
+    f64 = [a.astype(np.float64) for a in f]
+    want = O.project_vector_field(*f64, *H, 1e-13, 5000, dtype=np.float64)
+    d_in = np.linalg.norm(O.divergence_center(*f64, *H))
+    out = runs["float32"][0]
+    d_out = np.linalg.norm(O.divergence_center(*(a.astype(np.float64) for a in out), *H))
+    assert d_out / d_in < 1e-6, d_out / d_in
+    for a, b in zip(out, want):
+        assert rel(a, b) < 1e-6
+
+
 def _grid_ref(n=32):
     d = 1.0 / n
     ax = np.linspace(0.0, 1.0, n, dtype=np.float32)
