@@ -212,10 +212,16 @@ __global__ void __launch_bounds__(64 * kCgBY) batch_cg_step_kernel(const sw_batc
         if (st->stop) continue;
         const Grid g{S.nx, S.ny, S.nz};
         const CgTile t = batch_cg_tile(S, it);
+        // iteration k = iters + 1 wrote p into p1 (k odd) or p0 (k even); phi is
+        // updated on even iterations for both (march_cg.h), so p_prev is the other buffer
         const bool odd = st->iters & 1;
         const float* p = odd ? S.p0 : S.p1;
-        double s = cg_step_tile<float>(S.phi, S.r, p, g, st->alpha, 0.25 / (S.dx * S.dx), 0.25 / (S.dy * S.dy),
-                                0.25 / (S.dz * S.dz), t, lds);
+        const float* p_prev = odd ? S.p1 : S.p0;
+        const double qx = 0.25 / (S.dx * S.dx), qy = 0.25 / (S.dy * S.dy), qz = 0.25 / (S.dz * S.dz);
+        double s = odd ? cg_step_tile<float, kPhiPair>(S.phi, S.r, p, p_prev, g, st->alpha, st->alpha_prev, qx, qy,
+                                                       qz, t, lds)
+                       : cg_step_tile<float, kPhiSkip>(S.phi, S.r, p, p_prev, g, st->alpha, st->alpha_prev, qx, qy,
+                                                       qz, t, lds);
         s = block_sum(s, red);
         if (publish_slot(S.part, nullptr, (int)(k - S.item0), s, 0.0, &st->counter, S.n_items)) {
             double rs_new, unused;
@@ -223,6 +229,21 @@ __global__ void __launch_bounds__(64 * kCgBY) batch_cg_step_kernel(const sw_batc
             if (threadIdx.x == 0 && threadIdx.y == 0) cg_after_step(st, rs_new);
         }
         __syncthreads();
+    }
+}
+
+// The deferred phi term of each solve that stopped after an odd iteration.
+__global__ void __launch_bounds__(64 * kCgBY) batch_cg_flush_kernel(const sw_batch_cg* __restrict__ solves,
+                                                                    const CgState* states,
+                                                                    const int4* __restrict__ items, int64_t n_items) {
+    const ItemWalk w = item_walk(n_items);
+    for (int64_t k = w.first; k < w.hi; k += w.stride) {
+        const int4 it = items[k];
+        const sw_batch_cg& S = solves[it.x];
+        const CgState* st = states + it.x;
+        if ((st->iters & 1) == 0) continue;
+        const Grid g{S.nx, S.ny, S.nz};
+        cg_flush_phi_tile<float>(S.phi, S.p1, g, st->alpha, batch_cg_tile(S, it));
     }
 }
 
@@ -367,6 +388,7 @@ int sw_batch_project(const sw_batch_cg* solves, int32_t n_solves, const int32_t*
         if (chunk < 64) chunk *= 2;
     }
     if (rc != SW_OK) return rc;
+    hipLaunchKernelGGL(batch_cg_flush_kernel, dim3(nb), blk, 0, s, solves, st, it, n_items);
     hipLaunchKernelGGL(batch_cg_grad_sub_kernel, dim3(nb), blk, 0, s, solves, it, n_items);
     SW_LAUNCH_CHECK();
     for (int k = 0; k < n_solves; ++k) {

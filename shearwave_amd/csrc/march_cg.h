@@ -14,7 +14,8 @@ namespace sw {
 
 // Device-resident state of one CG solve. The host only reads stop/reason/iters.
 struct CgState {
-    double rs_old, rs0, alpha, beta, relres, max_div, tol, pad0;
+    double rs_old, rs0, alpha, beta, relres, max_div, tol;
+    double alpha_prev;  // alpha of the iteration before (deferred phi updates)
     long long iters;
     int stop, reason;
     unsigned int counter;  // slots published in the current kernel
@@ -86,6 +87,7 @@ __device__ inline void cg_after_dir(CgState* st, double denom) {
         st->stop = 1;
         st->reason = SW_CG_DENOM;
     } else {
+        st->alpha_prev = st->alpha;
         st->alpha = st->rs_old / denom;
     }
 }
@@ -116,7 +118,7 @@ __device__ inline void cg_after_step(CgState* st, double rs_new) {
 
 // Fresh state for a solve with tolerance tol and at most max_iters iterations.
 __device__ inline void cg_reset(CgState* st, double tol, long long max_iters) {
-    st->rs_old = st->rs0 = st->alpha = st->beta = st->relres = st->max_div = 0.0;
+    st->rs_old = st->rs0 = st->alpha = st->beta = st->relres = st->max_div = st->alpha_prev = 0.0;
     st->tol = tol;
     st->iters = 0;
     st->stop = max_iters <= 0 ? 1 : 0;
@@ -329,33 +331,45 @@ __device__ inline double cg_dir_tile(const V* __restrict__ r, const V* __restric
     return s;
 }
 
-// phi += alpha p, r -= alpha A p (A p recomputed from p) over the tile;
-// returns this thread's share of r.r (over r as stored). r and phi are read
-// and written once here (non-temporal); p keeps plain loads, its lines serve
-// the halos of the neighbouring tiles. Loaded values stay in V until the
-// iteration that consumes them.
-template <typename V>
+// phi updates of the step tile. PHI 1: phi += alpha p every iteration.
+// Deferred (PHI 0, then PHI 2): an odd iteration k leaves phi alone, and
+// iteration k+1 adds both of its terms, phi = (phi + alpha_k p_k) +
+// alpha_{k+1} p_{k+1}, as two fmas in that order: bit-identical to updating
+// every iteration, with phi read and written every other iteration only (p_k
+// is still in the other direction buffer). A solve that stops after an odd
+// iteration adds its pending term with cg_flush_phi.
+enum : int { kPhiSkip = 0, kPhiEach = 1, kPhiPair = 2 };
+
+// phi update(s) per PHI, r -= alpha A p (A p recomputed from p) over the
+// tile; returns this thread's share of r.r (over r as stored). r and phi are
+// read and written once here (non-temporal); p keeps plain loads, its lines
+// serve the halos of the neighbouring tiles. Loaded values stay in V until
+// the iteration that consumes them.
+template <typename V, int PHI>
 __device__ inline double cg_step_tile(double* __restrict__ phi, V* __restrict__ r, const V* __restrict__ p,
-                                      const Grid& g, double alpha, double qx, double qy, double qz, const CgTile& t,
-                                      CgLdsT<V>& lds) {
+                                      const V* __restrict__ p_prev, const Grid& g, double alpha, double alpha_prev,
+                                      double qx, double qy, double qz, const CgTile& t, CgLdsT<V>& lds) {
     const int r0 = threadIdx.y + 2, q0 = threadIdx.x + 2;
     const uint32_t cb = vb<V>(t.yzb), hb = vb<V>(t.hyzb);
     auto center = [&](int xx) { return at_byte(plane_at(p, xx, g), cb); };
     auto halo = [&](int xx) { return at_byte(plane_at(p, xx, g), hb); };
     auto once_r = [&](int xx) { return load_once(&at_byte(plane_at(r, xx, g), cb)); };
     auto once_phi = [&](int xx) { return load_once(&at_byte(plane_at(phi, xx, g), t.yzb)); };
+    auto once_pp = [&](int xx) { return load_once(&at_byte(plane_at(p_prev, xx, g), cb)); };
     const int x0 = t.x0, x1 = t.x1;
-    V pq[8];      // p of this column, slot k & 7 (planes x-2 .. x+2+D live)
-    V hq[4];      // p at this thread's halo point, slot k & 3 (planes x .. x+D)
-    V rq[4];      // r of this column, slot k & 3
-    double fq[4];  // phi of this column, slot k & 3
+    V pq[8];       // p of this column, slot k & 7 (planes x-2 .. x+2+D live)
+    V hq[4];       // p at this thread's halo point, slot k & 3 (planes x .. x+D)
+    V rq[4];       // r of this column, slot k & 3
+    double fq[4];  // phi of this column, slot k & 3 (PHI != 0)
+    V oq[4];       // p_prev of this column, slot k & 3 (PHI == 2)
 #pragma unroll
     for (int k = -2; k < 2 + kCgDist; ++k) pq[k & 7] = center(x0 + k);
 #pragma unroll
     for (int k = 0; k < kCgDist; ++k) {
         hq[k & 3] = halo(x0 + k);
         rq[k & 3] = once_r(x0 + k);
-        fq[k & 3] = once_phi(x0 + k);
+        if constexpr (PHI != kPhiSkip) fq[k & 3] = once_phi(x0 + k);
+        if constexpr (PHI == kPhiPair) oq[k & 3] = once_pp(x0 + k);
     }
     double s = 0.0;
     march8(x0, x1, [&](auto kc, int x) {
@@ -368,16 +382,35 @@ __device__ inline double cg_step_tile(double* __restrict__ phi, V* __restrict__ 
         pq[(K + 2 + kCgDist) & 7] = center(x + 2 + kCgDist);
         hq[(K + kCgDist) & 3] = halo(x + kCgDist);
         rq[(K + kCgDist) & 3] = once_r(x + kCgDist);
-        fq[(K + kCgDist) & 3] = once_phi(x + kCgDist);
+        if constexpr (PHI != kPhiSkip) fq[(K + kCgDist) & 3] = once_phi(x + kCgDist);
+        if constexpr (PHI == kPhiPair) oq[(K + kCgDist) & 3] = once_pp(x + kCgDist);
         __syncthreads();
         if (!t.inyz || !g.interior(x, t.y, t.z)) return;
         const double ap = cg_apply<V>(g, x, t.y, t.z, a, c, e, lds[B], r0, q0, qx, qy, qz);
-        store_out(&at_byte(plane_at(phi, x, g), t.yzb), fq[K & 3] + alpha * c);
+        if constexpr (PHI == kPhiEach)
+            store_out(&at_byte(plane_at(phi, x, g), t.yzb), fma(alpha, c, fq[K & 3]));
+        if constexpr (PHI == kPhiPair)
+            store_out(&at_byte(plane_at(phi, x, g), t.yzb),
+                      fma(alpha, c, fma(alpha_prev, (double)oq[K & 3], fq[K & 3])));
         const V rv = (V)((double)rq[K & 3] - alpha * ap);
         store_out(&at_byte(plane_at(r, x, g), cb), rv);
         s += (double)rv * (double)rv;
     });
     return s;
+}
+
+// The pending phi term of a solve whose last iteration k was odd: phi +=
+// alpha_k p_k over a tile (p is zero on the boundary layer, so boundary
+// voxels keep phi = 0).
+template <typename V>
+__device__ inline void cg_flush_phi_tile(double* __restrict__ phi, const V* __restrict__ p, const Grid& g,
+                                         double alpha, const CgTile& t) {
+    if (!t.inyz) return;
+    for (int x = t.x0; x < t.x1; ++x) {
+        if (!g.interior(x, t.y, t.z)) continue;
+        double& f = at_byte(plane_at(phi, x, g), t.yzb);
+        f = fma(alpha, (double)at_byte(plane_at(p, x, g), vb<V>(t.yzb)), f);
+    }
 }
 
 // r0 = -div_c(mask f) over a tile (no LDS: each f component is read at +-1

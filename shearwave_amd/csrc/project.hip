@@ -13,7 +13,10 @@
 // i.e. 64 B per voxel per iteration in float64 instead of the textbook 88 B,
 // with bit-identical arithmetic (Ap is recomputed by the same operations).
 // With float32 vectors (r, p_old, p in float; phi in double) it is 40 B; the
-// products, dot products and residual norm are float64 either way.
+// products, dot products and residual norm are float64 either way. The
+// x-marching kernels defer phi to every other iteration (march_cg.h), which
+// is bit-identical and takes one iteration pair from 2 x 64 to 112 B
+// (float64 vectors) or from 2 x 40 to 68 B (float32).
 // Scalars (alpha, beta, rs, stop flag) live in device memory; the last block
 // of each kernel reduces the per-block partials in a fixed order (so results
 // are deterministic) and applies the stopping rules. The host only polls the
@@ -128,7 +131,7 @@ __global__ void __launch_bounds__(256) cg_step_kernel(double* __restrict__ phi, 
             auto pz = [&](int k) { return k == 0 ? pv : (double)p[i + k]; };
             const double ap = -(SW_AXIS_TERM(px, x, g.nx, qx) + SW_AXIS_TERM(py, y, g.ny, qy) +
                                 SW_AXIS_TERM(pz, z, g.nz, qz));
-            phi[i] = phi[i] + alpha * pv;
+            phi[i] = fma(alpha, pv, phi[i]);
             const V rv = (V)((double)r[i] - alpha * ap);
             r[i] = rv;
             s += (double)rv * (double)rv;
@@ -174,22 +177,35 @@ __global__ void __launch_bounds__(64 * kCgBY) cg_dir_march_kernel(const V* __res
     }
 }
 
-template <typename V>
+// PHI: kPhiSkip on odd iterations, kPhiPair (p_prev = p of the odd one) on even ones.
+template <typename V, int PHI>
 __global__ void __launch_bounds__(64 * kCgBY) cg_step_march_kernel(double* __restrict__ phi, V* __restrict__ r,
-                                                                   const V* __restrict__ p, Grid g, double qx,
+                                                                   const V* __restrict__ p,
+                                                                   const V* __restrict__ p_prev, Grid g, double qx,
                                                                    double qy, double qz, int xchunk, int nty, int ntz,
                                                                    double* part_sum, CgState* st) {
     if (st->stop) return;
     __shared__ CgLdsT<V> lds;
     __shared__ double red[16];
     const CgTile t = cg_block_tile(g, xchunk, nty, ntz);
-    double s = cg_step_tile<V>(phi, r, p, g, st->alpha, qx, qy, qz, t, lds);
+    double s = cg_step_tile<V, PHI>(phi, r, p, p_prev, g, st->alpha, st->alpha_prev, qx, qy, qz, t, lds);
     s = block_sum(s, red);
     if (publish_slot(part_sum, nullptr, blockIdx.x, s, 0.0, &st->counter, gridDim.x)) {
         double rs_new, unused;
         reduce_slots(part_sum, nullptr, gridDim.x, red, &rs_new, &unused);
         if (threadIdx.x == 0 && threadIdx.y == 0) cg_after_step(st, rs_new);
     }
+}
+
+// After the march CG: the deferred phi term of a solve that stopped after an
+// odd iteration (its p is in p_odd, the buffer odd iterations write).
+template <typename V>
+__global__ void __launch_bounds__(64 * kCgBY) cg_flush_march_kernel(double* __restrict__ phi,
+                                                                    const V* __restrict__ p_odd, Grid g, int xchunk,
+                                                                    int nty, int ntz, const CgState* st) {
+    if ((st->iters & 1) == 0) return;
+    const CgTile t = cg_block_tile(g, xchunk, nty, ntz);
+    cg_flush_phi_tile<V>(phi, p_odd, g, st->alpha, t);
 }
 
 // One 1024-thread block per CU: long x chunks keep the per-chunk prologue
@@ -313,14 +329,22 @@ int run_cg(const Workspace& w, const Grid& g, bool march, const CgShape& cs, uns
     V* r = (V*)w.r;
     V* p_old = (V*)w.p0;
     V* p_new = (V*)w.p1;
+    // iteration k = launched + j + 1 writes p into p1 when k is odd, p0 when even
     while (!hs.stop && launched < max_iters) {
         const int64_t k = (max_iters - launched < chunk) ? (max_iters - launched) : chunk;
         for (int64_t j = 0; j < k; ++j) {
             if (march) {
+                const bool odd = ((launched + j) & 1) == 0;
                 hipLaunchKernelGGL(cg_dir_march_kernel<V>, dim3(cs.blocks()), dim3(64, kCgBY), 0, s, r, p_old, p_new,
                                    g, qx, qy, qz, cs.xchunk, cs.nty, cs.ntz, w.part_sum, w.st);
-                hipLaunchKernelGGL(cg_step_march_kernel<V>, dim3(cs.blocks()), dim3(64, kCgBY), 0, s, w.phi, r, p_new,
-                                   g, qx, qy, qz, cs.xchunk, cs.nty, cs.ntz, w.part_sum, w.st);
+                if (odd)
+                    hipLaunchKernelGGL((cg_step_march_kernel<V, kPhiSkip>), dim3(cs.blocks()), dim3(64, kCgBY), 0, s,
+                                       w.phi, r, p_new, p_old, g, qx, qy, qz, cs.xchunk, cs.nty, cs.ntz, w.part_sum,
+                                       w.st);
+                else
+                    hipLaunchKernelGGL((cg_step_march_kernel<V, kPhiPair>), dim3(cs.blocks()), dim3(64, kCgBY), 0, s,
+                                       w.phi, r, p_new, p_old, g, qx, qy, qz, cs.xchunk, cs.nty, cs.ntz, w.part_sum,
+                                       w.st);
             } else {
                 hipLaunchKernelGGL(cg_dir_kernel<V>, dim3(nb), blk, 0, s, r, p_old, p_new, g, qx, qy, qz, w.part_sum,
                                    w.st);
@@ -336,6 +360,11 @@ int run_cg(const Workspace& w, const Grid& g, bool march, const CgShape& cs, uns
         SW_TRY_HIP(hipMemcpyAsync(&hs, w.st, sizeof(CgState), hipMemcpyDeviceToHost, s));
         SW_TRY_HIP(hipStreamSynchronize(s));
         if (chunk < 64) chunk *= 2;
+    }
+    if (march) {
+        hipLaunchKernelGGL(cg_flush_march_kernel<V>, dim3(cs.blocks()), dim3(64, kCgBY), 0, s, w.phi, (const V*)w.p1,
+                           g, cs.xchunk, cs.nty, cs.ntz, w.st);
+        SW_LAUNCH_CHECK();
     }
     return SW_OK;
 }

@@ -147,3 +147,23 @@ def test_zero_force_solve_in_a_batch(sw, specs, mixed):
     u1, v1 = b.result(1)
     np.testing.assert_array_equal(u1, mixed[1][0])
     np.testing.assert_array_equal(v1, mixed[1][1])
+
+
+def test_batched_cg_stops_after_odd_and_even_iterations(sw):
+    """Solves of one batch stop after 5 and 6 iterations: the deferred phi term of the odd one is flushed."""
+    from shearwave_amd.batch import Batch
+
+    sps = [make_spec((24, 20, 70), 300, 10), make_spec((26, 19, 66), 301, 10)]
+    for sp, k in zip(sps, (5, 6)):
+        sp.tol, sp.max_iters = 1e-30, k
+    b = Batch(sps)
+    info = b.project()
+    for k, sp in enumerate(sps):
+        assert info[k]["reason"] == "max_iters" and info[k]["iters"] == sp.max_iters, info[k]
+        z = np.zeros(sp.shape, np.float32)
+        want = sw.project_vector_field(z, z, sp.force[2].astype(np.float32), *sp.h, 1e-30, sp.max_iters,
+                                       cg_vectors="float32")
+        got = b.solves[k].f.cpu().numpy()
+        for c in range(3):
+            if np.any(want[c]):
+                assert rel(got[c], want[c]) < 1e-6, (k, c, rel(got[c], want[c]))

@@ -189,6 +189,24 @@ def test_projection_early_exit_and_max_iters(sw):
         assert rel(a, b) < 1e-12
 
 
+@pytest.mark.parametrize("k", [1, 2, 5, 6])
+def test_projection_stops_after_odd_and_even_iterations(sw, k):
+    """The x-marching CG adds phi every other iteration; a stop after an odd one flushes the pending term."""
+    rng = np.random.default_rng(20 + k)
+    shape = (26, 19, 70)
+    f = [rng.standard_normal(shape) * taper(shape) for _ in range(3)]
+    want = O.project_vector_field(*f, *H, 1e-30, k, dtype=np.float64)
+    info = {}
+    got = sw.project_vector_field(*f, *H, 1e-30, k, dtype="float64", info=info)
+    assert info["reason"] == "max_iters" and info["iters"] == k
+    for a, b in zip(got, want):
+        assert rel(a, b) < 1e-12
+    got = sw.project_vector_field(*(a.astype(np.float32) for a in f), *H, 1e-30, k, info=info, cg_vectors="float32")
+    assert info["iters"] == k
+    for a, b in zip(got, want):
+        assert rel(a, b) < 1e-5
+
+
 # ------------------------------------------------------------------ time loop
 
 
