@@ -43,6 +43,8 @@ extern "C" {
 #define SW_KERNEL_MARCH 2   /* x-marching LDS tiles with register queues, one step per pass */
 #define SW_KERNEL_MARCH2 3  /* as MARCH, but two increment-form steps per pass wherever
                                no output falls between them (AUTO picks it) */
+#define SW_KERNEL_MARCH3 4  /* as MARCH2, but three force-free float32 steps per pass
+                               wherever no output falls between them */
 
 /* CG stop reasons */
 #define SW_CG_RUNNING 0

@@ -18,7 +18,7 @@ LIB_PATH = os.environ.get("SHEARWAVE_LIB") or os.path.join(os.path.dirname(os.pa
 SW_OK = 0
 SW_F32, SW_F64 = 0, 1
 SW_FORM_INCREMENT, SW_FORM_LEAPFROG = 0, 1
-SW_KERNEL_AUTO, SW_KERNEL_DIRECT, SW_KERNEL_MARCH, SW_KERNEL_MARCH2 = 0, 1, 2, 3
+SW_KERNEL_AUTO, SW_KERNEL_DIRECT, SW_KERNEL_MARCH, SW_KERNEL_MARCH2, SW_KERNEL_MARCH3 = 0, 1, 2, 3, 4
 CG_REASONS = {0: "running", 1: "converged", 2: "denom", 3: "nonfinite", 4: "max_iters", 5: "early_exit"}
 
 _p = C.c_void_p

@@ -29,6 +29,7 @@
 #include <utility>
 
 #include "march2_step.h"
+#include "march3_step.h"
 #include "march_step.h"
 #include "sw_common.h"
 
@@ -101,6 +102,25 @@ __global__ void __launch_bounds__(64 * (BY + 3)) step2_march_kernel(StepArgs<T> 
     const int txi = (int)(rest / (unsigned)nty);
     const int x0 = txi * xchunk;
     march2_tile<T, VISC, FORCE, BY>(a, x0, min(x0 + xchunk, a.g.nx), tyi * BY, tzi * 64, lds);
+}
+
+// SW_STEP3_BY: rows of the three-step tile (block of 64 x (BY + 5) threads).
+#ifndef SW_STEP3_BY
+#define SW_STEP3_BY 7
+#endif
+
+template <typename T, bool VISC, int BY>
+__global__ void __launch_bounds__(64 * (BY + 5)) step3_march_kernel(StepArgs<T> a, int xchunk, int nty, int ntz) {
+    __shared__ Step3Lds<T, VISC, BY> lds;
+    const unsigned nb = gridDim.x;
+    unsigned b = blockIdx.x;
+    if ((nb & 7u) == 0u) b = (b & 7u) * (nb >> 3) + (b >> 3);  // XCD-aware, as step_march_kernel
+    const int tzi = (int)(b % (unsigned)ntz);
+    const unsigned rest = b / (unsigned)ntz;
+    const int tyi = (int)(rest % (unsigned)nty);
+    const int txi = (int)(rest / (unsigned)nty);
+    const int x0 = txi * xchunk;
+    march3_tile<T, VISC, BY>(a, x0, min(x0 + xchunk, a.g.nx), tyi * BY, tzi * 64, lds);
 }
 
 struct MarchShape {
@@ -210,6 +230,30 @@ int launch_step2_rows(const StepArgs<T>& a, int tile_y, int x_chunk, hipStream_t
     return SW_OK;
 }
 
+// Three force-free increment-form steps in one pass (u_n, d_n -> u_{n+3},
+// d_{n+3}); float32 only (the float64 queues would not fit the registers of
+// a (BY + 5)-wave block).
+template <typename T>
+int launch_step3(const StepArgs<T>& a, bool visc, int x_chunk, hipStream_t s) {
+    if constexpr (DType<T>::code == SW_F32) {
+        constexpr int BY = SW_STEP3_BY;
+        auto launch = [&](auto kern) {
+            static const int res = resident_blocks(kern, 64 * (BY + 5));
+            const MarchShape ms = march_shape(a.g, BY, 64, x_chunk, res);
+            hipLaunchKernelGGL(kern, dim3(ms.blocks()), dim3(64, BY + 5), 0, s, a, ms.xchunk, ms.nty, ms.ntz);
+        };
+        if (visc)
+            launch(step3_march_kernel<T, true, BY>);
+        else
+            launch(step3_march_kernel<T, false, BY>);
+        SW_LAUNCH_CHECK();
+        return SW_OK;
+    } else {
+        sw::set_error("three-step pass: float32 only");
+        return SW_ERR_UNSUPPORTED;
+    }
+}
+
 template <typename T>
 int launch_step2(const StepArgs<T>& a, bool visc, bool force, int tile_y, int x_chunk, hipStream_t s) {
     if (visc) return force ? launch_step2_rows<T, true, true>(a, tile_y, x_chunk, s)
@@ -245,6 +289,9 @@ int run_steps(sw_fdtd_state* st, int64_t n_steps, const double* env, const sw_fd
     int kernel = st->kernel;
     const bool march_fits = g.nz >= 32 && g.ny >= 8 && g.nx >= 8 && g.plane() < kMaxMarchPlane;
     if (kernel == SW_KERNEL_AUTO) kernel = march_fits ? SW_KERNEL_MARCH2 : SW_KERNEL_DIRECT;
+    // three-step passes: float32 only; other storage runs two-step passes
+    const bool triples = kernel == SW_KERNEL_MARCH3 && incr && DType<T>::code == SW_F32;
+    if (kernel == SW_KERNEL_MARCH3) kernel = SW_KERNEL_MARCH2;
     SW_REQUIRE(kernel == SW_KERNEL_DIRECT || g.plane() < kMaxMarchPlane, SW_ERR_ARG,
                "march kernels need a y-z plane of fewer than %lld voxels", (long long)kMaxMarchPlane);
     // two-step passes (increment form, no output in between); the remaining
@@ -278,7 +325,11 @@ int run_steps(sw_fdtd_state* st, int64_t n_steps, const double* env, const sw_fd
 
     for (int64_t it = 0; it < n_steps;) {
         const double fs = env ? env[it] : 0.0;
-        const bool two = pairs && it + 1 < n_steps && !output_after(step0 + it);
+        // three force-free steps with no output after the first two
+        const bool three = triples && it + 2 < n_steps && !output_after(step0 + it) &&
+                           !output_after(step0 + it + 1) &&
+                           (!env || (env[it] == 0.0 && env[it + 1] == 0.0 && env[it + 2] == 0.0));
+        const bool two = !three && pairs && it + 1 < n_steps && !output_after(step0 + it);
         const double fs2 = (two && env) ? env[it + 1] : 0.0;
         const bool force = fs != 0.0 || fs2 != 0.0;
         SW_REQUIRE(!force || have_force, SW_ERR_ARG, "env[%lld] != 0 but the state has no force",
@@ -291,8 +342,9 @@ int run_steps(sw_fdtd_state* st, int64_t n_steps, const double* env, const sw_fd
             a.Ao[c] = (T*)st->u_next[c];
             a.Bo[c] = (T*)st->b_next[c];
         }
-        int rc = two ? launch_step2<T>(a, visc, force, st->tile_y, st->x_chunk, s)
-                     : launch_step_any<T>(a, incr, visc, force, one_kernel, one_tile_y, st->tile_z, st->x_chunk, s);
+        int rc = three ? launch_step3<T>(a, visc, st->x_chunk, s)
+                 : two ? launch_step2<T>(a, visc, force, st->tile_y, st->x_chunk, s)
+                       : launch_step_any<T>(a, incr, visc, force, one_kernel, one_tile_y, st->tile_z, st->x_chunk, s);
         if (rc) return rc;
         for (int c = 0; c < 3; ++c) {
             if (incr) {
@@ -305,7 +357,7 @@ int run_steps(sw_fdtd_state* st, int64_t n_steps, const double* env, const sw_fd
                 st->u_next[c] = old_b;
             }
         }
-        it += two ? 2 : 1;
+        it += three ? 3 : two ? 2 : 1;
         if (!out) continue;
         const int64_t step = step0 + it - 1;  // the last step of this pass
         if (out->snap_stride > 0 && (step + 1) % out->snap_stride == 0) {
@@ -343,7 +395,7 @@ extern "C" int sw_fdtd_run(sw_fdtd_state* st, int64_t n_steps, const double* env
     if ((rc = check_grid(st->nx, st->ny, st->nz))) return rc;
     SW_REQUIRE(st->form == SW_FORM_INCREMENT || st->form == SW_FORM_LEAPFROG, SW_ERR_ARG, "unknown form %d",
                st->form);
-    SW_REQUIRE(st->kernel >= SW_KERNEL_AUTO && st->kernel <= SW_KERNEL_MARCH2, SW_ERR_ARG, "unknown kernel %d",
+    SW_REQUIRE(st->kernel >= SW_KERNEL_AUTO && st->kernel <= SW_KERNEL_MARCH3, SW_ERR_ARG, "unknown kernel %d",
                st->kernel);
     SW_REQUIRE(n_steps >= 0, SW_ERR_ARG, "n_steps must be >= 0");
     SW_REQUIRE(st->dt > 0 && st->dx > 0 && st->dy > 0 && st->dz > 0, SW_ERR_ARG, "dt and spacing must be > 0");

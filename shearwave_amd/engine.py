@@ -22,7 +22,7 @@ _DTYPES = {
     "float64": (torch.float64, _abi.SW_F64, np.float64),
 }
 _KERNELS = {"auto": _abi.SW_KERNEL_AUTO, "direct": _abi.SW_KERNEL_DIRECT, "march": _abi.SW_KERNEL_MARCH,
-            "march2": _abi.SW_KERNEL_MARCH2}
+            "march2": _abi.SW_KERNEL_MARCH2, "march3": _abi.SW_KERNEL_MARCH3}
 _FORMS = {"increment": _abi.SW_FORM_INCREMENT, "leapfrog": _abi.SW_FORM_LEAPFROG}
 
 
