@@ -408,7 +408,7 @@ __device__ inline void cg_flush_phi_tile(double* __restrict__ phi, const V* __re
     if (!t.inyz) return;
     for (int x = t.x0; x < t.x1; ++x) {
         if (!g.interior(x, t.y, t.z)) continue;
-        double& f = at_byte(plane_at(phi, x, g), t.yzb);
+        auto& f = at_byte(plane_at(phi, x, g), t.yzb);
         f = fma(alpha, (double)at_byte(plane_at(p, x, g), vb<V>(t.yzb)), f);
     }
 }

@@ -73,13 +73,26 @@ __device__ __forceinline__ T* uniform_ptr(T* p) {
     return reinterpret_cast<T*>(((uint64_t)hi << 32) | lo);
 }
 
+// T in the global address space. Every buffer the kernels address through
+// at_byte is device (global) memory.
+template <typename T>
+using Global = __attribute__((address_space(1))) T;
+
 // Element at a 32-bit byte offset from a base that is uniform across the wave
 // (callers guarantee it): the access takes the scalar-base +
 // 32-bit-vector-offset form, so no 64-bit address is formed or kept per lane.
+// The rebuilt base is a global pointer: the loads and stores are global_*
+// instructions, not flat ones. A flat access also counts against the LDS
+// counter, so the lgkmcnt(0) wait ahead of each barrier would drain every
+// prefetched load with it.
 template <typename T>
-__device__ __forceinline__ T& at_byte(T* base, uint32_t off) {
-    using Byte = std::conditional_t<std::is_const<T>::value, const char, char>;
-    return *reinterpret_cast<T*>(reinterpret_cast<Byte*>(uniform_ptr(base)) + off);
+__device__ __forceinline__ Global<T>& at_byte(T* base, uint32_t off) {
+    using Byte = Global<std::conditional_t<std::is_const<T>::value, const char, char>>;
+    const uint64_t v = reinterpret_cast<uint64_t>(base);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    Byte* b = reinterpret_cast<Byte*>(((uint64_t)hi << 32) | lo);
+    return *reinterpret_cast<Global<T>*>(b + off);
 }
 
 // Row-structured launch used by the pointwise and small-stencil kernels:
@@ -147,10 +160,28 @@ __device__ __forceinline__ T load_once(const T* p) {
 #endif
 }
 
+template <typename T>
+__device__ __forceinline__ T load_once(const Global<T>* p) {
+#ifndef SW_NO_NT
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
+
 // Stores of data that is next read one launch later, after the L2 has turned
 // over, carry the non-temporal hint too (SW_NO_NT: plain stores).
 template <typename T>
 __device__ __forceinline__ void store_out(T* p, T v) {
+#ifndef SW_NO_NT
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
+template <typename T>
+__device__ __forceinline__ void store_out(Global<T>* p, T v) {
 #ifndef SW_NO_NT
     __builtin_nontemporal_store(v, p);
 #else
