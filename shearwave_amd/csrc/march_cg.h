@@ -311,6 +311,10 @@ __device__ inline double cg_dir_tile(const V* __restrict__ r, const V* __restric
     for (int k = 2; k < 2 + D; ++k) rq[k & 7] = center(x0 + k);
 #pragma unroll
     for (int k = 0; k < D; ++k) hq[k & 7] = halo(x0 + k);
+    // plane x0 is staged ahead of the loop; each iteration stages the next
+    // plane right after its barrier, so the LDS stores overlap this plane's work
+    lds[0][r0][q0] = (V)pq[0];
+    if (t.halo_thread) lds[0][t.ly][t.lz] = (V)pv(hq[0]);
     double s = 0.0;
     march8(x0, x1, [&](auto kc, int x) {
         constexpr int K = decltype(kc)::value;
@@ -318,11 +322,12 @@ __device__ inline double cg_dir_tile(const V* __restrict__ r, const V* __restric
         const double e = pv(rq[(K + 2) & 7]);  // plane x+2, loaded D iterations ago
         pq[(K + 2) & 7] = e;
         const double a = pq[(K + 6) & 7], c = pq[K & 7];
-        lds[B][r0][q0] = (V)c;
-        if (t.halo_thread) lds[B][t.ly][t.lz] = (V)pv(hq[K & 7]);
         rq[(K + 2 + D) & 7] = center(x + 2 + D);
         hq[(K + D) & 7] = halo(x + D);
         __syncthreads();
+        // plane x+1 into the other buffer: its last reads (plane x-1) ended before the barrier
+        lds[B ^ 1][r0][q0] = (V)pq[(K + 1) & 7];
+        if (t.halo_thread) lds[B ^ 1][t.ly][t.lz] = (V)pv(hq[(K + 1) & 7]);
         if (!t.inyz) return;
         const bool in = g.interior(x, t.y, t.z);
         if (in) s += c * cg_apply<V>(g, x, t.y, t.z, a, c, e, lds[B], r0, q0, qx, qy, qz);
@@ -371,20 +376,25 @@ __device__ inline double cg_step_tile(double* __restrict__ phi, V* __restrict__ 
         if constexpr (PHI != kPhiSkip) fq[k & 3] = once_phi(x0 + k);
         if constexpr (PHI == kPhiPair) oq[k & 3] = once_pp(x0 + k);
     }
+    // plane x0 is staged ahead of the loop; each iteration stages the next
+    // plane right after its barrier, so the LDS stores overlap this plane's work
+    lds[0][r0][q0] = pq[0];
+    if (t.halo_thread) lds[0][t.ly][t.lz] = hq[0];
     double s = 0.0;
     march8(x0, x1, [&](auto kc, int x) {
         constexpr int K = decltype(kc)::value;
         constexpr int B = K & 1;
         const V cv = pq[K & 7];
         const double a = (double)pq[(K + 6) & 7], c = (double)cv, e = (double)pq[(K + 2) & 7];
-        lds[B][r0][q0] = cv;
-        if (t.halo_thread) lds[B][t.ly][t.lz] = hq[K & 3];
         pq[(K + 2 + kCgDist) & 7] = center(x + 2 + kCgDist);
         hq[(K + kCgDist) & 3] = halo(x + kCgDist);
         rq[(K + kCgDist) & 3] = once_r(x + kCgDist);
         if constexpr (PHI != kPhiSkip) fq[(K + kCgDist) & 3] = once_phi(x + kCgDist);
         if constexpr (PHI == kPhiPair) oq[(K + kCgDist) & 3] = once_pp(x + kCgDist);
         __syncthreads();
+        // plane x+1 into the other buffer: its last reads (plane x-1) ended before the barrier
+        lds[B ^ 1][r0][q0] = pq[(K + 1) & 7];
+        if (t.halo_thread) lds[B ^ 1][t.ly][t.lz] = hq[(K + 1) & 3];
         if (!t.inyz || !g.interior(x, t.y, t.z)) return;
         const double ap = cg_apply<V>(g, x, t.y, t.z, a, c, e, lds[B], r0, q0, qx, qy, qz);
         if constexpr (PHI == kPhiEach)
