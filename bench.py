@@ -710,6 +710,8 @@ def main():
     t = time.perf_counter() - t0
     per_solve = t / args.steps
     value = vox * w["n_steps"] / per_solve / 1e9
+    # the final state of the timed solve, kept for the float32-against-float64 check below
+    final32 = (torch.stack(bench.solve.u_components()).double(), torch.stack(bench.solve.b_components()).double())
 
     # projection alone and the loop alone (one extra solve, phase split)
     s = bench.solve
@@ -781,11 +783,22 @@ def main():
         b64.run()
         torch.cuda.synchronize()
         t64 = time.perf_counter() - t0
+        # the whole 2000-step solve at full size: float32 storage (float32 CG vectors) against the
+        # float64 form, which the parity tests hold within 1e-10 of the float64 oracle
+        final64 = (torch.stack(b64.solve.u_components()), torch.stack(b64.solve.b_components()))
+        if args.dtype == "float32":
+            def rl2(a, b):
+                return float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+            rec["fp32_vs_fp64"] = {"rel_l2_u": rl2(final32[0], final64[0]),
+                                   "rel_l2_v": rl2(final32[1], final64[1]),
+                                   "note": "final u and d = v dT of the timed solves; gate 1e-5"}
+        del final64
         kt64 = time_step_kernel(b64.solve, args.kernel_steps // 2, 1)
         ab64 = W.alg_step_bytes(8, viscous=True, forced=False) * vox
         rec["fp64"] = {"value": round(vox * w["n_steps"] / t64 / 1e9, 3), "ms_per_solve": round(t64 * 1e3, 2),
                        "step_kernel_GBs": round(ab64 / kt64 / 1e9, 1), "frac": round(ab64 / kt64 / 1e9 / HBM_PEAK_GBS, 4)}
         del b64
+    del final32
     if not args.no_batch:
         del bench
         torch.cuda.empty_cache()

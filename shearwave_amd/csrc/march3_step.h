@@ -22,8 +22,10 @@
 // step n+1 at plane x (tile + 2, staged in a second LDS plane), step n+2 at
 // plane x-1 (tile + 1, staged in a third) and step n+3 at plane x-2. All
 // three LDS planes are double-buffered, so one barrier per plane suffices.
-// Every step calls the single-step arithmetic (point_inc), so a pass equals
-// three single steps bit for bit.
+// Every step calls the single-step arithmetic (point_inc), so the increments
+// equal those of three single steps bit for bit; the stored displacement
+// combines the last two increments as the two-step pass does,
+// u_{n+3} = u_{n+1} + (d_{n+2} + d_{n+3}).
 #pragma once
 
 #include <type_traits>
@@ -174,9 +176,10 @@ __device__ __forceinline__ void march3_tile(const StepArgs<T>& a, int x0, int x1
             const bool in = p_int && x - 2 > 0 && x - 2 < g.nx - 1;
             if (!in) i0 = i1 = i2 = T(0);
             const int64_t px = (int64_t)(x - 2) * P;
-            store_out(&at_byte(a.Ao[0] + px, pyz), in ? c.u0 + i0 : T(0));
-            store_out(&at_byte(a.Ao[1] + px, pyz), in ? c.u1 + i1 : T(0));
-            store_out(&at_byte(a.Ao[2] + px, pyz), in ? c.u2 + i2 : T(0));
+            const Col<T>& u1 = s1[(K + 2) & 3];  // step n+1 of plane x-2
+            store_out(&at_byte(a.Ao[0] + px, pyz), in ? u1.u0 + (c.d0 + i0) : T(0));
+            store_out(&at_byte(a.Ao[1] + px, pyz), in ? u1.u1 + (c.d1 + i1) : T(0));
+            store_out(&at_byte(a.Ao[2] + px, pyz), in ? u1.u2 + (c.d2 + i2) : T(0));
             store_out(&at_byte(a.Bo[0] + px, pyz), i0);
             store_out(&at_byte(a.Bo[1] + px, pyz), i1);
             store_out(&at_byte(a.Bo[2] + px, pyz), i2);
