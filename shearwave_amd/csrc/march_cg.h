@@ -7,6 +7,7 @@
 
 #include <cmath>
 #include <type_traits>
+#include <utility>
 
 #include "sw_common.h"
 
@@ -252,28 +253,29 @@ static_assert(kCgDist >= 1 && kCgDist <= 3, "CG step prefetch distance");
 template <int K>
 using Ic = std::integral_constant<int, K>;
 
-// body(Ic<k>{}, x) for x = x0 .. x1-1 with k = (x - x0) mod 8: ring slots
-// indexed by k are compile-time, so the rings live in registers and their
-// roles rotate without moves.
+// The x loop of the CG tiles is unrolled by kCgU planes so that every ring
+// slot is a compile-time index (the rings live in registers and rotate
+// without moves). Loads issued near the end of one unrolled trip are
+// consumed early in the next, and the compiler waits for every load in
+// flight at the loop header (vmcnt(0)): a longer trip drains the queues less
+// often.
+#ifndef SW_CG_UNROLL
+#define SW_CG_UNROLL 16
+#endif
+constexpr int kCgU = SW_CG_UNROLL, kCgM = kCgU - 1;
+static_assert(kCgU == 8 || kCgU == 16, "CG unroll");
+
+template <typename F, int... Ks>
+__device__ __forceinline__ bool march_trip(int x, int x1, F& body, std::integer_sequence<int, Ks...>) {
+    // left to right, stopping at the first plane past the chunk
+    return ((x + Ks < x1 && (body(Ic<Ks>{}, x + Ks), true)) && ...);
+}
+
+// body(Ic<k>{}, x) for x = x0 .. x1-1 with k = (x - x0) mod kCgU.
 template <typename F>
-__device__ __forceinline__ void march8(int x0, int x1, F&& body) {
-    for (int x = x0; x < x1; x += 8) {
-        body(Ic<0>{}, x);
-        if (x + 1 >= x1) break;
-        body(Ic<1>{}, x + 1);
-        if (x + 2 >= x1) break;
-        body(Ic<2>{}, x + 2);
-        if (x + 3 >= x1) break;
-        body(Ic<3>{}, x + 3);
-        if (x + 4 >= x1) break;
-        body(Ic<4>{}, x + 4);
-        if (x + 5 >= x1) break;
-        body(Ic<5>{}, x + 5);
-        if (x + 6 >= x1) break;
-        body(Ic<6>{}, x + 6);
-        if (x + 7 >= x1) break;
-        body(Ic<7>{}, x + 7);
-    }
+__device__ __forceinline__ void march_cg(int x0, int x1, F&& body) {
+    for (int x = x0; x < x1; x += kCgU)
+        if (!march_trip(x, x1, body, std::make_integer_sequence<int, kCgU>{})) break;
 }
 
 // r and p_old of one point, as loaded.
@@ -302,32 +304,32 @@ __device__ inline double cg_dir_tile(const V* __restrict__ r, const V* __restric
     auto pv = [&](const CgRaw<V>& v) { return as_stored<V>(fma(beta, (double)v.q, (double)v.r)); };
     const int x0 = t.x0, x1 = t.x1;
     constexpr int D = kCgDistDir;
-    double pq[8];    // p of this column at plane x0 + k in slot k & 7 (planes x-2 .. x+2 live)
-    CgRaw<V> rq[8];  // raw loads of this column in flight, slot k & 7 (planes x+3 .. x+2+D)
-    CgRaw<V> hq[8];  // raw loads of this thread's halo point, slot k & 7 (planes x .. x+D)
+    double pq[kCgU];    // p of this column at plane x0 + k in slot k & kCgM (planes x-2 .. x+2 live)
+    CgRaw<V> rq[kCgU];  // raw loads of this column in flight, slot k & kCgM (planes x+3 .. x+2+D)
+    CgRaw<V> hq[kCgU];  // raw loads of this thread's halo point, slot k & kCgM (planes x .. x+D)
 #pragma unroll
-    for (int k = -2; k < 2; ++k) pq[k & 7] = pv(center(x0 + k));
+    for (int k = -2; k < 2; ++k) pq[k & kCgM] = pv(center(x0 + k));
 #pragma unroll
-    for (int k = 2; k < 2 + D; ++k) rq[k & 7] = center(x0 + k);
+    for (int k = 2; k < 2 + D; ++k) rq[k & kCgM] = center(x0 + k);
 #pragma unroll
-    for (int k = 0; k < D; ++k) hq[k & 7] = halo(x0 + k);
+    for (int k = 0; k < D; ++k) hq[k & kCgM] = halo(x0 + k);
     // plane x0 is staged ahead of the loop; each iteration stages the next
     // plane right after its barrier, so the LDS stores overlap this plane's work
     lds[0][r0][q0] = (V)pq[0];
     if (t.halo_thread) lds[0][t.ly][t.lz] = (V)pv(hq[0]);
     double s = 0.0;
-    march8(x0, x1, [&](auto kc, int x) {
+    march_cg(x0, x1, [&](auto kc, int x) {
         constexpr int K = decltype(kc)::value;
         constexpr int B = K & 1;
-        const double e = pv(rq[(K + 2) & 7]);  // plane x+2, loaded D iterations ago
-        pq[(K + 2) & 7] = e;
-        const double a = pq[(K + 6) & 7], c = pq[K & 7];
-        rq[(K + 2 + D) & 7] = center(x + 2 + D);
-        hq[(K + D) & 7] = halo(x + D);
+        const double e = pv(rq[(K + 2) & kCgM]);  // plane x+2, loaded D iterations ago
+        pq[(K + 2) & kCgM] = e;
+        const double a = pq[(K + kCgU - 2) & kCgM], c = pq[K & kCgM];
+        rq[(K + 2 + D) & kCgM] = center(x + 2 + D);
+        hq[(K + D) & kCgM] = halo(x + D);
         __syncthreads();
         // plane x+1 into the other buffer: its last reads (plane x-1) ended before the barrier
-        lds[B ^ 1][r0][q0] = (V)pq[(K + 1) & 7];
-        if (t.halo_thread) lds[B ^ 1][t.ly][t.lz] = (V)pv(hq[(K + 1) & 7]);
+        lds[B ^ 1][r0][q0] = (V)pq[(K + 1) & kCgM];
+        if (t.halo_thread) lds[B ^ 1][t.ly][t.lz] = (V)pv(hq[(K + 1) & kCgM]);
         if (!t.inyz) return;
         const bool in = g.interior(x, t.y, t.z);
         if (in) s += c * cg_apply<V>(g, x, t.y, t.z, a, c, e, lds[B], r0, q0, qx, qy, qz);
@@ -362,13 +364,13 @@ __device__ inline double cg_step_tile(double* __restrict__ phi, V* __restrict__ 
     auto once_phi = [&](int xx) { return load_once(&at_byte(plane_at(phi, xx, g), t.yzb)); };
     auto once_pp = [&](int xx) { return load_once(&at_byte(plane_at(p_prev, xx, g), cb)); };
     const int x0 = t.x0, x1 = t.x1;
-    V pq[8];       // p of this column, slot k & 7 (planes x-2 .. x+2+D live)
+    V pq[kCgU];    // p of this column, slot k & kCgM (planes x-2 .. x+2+D live)
     V hq[4];       // p at this thread's halo point, slot k & 3 (planes x .. x+D)
     V rq[4];       // r of this column, slot k & 3
     double fq[4];  // phi of this column, slot k & 3 (PHI != 0)
     V oq[4];       // p_prev of this column, slot k & 3 (PHI == 2)
 #pragma unroll
-    for (int k = -2; k < 2 + kCgDist; ++k) pq[k & 7] = center(x0 + k);
+    for (int k = -2; k < 2 + kCgDist; ++k) pq[k & kCgM] = center(x0 + k);
 #pragma unroll
     for (int k = 0; k < kCgDist; ++k) {
         hq[k & 3] = halo(x0 + k);
@@ -381,19 +383,19 @@ __device__ inline double cg_step_tile(double* __restrict__ phi, V* __restrict__ 
     lds[0][r0][q0] = pq[0];
     if (t.halo_thread) lds[0][t.ly][t.lz] = hq[0];
     double s = 0.0;
-    march8(x0, x1, [&](auto kc, int x) {
+    march_cg(x0, x1, [&](auto kc, int x) {
         constexpr int K = decltype(kc)::value;
         constexpr int B = K & 1;
-        const V cv = pq[K & 7];
-        const double a = (double)pq[(K + 6) & 7], c = (double)cv, e = (double)pq[(K + 2) & 7];
-        pq[(K + 2 + kCgDist) & 7] = center(x + 2 + kCgDist);
+        const V cv = pq[K & kCgM];
+        const double a = (double)pq[(K + kCgU - 2) & kCgM], c = (double)cv, e = (double)pq[(K + 2) & kCgM];
+        pq[(K + 2 + kCgDist) & kCgM] = center(x + 2 + kCgDist);
         hq[(K + kCgDist) & 3] = halo(x + kCgDist);
         rq[(K + kCgDist) & 3] = once_r(x + kCgDist);
         if constexpr (PHI != kPhiSkip) fq[(K + kCgDist) & 3] = once_phi(x + kCgDist);
         if constexpr (PHI == kPhiPair) oq[(K + kCgDist) & 3] = once_pp(x + kCgDist);
         __syncthreads();
         // plane x+1 into the other buffer: its last reads (plane x-1) ended before the barrier
-        lds[B ^ 1][r0][q0] = pq[(K + 1) & 7];
+        lds[B ^ 1][r0][q0] = pq[(K + 1) & kCgM];
         if (t.halo_thread) lds[B ^ 1][t.ly][t.lz] = hq[(K + 1) & 3];
         if (!t.inyz || !g.interior(x, t.y, t.z)) return;
         const double ap = cg_apply<V>(g, x, t.y, t.z, a, c, e, lds[B], r0, q0, qx, qy, qz);
