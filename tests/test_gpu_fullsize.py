@@ -1,0 +1,64 @@
+"""The 256^3 drop-in workload at full size (2000 steps), through properties that do not need the oracle.
+
+The float64 form is held within 1e-10 of the float64 oracle by the parity
+tests (on grids the oracle finishes in seconds); here it is the reference
+for the float32 form on the full workload: 256^3, h = 0.25 mm, 8 seeded
+inclusions, per-voxel rho and eta, Gaussian push, CG tol 1e-8. Gates: 1e-5
+relative L2 on u and v, and a projected force whose divergence is below
+1e-6 of the input's (relative L2).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+
+
+@pytest.fixture(scope="module")
+def workload():
+    from shearwave_amd import workloads as W
+
+    return W.dropin_256()
+
+
+def _run(w, dtype):
+    from shearwave_amd import engine as E
+
+    s = E.Solve(w["shape"], (w["h"],) * 3, w["dt"], w["rho"], w["mu"], w["eta"], dtype=dtype, form="increment")
+    s.set_initial(None, None)
+    info, _ = s.set_force(None, None, E.to_device(w["bz"], dtype, w["shape"]), w["tol"], w["max_iters"])
+    assert info["reason"] == "converged", info
+    s.run(w["n_steps"], w["env"])
+    torch.cuda.synchronize()
+    out = (torch.stack(s.u_components()).double(), torch.stack(s.b_components()).double())
+    del s
+    torch.cuda.empty_cache()
+    return out
+
+
+def test_dropin_256_float32_within_1e5_of_float64(sw, workload):
+    u64, d64 = _run(workload, "float64")
+    u32, d32 = _run(workload, "float32")
+    assert float(u64.abs().max()) > 0
+    eu, ev = rel(u32, u64), rel(d32, d64)  # d = v dT in both forms
+    assert eu < 1e-5 and ev < 1e-5, (eu, ev)
+
+
+def test_projection_256_divergence_free(sw, workload):
+    w = workload
+    shape, h = w["shape"], w["h"]
+    f_in = torch.zeros((3,) + shape, dtype=torch.float64, device="cuda")
+    f_in[2] = torch.from_numpy(w["bz"].astype(np.float32).astype(np.float64)).cuda()
+    info = {}
+    out = sw.project_vector_field(*(f_in[c].float() for c in range(3)), h, h, h, w["tol"], w["max_iters"],
+                                  info=info)
+    assert info["reason"] == "converged" and info["iters"] > 100, info
+    d_in = sw.divergence_center(*(f_in[c] for c in range(3)), h, h, h)
+    d_out = sw.divergence_center(*(o.double() for o in out), h, h, h)
+    ratio = float(torch.linalg.norm(d_out) / torch.linalg.norm(d_in))
+    assert ratio < 1e-6, ratio
