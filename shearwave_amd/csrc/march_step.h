@@ -78,7 +78,8 @@ __device__ __forceinline__ Coef<T> load_coef(const StepArgs<T>& a, int64_t po, u
 // form): (d + m lap(u) + e lap(d) + f) (1 - vel_damp), with lap the 7-point
 // Laplacian scaled per axis. Contraction is off and every fused multiply-add
 // is written out, so the result does not depend on the kernel it is inlined
-// into: one pass of the two-step kernel equals two single steps bit for bit.
+// into: the increments of a multi-step pass equal those of single steps bit
+// for bit.
 template <typename T, bool VISC, bool FORCE>
 __device__ __forceinline__ T comp_inc(const StepArgs<T>& a, T u0, T um, T up, T uyp, T uym, T uzp, T uzm, T d0, T dm,
                                       T dp, T dyp, T dym, T dzp, T dzm, T m, T e, T f) {
