@@ -753,6 +753,7 @@ def main():
             "dt": w["dt"],
             "dt_cfl": w["dt_cfl"],
             "seed": 2,
+            "cg_vectors": args.dtype,
         },
         "roofline": {
             "bound": "hbm",

@@ -10,11 +10,12 @@ Build-only options (namespaced ``sw_``):
   * ``sw_dtype``: storage type, ``"float32"`` (default, as the reference) or ``"float64"``;
   * ``sw_form``: ``"increment"`` (default) or ``"leapfrog"`` state storage;
   * ``sw_kernel``: ``"auto"``, ``"march2"`` (two steps per pass wherever no
-    output falls between them), ``"march3"`` (three force-free float32 steps
-    per pass where no output falls between them, two-step passes elsewhere;
-    bit-identical, but slower than ``"march2"`` on gfx950), ``"march"`` (one
-    step per pass) or ``"direct"`` time-step kernel; ``"auto"`` is
-    ``"march2"`` on grids the march tiles fit;
+    output falls between them; the increments equal single steps' bit for
+    bit, and u is stored with one rounding per pass, u_n + (d_{n+1} +
+    d_{n+2})), ``"march3"`` (three force-free float32 steps per pass where no
+    output falls between them, two-step passes elsewhere; slower than
+    ``"march2"`` on gfx950), ``"march"`` (one step per pass) or ``"direct"``
+    time-step kernel; ``"auto"`` is ``"march2"`` on grids the march tiles fit;
   * ``sw_tile_y``, ``sw_tile_z``, ``sw_x_chunk``: march-kernel tile shape (0 = automatic);
   * ``sw_cg_vectors``: storage of the projection's CG vectors r and p,
     ``"float32"`` or ``"float64"`` (default: ``sw_dtype``); phi, the dot
