@@ -45,6 +45,14 @@ def test_python_binding_covers_header():
     rc = lib.sw_fdtd_run(None, 1, None, None, None)
     assert rc == 1 and b"null state" in lib.sw_last_error()
     assert lib.sw_project_workspace_bytes(8, 8, 8) > 4 * 8 * 512
+    # the CG vector storage is SW_F32 or SW_F64, like the field storage
+    info = _abi.CgInfo()
+    rc = lib.sw_project_vector_field(_abi.SW_F32, 5, None, None, None, 8, 8, 8, 1.0, 1.0, 1.0, 1e-8, 10, None, 0,
+                                     None, None, 1.0, ctypes.byref(info), None)
+    assert rc == 1 and b"dtype" in lib.sw_last_error()
+    rc = lib.sw_project_vector_field(_abi.SW_F32, _abi.SW_F32, None, None, None, 8, 8, 8, 1.0, 1.0, 1.0, 1e-8, 10,
+                                     None, 0, None, None, 1.0, ctypes.byref(info), None)
+    assert rc == 1 and b"null force" in lib.sw_last_error()
 
 
 def test_product_does_not_import_the_oracle():
