@@ -19,9 +19,10 @@ H = (0.5e-3, 0.5e-3, 0.5e-3)
 N_STEPS = 60
 
 
-def setup(dtype="float64", checkpoint_every=None, seed=0):
+def setup(dtype="float64", checkpoint_every=None, seed=0, shape=None):
     from shearwave_amd.adjoint import Problem, plane_sensors
 
+    SHAPE = shape or globals()["SHAPE"]
     rng = np.random.default_rng(seed)
     rho = 1000.0 * (1 + rng.uniform(-0.01, 0.01, SHAPE))
     mu = 3000.0 * (1 + 0.5 * rng.uniform(0, 1, SHAPE))
@@ -111,6 +112,29 @@ def test_fp32_gradient_close_to_fp64(sw, p64):
     prob, mu, eta, target = p64
     _, gmu, geta = loss_and_grad(prob, mu, eta, target)
     prob32, _, _, _ = setup("float32")
+    _, gmu32, geta32 = loss_and_grad(prob32, mu, eta, target)
+    for a, b in ((gmu32, gmu), (geta32, geta)):
+        assert float(torch.linalg.norm(a - b) / torch.linalg.norm(b)) < 1e-3
+
+
+MARCH_SHAPE = (22, 17, 70)  # partial y and z tiles of the x-marching adjoint (float32)
+
+
+def test_march_adjoint_matches_per_voxel_kernel(sw, monkeypatch):
+    """The x-marching float32 adjoint against the per-voxel kernel: same products, float32 rounding apart."""
+    prob, mu, eta, target = setup("float32", shape=MARCH_SHAPE, seed=3)
+    _, gmu, geta = loss_and_grad(prob, mu, eta, target)
+    monkeypatch.setenv("SW_ADJ_KERNEL", "direct")
+    _, gmu_d, geta_d = loss_and_grad(prob, mu, eta, target)
+    assert float(gmu_d.abs().max()) > 0 and float(geta_d.abs().max()) > 0
+    for a, b in ((gmu, gmu_d), (geta, geta_d)):
+        assert float(torch.linalg.norm(a - b) / torch.linalg.norm(b)) < 1e-5
+
+
+def test_march_adjoint_close_to_fp64(sw):
+    prob, mu, eta, target = setup("float64", shape=MARCH_SHAPE, seed=3)
+    _, gmu, geta = loss_and_grad(prob, mu, eta, target)
+    prob32, _, _, _ = setup("float32", shape=MARCH_SHAPE, seed=3)
     _, gmu32, geta32 = loss_and_grad(prob32, mu, eta, target)
     for a, b in ((gmu32, gmu), (geta32, geta)):
         assert float(torch.linalg.norm(a - b) / torch.linalg.norm(b)) < 1e-3
