@@ -111,19 +111,20 @@ class Problem:
 
         cur = 0
         inject(ub[cur], N)
+        # the forward states of one segment, each step written into its own slot
+        seg_u = torch.empty((self.K,) + vec, dtype=tdt, device=dev)
+        seg_d = torch.empty((self.K,) + vec, dtype=tdt, device=dev)
         for seg in range(len(ckpts) - 1, -1, -1):
             n0 = seg * self.K
             n1 = min(n0 + self.K, N)
             # recompute the forward states u_n, d_n for n = n0 .. n1-1
-            self._restore(*ckpts[seg])
-            states = []
-            s.step_count = n0
-            for n in range(n0, n1):
-                states.append(self._state())
-                if n + 1 < n1:
-                    s.run(1, self.env[n:n + 1])
+            seg_u[0].copy_(ckpts[seg][0])
+            seg_d[0].copy_(ckpts[seg][1])
+            for n in range(n0, n1 - 1):
+                k = n - n0
+                s.step_into(seg_u[k], seg_d[k], seg_u[k + 1], seg_d[k + 1], self.env[n])
             for n in range(n1 - 1, n0 - 1, -1):
-                u_n, d_n = states[n - n0]
+                u_n, d_n = seg_u[n - n0], seg_d[n - n0]
                 nxt = 1 - cur
                 check(lib().sw_adjoint_step(
                     self.code, ptrs(*u_n), ptrs(*d_n), ptrs(*ub[cur]), ptrs(*db[cur]), ptrs(*ub[nxt]),

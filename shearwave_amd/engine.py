@@ -248,6 +248,32 @@ class Solve:
         check(lib().sw_fdtd_run(C.byref(self.state), n_steps, env_arr, C.byref(out), stream_ptr()))
         self.step_count += n_steps
 
+    def step_into(self, u_src, d_src, u_dst, d_dst, env_value=None):
+        """One step from caller buffers (u_src, d_src) into (u_dst, d_dst), each
+        (3, nx, ny, nz) in the storage type, increment form, without touching
+        the solve's own state (the adjoint's recomputation writes each state of
+        a segment straight into its slot)."""
+        assert self.form == "increment"
+        st = self.state
+        keep = [list(getattr(st, f)) for f in ("u", "b", "u_next", "b_next")]
+        try:
+            for c in range(3):
+                st.u[c] = u_src[c].data_ptr()
+                st.b[c] = d_src[c].data_ptr()
+                st.u_next[c] = u_dst[c].data_ptr()
+                st.b_next[c] = d_dst[c].data_ptr()
+            env_arr = None
+            if env_value is not None:
+                env_np = np.asarray([float(env_value)], dtype=np.float64)
+                env_arr = env_np.ctypes.data_as(C.POINTER(C.c_double))
+            out = _abi.FdtdOutputs()
+            check(lib().sw_fdtd_run(C.byref(st), 1, env_arr, C.byref(out), stream_ptr()))
+        finally:
+            for f, vals in zip(("u", "b", "u_next", "b_next"), keep):
+                arr = getattr(st, f)
+                for c in range(3):
+                    arr[c] = vals[c]
+
     # --------------------------------------------------------------- access
     def _vec(self, ptr_array):
         return [self._by_ptr[ptr_array[c]] for c in range(3)]
