@@ -61,9 +61,9 @@ def parse():
     ap.add_argument("--no-radforce", action="store_true", help="skip the radiation-force run")
     ap.add_argument("--no-static", action="store_true", help="skip the static elasticity solve run")
     ap.add_argument("--config1", action="store_true", help="only run the 64^3 CPU-vs-GPU comparison and exit")
-    ap.add_argument("--only", choices=["radforce", "static"],
-                    help="only run that section (radiation force + acoustic strain, or the static solve), print "
-                         "its record and exit")
+    ap.add_argument("--only", choices=["radforce", "static", "batch", "adjoint"],
+                    help="only run that section (radiation force + acoustic strain, the static solve, the batching "
+                         "comparison or the differentiation run), print its record and exit")
     return ap.parse_args()
 
 
@@ -694,7 +694,9 @@ def main():
         print(json.dumps(config1_cpu_vs_gpu()), flush=True)
         return
     if args.only:
-        print(json.dumps({"radforce": bench_radforce, "static": bench_static}[args.only]()), flush=True)
+        sections = {"radforce": bench_radforce, "static": bench_static, "adjoint": bench_adjoint,
+                    "batch": lambda: bench_batch(args.batch_solves)}
+        print(json.dumps(sections[args.only]()), flush=True)
         return
     w = W.dropin_256(n=args.n, n_steps=args.nt)
     item = 4 if args.dtype == "float32" else 8
