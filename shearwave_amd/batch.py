@@ -29,6 +29,7 @@ from ._abi import check, lib
 STEP_BY = 8  # output rows of a batched step tile (blocks of 64 x 11 threads)
 MAX_PLANE = 1 << 28  # the kernels address a y-z plane with 32-bit offsets
 CG_BY = 16  # tile rows of the batched CG (fixed by the kernels)
+CG_XCHUNK = 192  # x planes per batched CG work item (longer items: fewer prologues and slot reductions)
 
 
 @dataclass
@@ -203,7 +204,7 @@ class Batch:
         for k, s in enumerate(solves):
             sp = s.spec
             nx, ny, nz = s.shape
-            xc = min(nx, 48)
+            xc = min(nx, CG_XCHUNK)
             n_items0 = len(items)
             for tx in range(math.ceil(nx / xc)):
                 for ty in range(math.ceil(ny / CG_BY)):
