@@ -17,7 +17,7 @@ import math
 import os
 import threading
 
-CANDIDATES = (16, 32, 48, 64, 96, 128, 192, 256)
+CANDIDATES = (16, 32, 48, 64, 96, 128)
 
 
 def _pow2(v):
