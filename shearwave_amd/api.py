@@ -38,7 +38,7 @@ def _is_dev(x):
 
 
 def _np_out(t, dtype=None):
-    a = t.detach().cpu().numpy()
+    a = E.to_host(t)
     return a if dtype is None else a.astype(dtype, copy=False)
 
 

@@ -301,5 +301,5 @@ class Batch:
                                     _abi.ptrs(d[0], d[1], d[2]), 1.0 / float(s.spec.dt), ua.data_ptr(), va.data_ptr(),
                                     n, E.stream_ptr()))
         if to_host:
-            return ua.cpu().numpy(), va.cpu().numpy()
+            return E.to_host(ua), E.to_host(va)
         return ua, va

@@ -57,6 +57,22 @@ def stream_ptr():
     return torch.cuda.current_stream().cuda_stream
 
 
+def to_host(t: torch.Tensor) -> np.ndarray:
+    """A device tensor as a NumPy array, through pinned host memory.
+
+    Device-to-host copies into pageable memory run at about 4 GB/s on this
+    platform, into pinned memory at about 57 GB/s. The array is a view of the
+    pinned tensor, which stays alive with it; torch's host allocator caches
+    the pinned blocks for reuse.
+    """
+    t = t.detach()
+    if t.device.type != "cuda":
+        return t.numpy()
+    h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+    h.copy_(t)
+    return h.numpy()
+
+
 def to_device(x, dtype, shape=None) -> torch.Tensor:
     """Contiguous device tensor of `dtype` from a NumPy array, scalar or tensor."""
     dev = device()
