@@ -61,7 +61,7 @@ def parse():
     ap.add_argument("--no-radforce", action="store_true", help="skip the radiation-force run")
     ap.add_argument("--no-static", action="store_true", help="skip the static elasticity solve run")
     ap.add_argument("--config1", action="store_true", help="only run the 64^3 CPU-vs-GPU comparison and exit")
-    ap.add_argument("--only", choices=["radforce", "static", "batch", "adjoint"],
+    ap.add_argument("--only", choices=["radforce", "static", "batch", "serve", "adjoint"],
                     help="only run that section (radiation force + acoustic strain, the static solve, the batching "
                          "comparison or the differentiation run), print its record and exit")
     return ap.parse_args()
@@ -695,7 +695,9 @@ def main():
         return
     if args.only:
         sections = {"radforce": bench_radforce, "static": bench_static, "adjoint": bench_adjoint,
-                    "batch": lambda: bench_batch(args.batch_solves)}
+                    "batch": lambda: bench_batch(args.batch_solves),
+                    "serve": lambda: bench_serve(tuple(float(r) for r in args.serve_rates.split(",")),
+                                                 args.serve_requests)}
         print(json.dumps(sections[args.only]()), flush=True)
         return
     w = W.dropin_256(n=args.n, n_steps=args.nt)

@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define SW_ABI_VERSION 2
+#define SW_ABI_VERSION 3
 
 /* status codes */
 #define SW_OK 0
@@ -234,7 +234,7 @@ int sw_graph_destroy(sw_graph* graph);
  * below 2^28 as for sw_batch_solve. */
 typedef struct sw_batch_cg {
     int32_t nx, ny, nz;
-    int32_t xchunk;                  /* x planes per CG work item */
+    int32_t xchunk;                  /* x planes per CG work item: a multiple of 48, or >= nx */
     double dx, dy, dz, tol;
     int64_t max_iters;
     float* f[3];                     /* force in, f_S out (in place) */
@@ -243,15 +243,20 @@ typedef struct sw_batch_cg {
     double scale;
     double* phi;                     /* float64 potential, nx*ny*nz */
     float *r, *p0, *p1;              /* float32 CG vectors, nx*ny*nz each (dot products stay float64) */
-    double* part;                    /* 2 * n_items float64 partial slots */
-    int32_t item0, n_items;          /* this solve's contiguous range in the CG item list */
+    double* part;                    /* 2 * n_slots float64 partial slots, n_slots =
+                                        ceil(nx/48) ceil(ny/16) ceil(nz/64) */
+    int32_t item0, n_items;          /* this solve's range in the CG item list (the caller's
+                                        bookkeeping; the kernels do not read it) */
 } sw_batch_cg;
 
 /* Bytes of device CG state per solve. */
 int64_t sw_cg_state_bytes(void);
 
 /* Project every solve of the batch; CG items are (solve, z tile, y tile,
- * x chunk) quadruples over 16-row tiles (64 z columns wide), grouped by solve. `states` is device
+ * x chunk) quadruples over 16-row tiles (64 z columns wide). Dot products are
+ * summed per 48-plane x sub-chunk of a tile, then over a solve's slots in a
+ * fixed order: a solve's result depends neither on xchunk nor on the other
+ * solves of the batch. `states` is device
  * memory of n_solves * sw_cg_state_bytes() bytes. Runs until every solve has
  * stopped; info (host, n_solves entries) receives each solve's report. */
 int sw_batch_project(const sw_batch_cg* solves, int32_t n_solves, const int32_t* items, int64_t n_items,

@@ -29,7 +29,27 @@ from ._abi import check, lib
 STEP_BY = 8  # output rows of a batched step tile (blocks of 64 x 11 threads)
 MAX_PLANE = 1 << 28  # the kernels address a y-z plane with 32-bit offsets
 CG_BY = 16  # tile rows of the batched CG (fixed by the kernels)
-CG_XCHUNK = 192  # x planes per batched CG work item (longer items: fewer prologues and slot reductions)
+CG_SUB = 48  # x planes per partial sum of the batched CG (fixed by the kernels)
+CG_XCHUNKS = (192, 96, 48)  # x planes per batched CG work item, longest first
+CG_ITEMS_PER_CU = 4  # a longer item is taken only while the call keeps this many items per CU
+
+
+def cg_xchunk(shapes, n_cu):
+    """x planes per CG work item for one batched projection of grids `shapes`.
+
+    Longer items pay fewer march prologues, but a call with few solves needs
+    short items to occupy every CU: the longest of CG_XCHUNKS that still gives
+    CG_ITEMS_PER_CU items per CU, else the shortest. The kernels sum every
+    dot product per CG_SUB-plane sub-chunk, so the choice does not change any
+    result.
+    """
+    def n_items(xc):
+        return sum(math.ceil(nx / xc) * math.ceil(ny / CG_BY) * math.ceil(nz / 64) for nx, ny, nz in shapes)
+
+    for xc in CG_XCHUNKS:
+        if n_items(xc) >= CG_ITEMS_PER_CU * n_cu:
+            return xc
+    return CG_XCHUNKS[-1]
 
 
 @dataclass
@@ -194,17 +214,25 @@ class Batch:
         return raw
 
     # ---------------------------------------------------------- projection
-    def project(self, solves=None):
-        """Project every solve's force with one batched CG; keeps dT^2 f_S / rho for the loop."""
+    def project(self, solves=None, xchunk=None):
+        """Project every solve's force with one batched CG; keeps dT^2 f_S / rho for the loop.
+
+        `xchunk` (a multiple of CG_SUB) overrides :func:`cg_xchunk`; it changes
+        the speed only.
+        """
         solves = self.solves if solves is None else solves
         dev = E.device()
         cgs = (_abi.BatchCg * len(solves))()
         items = []
         keep = []
+        if xchunk is None:
+            xchunk = cg_xchunk([s.shape for s in solves], torch.cuda.get_device_properties(dev).multi_processor_count)
+        elif xchunk < 1 or xchunk % CG_SUB:
+            raise ValueError(f"xchunk must be a positive multiple of {CG_SUB}, got {xchunk}")
         for k, s in enumerate(solves):
             sp = s.spec
             nx, ny, nz = s.shape
-            xc = min(nx, CG_XCHUNK)
+            xc = min(nx, xchunk)
             n_items0 = len(items)
             for tx in range(math.ceil(nx / xc)):
                 for ty in range(math.ceil(ny / CG_BY)):
@@ -212,7 +240,8 @@ class Batch:
                         items.append((k, tz, ty, tx))
             phi = torch.empty(s.shape, dtype=torch.float64, device=dev)
             vec = torch.empty((3,) + s.shape, dtype=torch.float32, device=dev)  # r, p0, p1
-            part = torch.empty(2 * (len(items) - n_items0), dtype=torch.float64, device=dev)
+            n_slots = math.ceil(nx / CG_SUB) * math.ceil(ny / CG_BY) * math.ceil(nz / 64)
+            part = torch.empty(2 * n_slots, dtype=torch.float64, device=dev)
             keep += [phi, vec, part]
             c = cgs[k]
             c.nx, c.ny, c.nz = s.shape

@@ -140,6 +140,20 @@ __device__ inline CgTile batch_cg_tile(const sw_batch_cg& S, int4 it) {
     return cg_tile(g, x0, min(x0 + S.xchunk, S.nx), it.z * kCgBY, it.y * 64);
 }
 
+// A solve's partial slots: one per (x sub-chunk of kCgSub planes, y tile,
+// z tile), in that order. An item of xchunk planes (a multiple of kCgSub, or
+// all of nx) publishes one slot per sub-chunk it covers, each summed over the
+// same planes in the same order as an item of kCgSub planes would, so a
+// solve's dot products do not depend on the item length.
+__device__ inline int cg_n_slots(const sw_batch_cg& S) {
+    return ((S.nx + kCgSub - 1) / kCgSub) * ((S.ny + kCgBY - 1) / kCgBY) * ((S.nz + 63) / 64);
+}
+
+__device__ inline int cg_slot(const sw_batch_cg& S, int4 it, int j) {
+    const int nty = (S.ny + kCgBY - 1) / kCgBY, ntz = (S.nz + 63) / 64;
+    return ((it.w * S.xchunk / kCgSub + j) * nty + it.z) * ntz + it.y;
+}
+
 __global__ void __launch_bounds__(64 * kCgBY) batch_cg_init_kernel(const sw_batch_cg* __restrict__ solves, CgState* states,
                                                                    const int4* __restrict__ items, int64_t n_items) {
     __shared__ double red[16];
@@ -150,20 +164,26 @@ __global__ void __launch_bounds__(64 * kCgBY) batch_cg_init_kernel(const sw_batc
         CgState* st = states + it.x;
         const Grid g{S.nx, S.ny, S.nz};
         const CgTile t = batch_cg_tile(S, it);
-        double s = 0.0, mx = 0.0;
-        cg_init_tile<float, float>(S.f[0], S.f[1], S.f[2], S.phi, S.r, S.p0, g, 0.5 / S.dx, 0.5 / S.dy, 0.5 / S.dz,
-                                   t, s, mx);
-        s = block_sum(s, red);
-        mx = block_max(mx, red);
-        if (publish_slot(S.part, S.part + S.n_items, (int)(k - S.item0), s, mx, &st->counter, S.n_items)) {
-            double tot, m;
-            reduce_slots(S.part, S.part + S.n_items, S.n_items, red, &tot, &m);
-            if (threadIdx.x == 0 && threadIdx.y == 0) {
-                const int stop = st->stop, reason = st->reason;  // max_iters == 0 keeps its stop
-                cg_after_init(st, tot, m);
-                if (stop && !st->stop) {
-                    st->stop = stop;
-                    st->reason = reason;
+        const int n_slots = cg_n_slots(S);
+        for (int j = 0; t.x0 + j * kCgSub < t.x1; ++j) {
+            CgTile ts = t;
+            ts.x0 = t.x0 + j * kCgSub;
+            ts.x1 = min(ts.x0 + kCgSub, t.x1);
+            double s = 0.0, mx = 0.0;
+            cg_init_tile<float, float>(S.f[0], S.f[1], S.f[2], S.phi, S.r, S.p0, g, 0.5 / S.dx, 0.5 / S.dy,
+                                       0.5 / S.dz, ts, s, mx);
+            s = block_sum(s, red);
+            mx = block_max(mx, red);
+            if (publish_slot(S.part, S.part + n_slots, cg_slot(S, it, j), s, mx, &st->counter, n_slots)) {
+                double tot, m;
+                reduce_slots(S.part, S.part + n_slots, n_slots, red, &tot, &m);
+                if (threadIdx.x == 0 && threadIdx.y == 0) {
+                    const int stop = st->stop, reason = st->reason;  // max_iters == 0 keeps its stop
+                    cg_after_init(st, tot, m);
+                    if (stop && !st->stop) {
+                        st->stop = stop;
+                        st->reason = reason;
+                    }
                 }
             }
         }
@@ -188,12 +208,20 @@ __global__ void __launch_bounds__(64 * kCgBY) batch_cg_dir_kernel(const sw_batch
         const bool odd = st->iters & 1;
         const float* p_old = odd ? S.p1 : S.p0;
         float* p_new = odd ? S.p0 : S.p1;
-        double s = cg_dir_tile<float>(S.r, p_old, p_new, g, st->beta, 0.25 / (S.dx * S.dx), 0.25 / (S.dy * S.dy),
-                               0.25 / (S.dz * S.dz), t, lds);
-        s = block_sum(s, red);
-        if (publish_slot(S.part, nullptr, (int)(k - S.item0), s, 0.0, &st->counter, S.n_items)) {
+        const int n_slots = cg_n_slots(S);
+        // one slot per sub-chunk, stored as it ends and counted once per item:
+        // the block that completes a solve's slots has passed every other item
+        // of the solve, which read beta before marching
+        auto sink = [&](double& share, int j) {
+            const double b = block_sum(share, red);
+            share = 0.0;
+            if (threadIdx.x == 0 && threadIdx.y == 0) S.part[cg_slot(S, it, j)] = b;
+        };
+        cg_dir_tile<float>(S.r, p_old, p_new, g, st->beta, 0.25 / (S.dx * S.dx), 0.25 / (S.dy * S.dy),
+                           0.25 / (S.dz * S.dz), t, lds, sink);
+        if (publish_count(&st->counter, (t.x1 - t.x0 + kCgSub - 1) / kCgSub, n_slots)) {
             double denom, unused;
-            reduce_slots(S.part, nullptr, S.n_items, red, &denom, &unused);
+            reduce_slots(S.part, nullptr, n_slots, red, &denom, &unused);
             if (threadIdx.x == 0 && threadIdx.y == 0) cg_after_dir(st, denom);
         }
         __syncthreads();
@@ -218,14 +246,20 @@ __global__ void __launch_bounds__(64 * kCgBY) batch_cg_step_kernel(const sw_batc
         const float* p = odd ? S.p0 : S.p1;
         const float* p_prev = odd ? S.p1 : S.p0;
         const double qx = 0.25 / (S.dx * S.dx), qy = 0.25 / (S.dy * S.dy), qz = 0.25 / (S.dz * S.dz);
-        double s = odd ? cg_step_tile<float, kPhiPair>(S.phi, S.r, p, p_prev, g, st->alpha, st->alpha_prev, qx, qy,
-                                                       qz, t, lds)
-                       : cg_step_tile<float, kPhiSkip>(S.phi, S.r, p, p_prev, g, st->alpha, st->alpha_prev, qx, qy,
-                                                       qz, t, lds);
-        s = block_sum(s, red);
-        if (publish_slot(S.part, nullptr, (int)(k - S.item0), s, 0.0, &st->counter, S.n_items)) {
+        const int n_slots = cg_n_slots(S);
+        auto sink = [&](double& share, int j) {
+            const double b = block_sum(share, red);
+            share = 0.0;
+            if (threadIdx.x == 0 && threadIdx.y == 0) S.part[cg_slot(S, it, j)] = b;
+        };
+        const double alpha = st->alpha, alpha_prev = st->alpha_prev;
+        if (odd)
+            cg_step_tile<float, kPhiPair>(S.phi, S.r, p, p_prev, g, alpha, alpha_prev, qx, qy, qz, t, lds, sink);
+        else
+            cg_step_tile<float, kPhiSkip>(S.phi, S.r, p, p_prev, g, alpha, alpha_prev, qx, qy, qz, t, lds, sink);
+        if (publish_count(&st->counter, (t.x1 - t.x0 + kCgSub - 1) / kCgSub, n_slots)) {
             double rs_new, unused;
-            reduce_slots(S.part, nullptr, S.n_items, red, &rs_new, &unused);
+            reduce_slots(S.part, nullptr, n_slots, red, &rs_new, &unused);
             if (threadIdx.x == 0 && threadIdx.y == 0) cg_after_step(st, rs_new);
         }
         __syncthreads();

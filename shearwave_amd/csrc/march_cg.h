@@ -51,6 +51,22 @@ __device__ inline bool publish_slot(double* part_sum, double* part_max, int slot
     return last;
 }
 
+// Count k slots, stored by thread 0 of this block since its last count, as
+// published (one fence and one atomic per work item); returns true (in every
+// thread) in the block that completed the n slots.
+__device__ inline bool publish_count(unsigned int* counter, unsigned int k, unsigned int n) {
+    __shared__ int last;
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+    if (tid == 0) {
+        __threadfence();
+        const unsigned prev = atomicAdd(counter, k);
+        last = (prev + k == n);
+    }
+    __syncthreads();
+    if (last) __threadfence();
+    return last;
+}
+
 // In the last block: sum (and max) over the n slots in a fixed order.
 __device__ inline void reduce_slots(const double* part_sum, const double* part_max, int n, double* red,
                                     double* out_sum, double* out_max) {
@@ -271,11 +287,26 @@ __device__ __forceinline__ bool march_trip(int x, int x1, F& body, std::integer_
     return ((x + Ks < x1 && (body(Ic<Ks>{}, x + Ks), true)) && ...);
 }
 
-// body(Ic<k>{}, x) for x = x0 .. x1-1 with k = (x - x0) mod kCgU.
-template <typename F>
-__device__ __forceinline__ void march_cg(int x0, int x1, F&& body) {
-    for (int x = x0; x < x1; x += kCgU)
-        if (!march_trip(x, x1, body, std::make_integer_sequence<int, kCgU>{})) break;
+// Sub-chunks of kCgSub x planes: a tile hands its dot-product share to its
+// sink at the end of each one (and at x1), so partial sums can cover the same
+// planes whatever the length of the work item that marched over them.
+constexpr int kCgSub = 48;
+static_assert(kCgSub % kCgU == 0, "CG sub-chunk");
+
+// A sink that keeps one running share for the whole tile.
+struct CgNoSink {
+    __device__ void operator()(double&, int) const {}
+};
+
+// body(Ic<k>{}, x) for x = x0 .. x1-1 with k = (x - x0) mod kCgU; sink(j) after
+// the last plane of sub-chunk j (planes x0 + j kCgSub ..), uniform over the block.
+template <typename F, typename G>
+__device__ __forceinline__ void march_cg(int x0, int x1, F&& body, G&& sink) {
+    for (int x = x0; x < x1; x += kCgU) {
+        const bool full = march_trip(x, x1, body, std::make_integer_sequence<int, kCgU>{});
+        if (x + kCgU >= x1 || (x + kCgU - x0) % kCgSub == 0) sink((x - x0) / kCgSub);
+        if (!full) break;
+    }
 }
 
 // r and p_old of one point, as loaded.
@@ -286,11 +317,12 @@ struct CgRaw {
 
 // p = r + beta p_old (one fma, rounded to V, so every recomputation is
 // bit-identical to the stored p), A p. Writes p over the tile; returns this
-// thread's share of p.Ap.
-template <typename V>
+// thread's share of p.Ap (what the sink left of it: sink(share, j) may take
+// and clear the share at the end of sub-chunk j).
+template <typename V, typename Sink = CgNoSink>
 __device__ inline double cg_dir_tile(const V* __restrict__ r, const V* __restrict__ p_old, V* __restrict__ p,
                                      const Grid& g, double beta, double qx, double qy, double qz, const CgTile& t,
-                                     CgLdsT<V>& lds) {
+                                     CgLdsT<V>& lds, Sink sink = Sink{}) {
     const int r0 = threadIdx.y + 2, q0 = threadIdx.x + 2;
     const uint32_t cb = vb<V>(t.yzb), hb = vb<V>(t.hyzb);
     // plain loads: these lines also serve the halos of the neighbouring tiles
@@ -334,7 +366,7 @@ __device__ inline double cg_dir_tile(const V* __restrict__ r, const V* __restric
         const bool in = g.interior(x, t.y, t.z);
         if (in) s += c * cg_apply<V>(g, x, t.y, t.z, a, c, e, lds[B], r0, q0, qx, qy, qz);
         at_byte(plane_at(p, x, g), cb) = in ? (V)c : (V)0;
-    });
+    }, [&](int j) { sink(s, j); });
     return s;
 }
 
@@ -348,14 +380,16 @@ __device__ inline double cg_dir_tile(const V* __restrict__ r, const V* __restric
 enum : int { kPhiSkip = 0, kPhiEach = 1, kPhiPair = 2 };
 
 // phi update(s) per PHI, r -= alpha A p (A p recomputed from p) over the
-// tile; returns this thread's share of r.r (over r as stored). r and phi are
+// tile; returns this thread's share of r.r (over r as stored; the sink works
+// as in cg_dir_tile). r and phi are
 // read and written once here (non-temporal); p keeps plain loads, its lines
 // serve the halos of the neighbouring tiles. Loaded values stay in V until
 // the iteration that consumes them.
-template <typename V, int PHI>
+template <typename V, int PHI, typename Sink = CgNoSink>
 __device__ inline double cg_step_tile(double* __restrict__ phi, V* __restrict__ r, const V* __restrict__ p,
                                       const V* __restrict__ p_prev, const Grid& g, double alpha, double alpha_prev,
-                                      double qx, double qy, double qz, const CgTile& t, CgLdsT<V>& lds) {
+                                      double qx, double qy, double qz, const CgTile& t, CgLdsT<V>& lds,
+                                      Sink sink = Sink{}) {
     const int r0 = threadIdx.y + 2, q0 = threadIdx.x + 2;
     const uint32_t cb = vb<V>(t.yzb), hb = vb<V>(t.hyzb);
     auto center = [&](int xx) { return at_byte(plane_at(p, xx, g), cb); };
@@ -407,7 +441,7 @@ __device__ inline double cg_step_tile(double* __restrict__ phi, V* __restrict__ 
         const V rv = (V)((double)rq[K & 3] - alpha * ap);
         store_out(&at_byte(plane_at(r, x, g), cb), rv);
         s += (double)rv * (double)rv;
-    });
+    }, [&](int j) { sink(s, j); });
     return s;
 }
 

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
     missing = [s for s in declared_symbols() if not hasattr(lib, s)]
     assert not missing, missing
     lib.sw_abi_version.restype = ctypes.c_int
-    assert lib.sw_abi_version() == 2
+    assert lib.sw_abi_version() == 3
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="HIP library not built")

@@ -172,3 +172,26 @@ def test_batched_cg_stops_after_odd_and_even_iterations(sw):
         for c in range(3):
             if np.any(want[c]):
                 assert rel(got[c], want[c]) < 1e-6, (k, c, rel(got[c], want[c]))
+
+
+def test_batched_cg_item_length_changes_nothing(sw):
+    """Items of 48, 96 or 192 x planes: the kernels sum each dot product per 48-plane
+    sub-chunk, so the projected forces and iteration counts agree bit for bit (grids of
+    several sub-chunks, the last one ragged, and one shorter than a sub-chunk)."""
+    from shearwave_amd.batch import Batch
+
+    sps = [make_spec((150, 20, 70), 310, 4), make_spec((97, 17, 66), 311, 4), make_spec((40, 12, 64), 312, 4)]
+    got = []
+    for xc in (48, 96, 192):
+        b = Batch(sps)
+        info = b.project(xchunk=xc)
+        assert all(i["reason"] == "converged" for i in info), info
+        got.append(([i["iters"] for i in info], [[t.cpu().numpy() for t in s.force] for s in b.solves]))
+    for iters, forces in got[1:]:
+        assert iters == got[0][0]
+        for fa, fb in zip(forces, got[0][1]):
+            for a, c in zip(fa, fb):
+                np.testing.assert_array_equal(a, c)
+    assert any(np.any(f) for f in got[0][1][0])
+    with pytest.raises(ValueError, match="multiple of 48"):
+        Batch(sps[:1]).project(xchunk=100)
