@@ -256,7 +256,9 @@ int64_t sw_cg_state_bytes(void);
  * x chunk) quadruples over 16-row tiles (64 z columns wide). Dot products are
  * summed per 48-plane x sub-chunk of a tile, then over a solve's slots in a
  * fixed order: a solve's result depends neither on xchunk nor on the other
- * solves of the batch. `states` is device
+ * solves of the batch. The descriptors and items are checked first (one
+ * small device-to-host copy): SW_ERR_ARG unless every tile of every solve is
+ * listed exactly once and each xchunk is valid. `states` is device
  * memory of n_solves * sw_cg_state_bytes() bytes. Runs until every solve has
  * stopped; info (host, n_solves entries) receives each solve's report. */
 int sw_batch_project(const sw_batch_cg* solves, int32_t n_solves, const int32_t* items, int64_t n_items,

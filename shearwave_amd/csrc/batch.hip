@@ -385,11 +385,58 @@ int sw_graph_destroy(sw_graph* graph) {
 
 int64_t sw_cg_state_bytes(void) { return (int64_t)sizeof(CgState); }
 
+namespace {
+// The descriptors and items (device memory), checked on the host before any
+// launch: a solve whose slots were not each covered once would never complete
+// its reductions, and the host loop would wait on it.
+int check_batch_cg(const sw_batch_cg* solves, int32_t n_solves, const int32_t* items, int64_t n_items,
+                   hipStream_t s) {
+    std::vector<sw_batch_cg> d(n_solves);
+    std::vector<int4> it(n_items);
+    SW_TRY_HIP(hipMemcpyAsync(d.data(), solves, sizeof(sw_batch_cg) * n_solves, hipMemcpyDeviceToHost, s));
+    SW_TRY_HIP(hipMemcpyAsync(it.data(), items, sizeof(int4) * n_items, hipMemcpyDeviceToHost, s));
+    SW_TRY_HIP(hipStreamSynchronize(s));
+    std::vector<int64_t> base(n_solves + 1, 0);  // each solve's slots in one covered-once table
+    for (int k = 0; k < n_solves; ++k) {
+        const sw_batch_cg& S = d[k];
+        SW_REQUIRE(S.nx >= 1 && S.ny >= 1 && S.nz >= 1 && (int64_t)S.ny * S.nz < kMaxMarchPlane, SW_ERR_ARG,
+                   "batched CG solve %d: grid (%d,%d,%d) out of range", k, S.nx, S.ny, S.nz);
+        SW_REQUIRE(S.xchunk >= 1 && (S.xchunk % kCgSub == 0 || S.xchunk >= S.nx), SW_ERR_ARG,
+                   "batched CG solve %d: xchunk %d must be a multiple of %d or at least nx", k, S.xchunk, kCgSub);
+        SW_REQUIRE(S.f[0] && S.f[1] && S.f[2] && S.phi && S.r && S.p0 && S.p1 && S.part, SW_ERR_ARG,
+                   "batched CG solve %d: null field or workspace", k);
+        base[k + 1] = base[k] + (int64_t)((S.nx + kCgSub - 1) / kCgSub) * ((S.ny + kCgBY - 1) / kCgBY) *
+                                    ((S.nz + 63) / 64);
+    }
+    std::vector<unsigned char> seen(base[n_solves], 0);
+    for (int64_t j = 0; j < n_items; ++j) {
+        const int4 t = it[j];
+        SW_REQUIRE(t.x >= 0 && t.x < n_solves, SW_ERR_ARG, "CG item %lld: solve %d out of range", (long long)j, t.x);
+        const sw_batch_cg& S = d[t.x];
+        const int nty = (S.ny + kCgBY - 1) / kCgBY, ntz = (S.nz + 63) / 64;
+        SW_REQUIRE(t.y >= 0 && t.y < ntz && t.z >= 0 && t.z < nty && t.w >= 0 && (int64_t)t.w * S.xchunk < S.nx,
+                   SW_ERR_ARG, "CG item %lld: tile (%d,%d,%d) outside solve %d", (long long)j, t.y, t.z, t.w, t.x);
+        const int x0 = t.w * S.xchunk, x1 = std::min(x0 + S.xchunk, S.nx);
+        for (int x = x0; x < x1; x += kCgSub) {
+            const int64_t slot = base[t.x] + ((int64_t)(x / kCgSub) * nty + t.z) * ntz + t.y;
+            SW_REQUIRE(!seen[slot], SW_ERR_ARG, "CG item %lld: tile (%d,%d,%d) of solve %d listed twice",
+                       (long long)j, t.y, t.z, t.w, t.x);
+            seen[slot] = 1;
+        }
+    }
+    for (int k = 0; k < n_solves; ++k)
+        for (int64_t q = base[k]; q < base[k + 1]; ++q)
+            SW_REQUIRE(seen[q], SW_ERR_ARG, "batched CG solve %d: its items do not cover the grid", k);
+    return SW_OK;
+}
+}  // namespace
+
 int sw_batch_project(const sw_batch_cg* solves, int32_t n_solves, const int32_t* items, int64_t n_items,
                      void* states, sw_cg_info* info, void* stream) {
     SW_REQUIRE(solves && items && info && states, SW_ERR_ARG, "null pointer");
     SW_REQUIRE(n_solves >= 1 && n_items >= 1, SW_ERR_ARG, "empty batch");
     hipStream_t s = (hipStream_t)stream;
+    if (const int rc = check_batch_cg(solves, n_solves, items, n_items, s)) return rc;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_cg_dir_kernel, 64 * kCgBY, 0) != hipSuccess ||
         per_cu < 1)

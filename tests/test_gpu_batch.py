@@ -195,3 +195,14 @@ def test_batched_cg_item_length_changes_nothing(sw):
     assert any(np.any(f) for f in got[0][1][0])
     with pytest.raises(ValueError, match="multiple of 48"):
         Batch(sps[:1]).project(xchunk=100)
+
+
+def test_batched_cg_rejects_bad_descriptors(sw, monkeypatch):
+    """The library checks the descriptors before any launch: an x chunk that is not a
+    multiple of 48 (and shorter than nx) is an argument error, not a stalled reduction."""
+    from shearwave_amd import batch as B
+
+    b = B.Batch([make_spec((150, 20, 70), 313, 4)])
+    monkeypatch.setattr(B, "cg_xchunk", lambda shapes, n_cu: 100)
+    with pytest.raises(RuntimeError, match="multiple of 48"):
+        b.project()
