@@ -26,3 +26,17 @@ def test_plan_covers_every_step_once_in_order(n, two_step):
 def test_plan_pairs_steps():
     assert [(int(e) >> 1, int(e) & 1) for e in pass_plan(5)] == [(0, 1), (2, 1), (4, 0)]
     assert all(int(e) & 1 == 0 for e in pass_plan(9, two_step=False))
+
+
+def test_cg_xchunk_rule():
+    """The batched CG item length: the longest of 192, 96, 48 planes that keeps four items
+    per CU in the call, else 48 (host logic only)."""
+    from shearwave_amd.batch import CG_SUB, CG_XCHUNKS, cg_xchunk
+
+    assert all(xc % CG_SUB == 0 for xc in CG_XCHUNKS)
+    one = [(128, 128, 160)]  # 8 y tiles x 3 z tiles: 24 items at 192 planes, 72 at 48
+    assert cg_xchunk(one, 256) == 48
+    assert cg_xchunk(one, 6) == 192  # 24 items >= 4 x 6 CUs
+    assert cg_xchunk(one, 9) == 96  # 24 < 36 <= 48 items at 96 planes
+    assert cg_xchunk(one * 48, 256) == 192  # 1152 items >= 1024
+    assert cg_xchunk([(8, 8, 8)], 256) == 48
