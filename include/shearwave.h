@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define SW_ABI_VERSION 3
+#define SW_ABI_VERSION 4
 
 /* status codes */
 #define SW_OK 0
@@ -81,11 +81,13 @@ int sw_gradient_center(int dtype, const void* phi, void* gx, void* gy, void* gz,
 /* ------------------------------------------------------------- projection */
 
 typedef struct sw_cg_info {
-    int64_t iters;   /* number of phi updates performed */
-    double relres;   /* last sqrt(r.r / rs0) */
-    double max_div;  /* max |div_c(mask f)| of the input */
-    int32_t reason;  /* SW_CG_* */
+    int64_t iters;      /* number of phi updates performed */
+    double relres;      /* last sqrt(r.r / rs0) of the residual the CG iterates on */
+    double max_div;     /* max |div_c(mask f)| of the input */
+    int32_t reason;     /* SW_CG_* */
     int32_t pad;
+    double true_relres; /* sqrt(|b - A phi|^2 / rs0), b = -div_c(mask f) in float64, from phi as
+                           returned (-1 where not computed) */
 } sw_cg_info;
 
 /* Bytes of device workspace sw_project_vector_field needs for an nx*ny*nz grid. */
@@ -153,6 +155,14 @@ typedef struct sw_fdtd_outputs {
     int64_t n_traces;
     void* trace_out;
     int64_t trace_ld;
+    /* non-finite detection (no reference counterpart: the reference lets an
+     * unstable run fill u with inf/NaN silently). Every finite_stride global
+     * steps (0 = never) u is checked, after the first pass that ends at or
+     * past such a step, so the check never splits a pass. If u holds an inf or
+     * NaN and *nonfinite_step (device int64; the caller sets it to -1) is still
+     * negative, it becomes the global index of that pass's last step. */
+    int64_t finite_stride;
+    int64_t* nonfinite_step;
 } sw_fdtd_outputs;
 
 /* The time loop of shear_fdtd (solver.py:194-239) for n_steps steps with
@@ -178,9 +188,12 @@ int sw_export_state(int dtype, int form, const void* const* u, const void* const
 int sw_combine3(int dtype, const void* const* a, const void* const* b, double alpha, double beta, void* const* out,
                 int64_t n, void* stream);
 
-/* out[k] = f[idx[k]] for k < count (device int64 indices). */
-int sw_gather_traces(int dtype, const void* f, const int64_t* idx, int64_t count, void* out, int64_t out_stride,
-                     void* stream);
+/* out[k*out_stride] = f[idx[k]] for k < count (device int64 flat indices into
+ * the n elements of f): the per-step sensor traces of shear_fdtd_jax
+ * (solver.py:396-413, 444-447). An index outside [0, n) reads nothing and
+ * yields NaN (the kernel checks every index, so a bad one cannot fault). */
+int sw_gather_traces(int dtype, const void* f, int64_t n, const int64_t* idx, int64_t count, void* out,
+                     int64_t out_stride, void* stream);
 
 /* ---------------------------------------------------------------- batching */
 
@@ -278,10 +291,13 @@ int sw_adjoint_step(int dtype, const void* const* u, const void* const* d, const
                     const void* eta_over_rho, double* grad_m, double* grad_e, int64_t nx, int64_t ny, int64_t nz,
                     double dx, double dy, double dz, double dt, double vel_damp, void* stream);
 
-/* f[idx[k]] += grad[k*stride] for k < count (unique device int64 indices): the
- * misfit gradient of sensor traces entering the adjoint of u_z. */
-int sw_inject_traces(int dtype, void* f, const int64_t* idx, int64_t count, const void* grad, int64_t stride,
-                     void* stream);
+/* f[idx[k]] += grad[k*stride] for k < count (unique device int64 flat indices
+ * into the n elements of f): the misfit gradient of sensor traces entering the
+ * adjoint of u_z. An index outside [0, n) is skipped. No reference
+ * counterpart (the reference has no adjoint; its JAX backend differentiates
+ * shear_fdtd_jax, solver.py:246). */
+int sw_inject_traces(int dtype, void* f, int64_t n, const int64_t* idx, int64_t count, const void* grad,
+                     int64_t stride, void* stream);
 
 /* ------------------------------------------------------------------ strain */
 

@@ -40,7 +40,8 @@ class SwError(RuntimeError):
 
 
 class CgInfo(C.Structure):
-    _fields_ = [("iters", _i64), ("relres", _d), ("max_div", _d), ("reason", C.c_int32), ("pad", C.c_int32)]
+    _fields_ = [("iters", _i64), ("relres", _d), ("max_div", _d), ("reason", C.c_int32), ("pad", C.c_int32),
+                ("true_relres", _d)]
 
 
 class StaticInfo(C.Structure):
@@ -84,6 +85,8 @@ class FdtdOutputs(C.Structure):
         ("n_traces", _i64),
         ("trace_out", _p),
         ("trace_ld", _i64),
+        ("finite_stride", _i64),
+        ("nonfinite_step", _p),
     ]
 
 
@@ -177,7 +180,7 @@ SIGNATURES = {
     "sw_export_state": (_i32, [_i32, _i32, C.POINTER(_p), C.POINTER(_p), _d, _p, _p, _i64, _p]),
     "sw_import_state": (_i32, [_i32, _i32, _p, _p, _d, C.POINTER(_p), C.POINTER(_p), _i64, _i64, _i64, _p]),
     "sw_combine3": (_i32, [_i32, C.POINTER(_p), C.POINTER(_p), _d, _d, C.POINTER(_p), _i64, _p]),
-    "sw_gather_traces": (_i32, [_i32, _p, _p, _i64, _p, _i64, _p]),
+    "sw_gather_traces": (_i32, [_i32, _p, _i64, _p, _i64, _p, _i64, _p]),
     "sw_batch_step": (_i32, [_p, _p, _i64, _p, _i32, _i32, _i64, _p]),
     "sw_batch_graph_create": (_i32, [_p, _p, _i64, _p, _i32, _i32, _i64, C.POINTER(_p)]),
     "sw_graph_launch": (_i32, [_p, _p]),
@@ -186,7 +189,7 @@ SIGNATURES = {
     "sw_batch_project": (_i32, [_p, _i32, _p, _i64, _p, C.POINTER(CgInfo), _p]),
     "sw_adjoint_step": (_i32, [_i32, C.POINTER(_p), C.POINTER(_p), C.POINTER(_p), C.POINTER(_p), C.POINTER(_p),
                                C.POINTER(_p), _p, _p, _p, _p, _i64, _i64, _i64, _d, _d, _d, _d, _d, _p]),
-    "sw_inject_traces": (_i32, [_i32, _p, _p, _i64, _p, _i64, _p]),
+    "sw_inject_traces": (_i32, [_i32, _p, _i64, _p, _i64, _p, _i64, _p]),
 }
 
 _lock = threading.Lock()

@@ -105,7 +105,7 @@ class Problem:
 
         def inject(buf, n):
             # dJ/du_n for n >= 1: traces column n-1 holds u_z after step n
-            check(lib().sw_inject_traces(self.code, buf[2].data_ptr(), self.sensors.data_ptr(), n_sens,
+            check(lib().sw_inject_traces(self.code, buf[2].data_ptr(), buf[2].numel(), self.sensors.data_ptr(), n_sens,
                                          g[:, n - 1].data_ptr() if g.stride(1) == 1 else g[:, n - 1].contiguous().data_ptr(),
                                          g.stride(0), st))
 

@@ -22,10 +22,16 @@ Build-only options (namespaced ``sw_``):
     products and the residual norm are float64 in both, as poisson_cg keeps
     them (solver.py:1020-1041);
   * ``sw_strain_stride``: with ``shear_fdtd(..., return_strain=True)``, the
-    four strain invariants every that many steps, computed in-loop.
+    four strain invariants every that many steps, computed in-loop;
+  * ``sw_check_finite``: check u for inf/NaN every that many steps (default
+    ``max(1, n_steps // 20)``, 0 = never) and warn (``RuntimeWarning``) with
+    the first step found; the returned arrays are unchanged, as the
+    reference returns them (an unstable dT fills them with inf/NaN).
 """
 
 from __future__ import annotations
+
+import warnings
 
 import numpy as np
 import torch
@@ -143,6 +149,7 @@ def shear_fdtd(  # noqa: PLR0912, PLR0915
     np_dtype = np.float32 if dtype == "float32" else np.float64
     form = opts.get("sw_form", "increment")
     strain_stride = int(opts.get("sw_strain_stride", 0) or 0)
+    finite_stride = int(opts.get("sw_check_finite", max(1, n_steps // 20)) or 0)
     if return_strain and strain_stride <= 0:
         raise ValueError("return_strain needs opts['sw_strain_stride'] > 0.")
 
@@ -197,7 +204,7 @@ def shear_fdtd(  # noqa: PLR0912, PLR0915
         tr_idx = torch.from_numpy(trace_flat).to(dev)
         tr_out = torch.zeros((trace_flat.shape[0], n_steps), dtype=tdt, device=dev)
     outs = dict(snap_stride=snapshot_stride or 0, snap_out=snaps, inv_stride=strain_stride if return_strain else 0,
-                inv_out=inv, trace_idx=tr_idx, trace_out=tr_out)
+                inv_out=inv, trace_idx=tr_idx, trace_out=tr_out, finite_stride=finite_stride)
 
     if per_step:
         env = np.ones(n_steps)
@@ -235,6 +242,11 @@ def shear_fdtd(  # noqa: PLR0912, PLR0915
                 solve.reproject(poisson_tol, poisson_max_iters)
 
     u_aos, v_aos = solve.export()
+    if finite_stride:
+        bad = solve.first_nonfinite_step()
+        if bad is not None:
+            warnings.warn(f"shear_fdtd: u holds inf/NaN after step {bad} (unstable dT or inputs?)", RuntimeWarning,
+                          stacklevel=2)
     v_dtype = np.float64 if (dtype == "float32" and isinstance(dT, np.float64)) else None
     if return_device:
         res = [u_aos, v_aos if v_dtype is None else v_aos.to(torch.float64)]

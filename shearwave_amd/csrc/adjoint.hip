@@ -287,11 +287,15 @@ __global__ void __launch_bounds__(64 * BY) adjoint_march_kernel(AdjArgs<T> a, in
     });
 }
 
+// f[idx[k]] += grad[k]; an index outside [0, n) is skipped.
 template <typename T>
-__global__ void __launch_bounds__(256) inject_kernel(T* __restrict__ f, const int64_t* __restrict__ idx, int64_t count,
-                                                     const T* __restrict__ grad, int64_t stride) {
+__global__ void __launch_bounds__(256) inject_kernel(T* __restrict__ f, int64_t n, const int64_t* __restrict__ idx,
+                                                     int64_t count, const T* __restrict__ grad, int64_t stride) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < count) f[idx[k]] += grad[k * stride];
+    if (k < count) {
+        const int64_t j = idx[k];
+        if (j >= 0 && j < n) f[j] += grad[k * stride];
+    }
 }
 
 }  // namespace
@@ -374,16 +378,16 @@ int sw_adjoint_step(int dtype, const void* const* u, const void* const* d, const
     return SW_OK;
 }
 
-int sw_inject_traces(int dtype, void* f, const int64_t* idx, int64_t count, const void* grad, int64_t stride,
-                     void* stream) {
+int sw_inject_traces(int dtype, void* f, int64_t n, const int64_t* idx, int64_t count, const void* grad,
+                     int64_t stride, void* stream) {
     int rc = check_dtype(dtype);
     if (rc) return rc;
-    SW_REQUIRE(count >= 0 && stride >= 1, SW_ERR_ARG, "bad count/stride");
+    SW_REQUIRE(count >= 0 && stride >= 1 && n >= 0, SW_ERR_ARG, "bad count/stride/n");
     if (count == 0) return SW_OK;
     SW_REQUIRE(f && idx && grad, SW_ERR_ARG, "null pointer");
     SW_DISPATCH(dtype, T,
                 hipLaunchKernelGGL(inject_kernel<T>, dim3((unsigned)((count + 255) / 256)), dim3(256), 0,
-                                   (hipStream_t)stream, (T*)f, idx, count, (const T*)grad, stride));
+                                   (hipStream_t)stream, (T*)f, n, idx, count, (const T*)grad, stride));
     SW_LAUNCH_CHECK();
     return SW_OK;
 }

@@ -131,7 +131,7 @@ __global__ void advance_kernel(long long* counter) { *counter += 1; }
 
 __global__ void batch_cg_reset_kernel(const sw_batch_cg* solves, int n, CgState* states) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < n) cg_reset(states + s, solves[s].tol, (long long)solves[s].max_iters);
+    if (s < n) cg_reset(states + s, solves[s].tol, (long long)solves[s].max_iters, solves[s].tol);
 }
 
 __device__ inline CgTile batch_cg_tile(const sw_batch_cg& S, int4 it) {
@@ -347,22 +347,22 @@ int sw_batch_graph_create(const sw_batch_solve* solves, const int32_t* items, in
     SW_TRY_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
     hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) {
-        hipStreamDestroy(cap);
+        (void)hipStreamDestroy(cap);
         SW_TRY_HIP(e);
     }
     const int rc = launch_batch_passes(solves, items, n_items, gpass, blocks, n_passes, cap);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(cap, &g);
-    hipStreamDestroy(cap);
+    (void)hipStreamDestroy(cap);
     if (rc != SW_OK) {
-        if (g) hipGraphDestroy(g);
+        if (g) (void)hipGraphDestroy(g);
         return rc;
     }
     SW_TRY_HIP(e);
     hipGraphExec_t ex = nullptr;
     e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     if (e != hipSuccess) {
-        hipGraphDestroy(g);
+        (void)hipGraphDestroy(g);
         SW_TRY_HIP(e);
     }
     *graph = new sw_graph{g, ex};
@@ -377,8 +377,8 @@ int sw_graph_launch(sw_graph* graph, void* stream) {
 
 int sw_graph_destroy(sw_graph* graph) {
     if (!graph) return SW_OK;
-    hipGraphExecDestroy(graph->exec);
-    hipGraphDestroy(graph->graph);
+    (void)hipGraphExecDestroy(graph->exec);
+    (void)hipGraphDestroy(graph->graph);
     delete graph;
     return SW_OK;
 }
@@ -478,6 +478,7 @@ int sw_batch_project(const sw_batch_cg* solves, int32_t n_solves, const int32_t*
         info[k].max_div = h[k].max_div;
         info[k].reason = h[k].reason;
         info[k].pad = 0;
+        info[k].true_relres = h[k].true_relres;  // -1: the batched CG has no true-residual pass
     }
     return SW_OK;
 }

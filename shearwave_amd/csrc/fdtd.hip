@@ -279,6 +279,18 @@ int launch_step_any(const StepArgs<T>& a, bool incr, bool visc, bool force, int 
     return SW_ERR_ARG;
 }
 
+// Non-finite detection: if any of the three fields holds an inf or NaN and
+// *flag is still -1, *flag = step. One compare-and-swap per wave that saw one.
+template <typename T>
+__global__ void __launch_bounds__(256) finite_check_kernel(const T* __restrict__ a, const T* __restrict__ b,
+                                                           const T* __restrict__ c, int64_t n, int64_t step,
+                                                           long long* flag) {
+    bool bad = false;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        bad |= !(isfinite(a[i]) && isfinite(b[i]) && isfinite(c[i]));
+    if (__any(bad) && (threadIdx.x & (kWave - 1)) == 0) atomicCAS((unsigned long long*)flag, ~0ull, (unsigned long long)step);
+}
+
 template <typename T>
 int run_steps(sw_fdtd_state* st, int64_t n_steps, const double* env, const sw_fdtd_outputs* out, hipStream_t s) {
     const Grid g{(int)st->nx, (int)st->ny, (int)st->nz};
@@ -357,9 +369,15 @@ int run_steps(sw_fdtd_state* st, int64_t n_steps, const double* env, const sw_fd
                 st->u_next[c] = old_b;
             }
         }
+        const int64_t done_before = step0 + it;
         it += three ? 3 : two ? 2 : 1;
         if (!out) continue;
         const int64_t step = step0 + it - 1;  // the last step of this pass
+        if (out->finite_stride > 0 && (step + 1) / out->finite_stride > done_before / out->finite_stride) {
+            hipLaunchKernelGGL(finite_check_kernel<T>, dim3(flat_grid(n)), dim3(256), 0, s, (const T*)st->u[0],
+                               (const T*)st->u[1], (const T*)st->u[2], n, step, (long long*)out->nonfinite_step);
+            SW_LAUNCH_CHECK();
+        }
         if (out->snap_stride > 0 && (step + 1) % out->snap_stride == 0) {
             const int64_t k = (step + 1) / out->snap_stride - 1;
             rc = sw_soa_to_aos3(DType<T>::code, st->u[0], st->u[1], st->u[2], (T*)out->snap_out + k * 3 * n, n, s);
@@ -374,7 +392,7 @@ int run_steps(sw_fdtd_state* st, int64_t n_steps, const double* env, const sw_fd
             if (rc) return rc;
         }
         if (out->n_traces > 0) {
-            rc = sw_gather_traces(DType<T>::code, st->u[2], out->trace_idx, out->n_traces,
+            rc = sw_gather_traces(DType<T>::code, st->u[2], n, out->trace_idx, out->n_traces,
                                   (T*)out->trace_out + step, out->trace_ld, s);
             if (rc) return rc;
         }
@@ -409,6 +427,7 @@ extern "C" int sw_fdtd_run(sw_fdtd_state* st, int64_t n_steps, const double* env
         SW_REQUIRE(out->snap_stride <= 0 || out->snap_out, SW_ERR_ARG, "snap_out is null");
         SW_REQUIRE(out->inv_stride <= 0 || out->inv_out, SW_ERR_ARG, "inv_out is null");
         SW_REQUIRE(out->n_traces <= 0 || (out->trace_idx && out->trace_out), SW_ERR_ARG, "null trace buffers");
+        SW_REQUIRE(out->finite_stride <= 0 || out->nonfinite_step, SW_ERR_ARG, "nonfinite_step is null");
         SW_REQUIRE(out->n_traces <= 0 || out->trace_ld >= out->step0 + n_steps, SW_ERR_ARG,
                    "trace_ld %lld < last step %lld", (long long)out->trace_ld, (long long)(out->step0 + n_steps));
     }

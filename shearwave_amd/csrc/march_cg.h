@@ -22,7 +22,8 @@ struct CgState {
     unsigned int counter;  // slots published in the current kernel
     int pad1;
     long long max_iters;   // stop after this many phi updates (solver.py:1033)
-    int pad2[4];
+    double true_relres;    // sqrt(|b - A phi|^2 / rs0) from the last true-residual pass
+    double final_tol;      // the caller's tol; `tol` may be a looser stage before a residual replacement
 };
 static_assert(sizeof(CgState) <= 128, "CgState too large");
 
@@ -133,10 +134,43 @@ __device__ inline void cg_after_step(CgState* st, double rs_new) {
     }
 }
 
+// After a true-residual pass (r = b - A phi, r.r = rs). Without `replace` it
+// only records the norm. With it, r replaces the recursively updated residual:
+// the solve stops if the true norm meets the caller's tol (or the iteration
+// budget is spent), and otherwise continues from it as from an ordinary
+// iteration (beta = rs / rs_old, solver.py:1047) towards the stage tol `next_tol`.
+__device__ inline void cg_after_true_residual(CgState* st, double rs, int replace, double next_tol) {
+    st->counter = 0u;
+    const double rel = sqrt(rs / st->rs0);
+    st->true_relres = rel;
+    if (!replace) return;
+    st->relres = rel;
+    if (!isfinite(rs)) {
+        st->stop = 1;
+        st->reason = SW_CG_NONFINITE;
+    } else if (rel < st->final_tol) {
+        st->stop = 1;
+        st->reason = SW_CG_CONVERGED;
+    } else if (st->iters >= st->max_iters) {
+        st->stop = 1;
+        st->reason = SW_CG_MAX_ITERS;
+    } else {
+        st->beta = rs / fmax(st->rs_old, kEps64);
+        st->rs_old = rs;
+        st->tol = next_tol;
+        st->stop = 0;
+        st->reason = SW_CG_RUNNING;
+    }
+}
+
 // Fresh state for a solve with tolerance tol and at most max_iters iterations.
-__device__ inline void cg_reset(CgState* st, double tol, long long max_iters) {
+// `stage_tol` (>= tol) is where the recursion first stops for a true-residual
+// check; pass tol when no residual replacement is planned.
+__device__ inline void cg_reset(CgState* st, double tol, long long max_iters, double stage_tol) {
     st->rs_old = st->rs0 = st->alpha = st->beta = st->relres = st->max_div = st->alpha_prev = 0.0;
-    st->tol = tol;
+    st->true_relres = -1.0;
+    st->final_tol = tol;
+    st->tol = stage_tol;
     st->iters = 0;
     st->stop = max_iters <= 0 ? 1 : 0;
     st->reason = max_iters <= 0 ? SW_CG_MAX_ITERS : SW_CG_RUNNING;

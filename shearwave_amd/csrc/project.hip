@@ -33,7 +33,9 @@ namespace {
 
 constexpr int kMaxRedBlocks = 2048;
 
-__global__ void cg_reset_kernel(CgState* st, double tol, long long max_iters) { cg_reset(st, tol, max_iters); }
+__global__ void cg_reset_kernel(CgState* st, double tol, long long max_iters, double stage_tol) {
+    cg_reset(st, tol, max_iters, stage_tol);
+}
 
 // r = -div_c(mask f) on the interior, p_old = 0, phi = 0; reduces r.r and max|div|.
 template <typename T, typename V>
@@ -78,6 +80,51 @@ __global__ void __launch_bounds__(256) cg_init_kernel(const T* __restrict__ fx, 
 // `v(k)` returns x at offset k*stride; c is the voxel's coordinate on the axis.
 #define SW_AXIS_TERM(v, c, n, q) \
     ((((c) + 1 < (n) - 1) ? ((v)(2) - (v)(0)) : 0.0) - (((c) - 1 > 0) ? ((v)(0) - (v)(-2)) : 0.0)) * (q)
+
+// True residual r = b - A phi in float64, with b = -div_c(mask f) formed as
+// cg_init_kernel forms it but never rounded to V, and phi as accumulated (every
+// deferred term added). Reduces r.r over the values as rounded to V. With
+// `replace` the rounded values overwrite r and the last block restarts the
+// recursion from them (cg_after_true_residual); otherwise r is left alone and
+// only the norm is recorded.
+template <typename T, typename V>
+__global__ void __launch_bounds__(256) cg_true_residual_kernel(const T* __restrict__ fx, const T* __restrict__ fy,
+                                                               const T* __restrict__ fz,
+                                                               const double* __restrict__ phi, V* __restrict__ r,
+                                                               Grid g, double i2hx, double i2hy, double i2hz,
+                                                               double qx, double qy, double qz, double* part_sum,
+                                                               CgState* st, int replace, double next_tol) {
+    __shared__ double red[16];
+    const int64_t P = g.plane();
+    const int64_t R = g.nz;
+    double s = 0.0;
+    SW_FOR_EACH_VOXEL(g, {
+        if (g.interior(x, y, z)) {
+            const double xp = (x + 1 < g.nx - 1) ? (double)fx[i + P] : 0.0;
+            const double xm = (x - 1 > 0) ? (double)fx[i - P] : 0.0;
+            const double yp = (y + 1 < g.ny - 1) ? (double)fy[i + R] : 0.0;
+            const double ym = (y - 1 > 0) ? (double)fy[i - R] : 0.0;
+            const double zp = (z + 1 < g.nz - 1) ? (double)fz[i + 1] : 0.0;
+            const double zm = (z - 1 > 0) ? (double)fz[i - 1] : 0.0;
+            const double b = -((xp - xm) * i2hx + (yp - ym) * i2hy + (zp - zm) * i2hz);
+            const double fv = phi[i];
+            auto px = [&](int k) { return k == 0 ? fv : phi[i + k * P]; };
+            auto py = [&](int k) { return k == 0 ? fv : phi[i + k * R]; };
+            auto pz = [&](int k) { return k == 0 ? fv : phi[i + k]; };
+            const double aphi = -(SW_AXIS_TERM(px, x, g.nx, qx) + SW_AXIS_TERM(py, y, g.ny, qy) +
+                                  SW_AXIS_TERM(pz, z, g.nz, qz));
+            const V rv = (V)(b - aphi);
+            s += (double)rv * (double)rv;
+            if (replace) r[i] = rv;
+        }
+    });
+    s = block_sum(s, red);
+    if (publish_slot(part_sum, nullptr, blockIdx.x, s, 0.0, &st->counter, gridDim.x)) {
+        double rs, unused;
+        reduce_slots(part_sum, nullptr, gridDim.x, red, &rs, &unused);
+        if (threadIdx.x == 0 && threadIdx.y == 0) cg_after_true_residual(st, rs, replace, next_tol);
+    }
+}
 
 // Direction update fused with A p and p.Ap.
 template <typename V>
@@ -197,15 +244,15 @@ __global__ void __launch_bounds__(64 * kCgBY) cg_step_march_kernel(double* __res
     }
 }
 
-// After the march CG: the deferred phi term of a solve that stopped after an
-// odd iteration (its p is in p_odd, the buffer odd iterations write).
+// After the march CG: the deferred phi term of a solve whose last iteration
+// left its term pending (an odd number of iterations since phi was last
+// complete; the host launches it only then). p_last is that iteration's p.
 template <typename V>
 __global__ void __launch_bounds__(64 * kCgBY) cg_flush_march_kernel(double* __restrict__ phi,
-                                                                    const V* __restrict__ p_odd, Grid g, int xchunk,
+                                                                    const V* __restrict__ p_last, Grid g, int xchunk,
                                                                     int nty, int ntz, const CgState* st) {
-    if ((st->iters & 1) == 0) return;
     const CgTile t = cg_block_tile(g, xchunk, nty, ntz);
-    cg_flush_phi_tile<V>(phi, p_odd, g, st->alpha, t);
+    cg_flush_phi_tile<V>(phi, p_last, g, st->alpha, t);
 }
 
 // One 1024-thread block per CU: long x chunks keep the per-chunk prologue
@@ -316,55 +363,88 @@ unsigned red_grid(const Grid& g) {
     return row_grid(g, cap);
 }
 
+// Stop levels of the float32-vector CG before its true-residual checks: each
+// level is this factor below the one before, down to the caller's tol.
+constexpr double kStageFactor = 1e-4;
+// Residual replacements after which the solve ends with whatever it reached.
+constexpr int kMaxReplacements = 8;
+
 // Launch the CG from r0 to its stop on the stream; returns the final state
 // in hs (host). V is the storage of r and p.
-template <typename V>
-int run_cg(const Workspace& w, const Grid& g, bool march, const CgShape& cs, unsigned nb, double qx, double qy,
-           double qz, int64_t max_iters, CgState& hs, hipStream_t s) {
+//
+// Float64 vectors iterate to tol, as poisson_cg does. With float32 vectors the
+// recursively updated r drifts from b - A phi by float32 rounding (r0 itself
+// is b rounded), so the recursion stops first at a looser stage level; there
+// the true residual b - A phi is formed in float64 and replaces r (residual
+// replacement), and iteration continues to the next level, down to tol, until
+// the TRUE residual meets tol. Each replacement removes the drift accumulated
+// so far, which scales with the residual at that point, so the final drift is
+// ~ eps32 * (last stage level) relative to r0, far below tol. Either way a
+// last pass records the true residual of the returned phi.
+template <typename T, typename V>
+int run_cg(const Workspace& w, const Grid& g, bool march, const CgShape& cs, unsigned nb, const T* fx, const T* fy,
+           const T* fz, double i2hx, double i2hy, double i2hz, double qx, double qy, double qz, double tol,
+           int64_t max_iters, double stage, CgState& hs, hipStream_t s) {
     const dim3 blk = row_block();
+    const bool replacing = sizeof(V) == 4;
     SW_TRY_HIP(hipMemcpyAsync(&hs, w.st, sizeof(CgState), hipMemcpyDeviceToHost, s));
     SW_TRY_HIP(hipStreamSynchronize(s));
-    int64_t launched = 0;
-    int64_t chunk = 8;
+    V* const p_odd = (V*)w.p1;   // iteration k (counted from 1) writes its p here when k is odd
+    V* const p_even = (V*)w.p0;  // and here when k is even
     V* r = (V*)w.r;
-    V* p_old = (V*)w.p0;
-    V* p_new = (V*)w.p1;
-    // iteration k = launched + j + 1 writes p into p1 when k is odd, p0 when even
-    while (!hs.stop && launched < max_iters) {
-        const int64_t k = (max_iters - launched < chunk) ? (max_iters - launched) : chunk;
-        for (int64_t j = 0; j < k; ++j) {
-            if (march) {
-                const bool odd = ((launched + j) & 1) == 0;
-                hipLaunchKernelGGL(cg_dir_march_kernel<V>, dim3(cs.blocks()), dim3(64, kCgBY), 0, s, r, p_old, p_new,
-                                   g, qx, qy, qz, cs.xchunk, cs.nty, cs.ntz, w.part_sum, w.st);
-                if (odd)
-                    hipLaunchKernelGGL((cg_step_march_kernel<V, kPhiSkip>), dim3(cs.blocks()), dim3(64, kCgBY), 0, s,
-                                       w.phi, r, p_new, p_old, g, qx, qy, qz, cs.xchunk, cs.nty, cs.ntz, w.part_sum,
-                                       w.st);
-                else
-                    hipLaunchKernelGGL((cg_step_march_kernel<V, kPhiPair>), dim3(cs.blocks()), dim3(64, kCgBY), 0, s,
-                                       w.phi, r, p_new, p_old, g, qx, qy, qz, cs.xchunk, cs.nty, cs.ntz, w.part_sum,
-                                       w.st);
-            } else {
-                hipLaunchKernelGGL(cg_dir_kernel<V>, dim3(nb), blk, 0, s, r, p_old, p_new, g, qx, qy, qz, w.part_sum,
-                                   w.st);
-                hipLaunchKernelGGL(cg_step_kernel<V>, dim3(nb), blk, 0, s, w.phi, r, p_new, g, qx, qy, qz, w.part_sum,
-                                   w.st);
+    int64_t base = 0;  // iterations whose phi terms are all in phi (the pairing restarts after a flush)
+    for (int round = 0;; ++round) {
+        int64_t launched = hs.iters;
+        int64_t chunk = 8;
+        while (!hs.stop && launched < max_iters) {
+            const int64_t k = (max_iters - launched < chunk) ? (max_iters - launched) : chunk;
+            for (int64_t j = 0; j < k; ++j) {
+                const int64_t it = launched + j + 1;
+                V* p_new = (it & 1) ? p_odd : p_even;
+                V* p_old = (it & 1) ? p_even : p_odd;
+                if (march) {
+                    const bool skip = ((it - base) & 1) == 1;
+                    hipLaunchKernelGGL(cg_dir_march_kernel<V>, dim3(cs.blocks()), dim3(64, kCgBY), 0, s, r, p_old,
+                                       p_new, g, qx, qy, qz, cs.xchunk, cs.nty, cs.ntz, w.part_sum, w.st);
+                    if (skip)
+                        hipLaunchKernelGGL((cg_step_march_kernel<V, kPhiSkip>), dim3(cs.blocks()), dim3(64, kCgBY), 0,
+                                           s, w.phi, r, p_new, p_old, g, qx, qy, qz, cs.xchunk, cs.nty, cs.ntz,
+                                           w.part_sum, w.st);
+                    else
+                        hipLaunchKernelGGL((cg_step_march_kernel<V, kPhiPair>), dim3(cs.blocks()), dim3(64, kCgBY), 0,
+                                           s, w.phi, r, p_new, p_old, g, qx, qy, qz, cs.xchunk, cs.nty, cs.ntz,
+                                           w.part_sum, w.st);
+                } else {
+                    hipLaunchKernelGGL(cg_dir_kernel<V>, dim3(nb), blk, 0, s, r, p_old, p_new, g, qx, qy, qz,
+                                       w.part_sum, w.st);
+                    hipLaunchKernelGGL(cg_step_kernel<V>, dim3(nb), blk, 0, s, w.phi, r, p_new, g, qx, qy, qz,
+                                       w.part_sum, w.st);
+                }
             }
-            V* t = p_old;
-            p_old = p_new;
-            p_new = t;
+            SW_LAUNCH_CHECK();
+            launched += k;
+            SW_TRY_HIP(hipMemcpyAsync(&hs, w.st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+            SW_TRY_HIP(hipStreamSynchronize(s));
+            if (chunk < 64) chunk *= 2;
         }
+        // hs is final here: kernels launched after the stop returned at once
+        if (march && ((hs.iters - base) & 1)) {
+            hipLaunchKernelGGL(cg_flush_march_kernel<V>, dim3(cs.blocks()), dim3(64, kCgBY), 0, s, w.phi,
+                               (const V*)((hs.iters & 1) ? p_odd : p_even), g, cs.xchunk, cs.nty, cs.ntz, w.st);
+            SW_LAUNCH_CHECK();
+        }
+        base = hs.iters;
+        if (hs.reason == SW_CG_EARLY_EXIT) break;  // phi = 0 solves the system exactly
+        const bool replace = replacing && hs.reason == SW_CG_CONVERGED && round < kMaxReplacements;
+        double next = stage * kStageFactor;
+        if (next < tol) next = tol;
+        hipLaunchKernelGGL((cg_true_residual_kernel<T, V>), dim3(nb), blk, 0, s, fx, fy, fz, (const double*)w.phi, r,
+                           g, i2hx, i2hy, i2hz, qx, qy, qz, w.part_sum, w.st, replace ? 1 : 0, next);
         SW_LAUNCH_CHECK();
-        launched += k;
         SW_TRY_HIP(hipMemcpyAsync(&hs, w.st, sizeof(CgState), hipMemcpyDeviceToHost, s));
         SW_TRY_HIP(hipStreamSynchronize(s));
-        if (chunk < 64) chunk *= 2;
-    }
-    if (march) {
-        hipLaunchKernelGGL(cg_flush_march_kernel<V>, dim3(cs.blocks()), dim3(64, kCgBY), 0, s, w.phi, (const V*)w.p1,
-                           g, cs.xchunk, cs.nty, cs.ntz, w.st);
-        SW_LAUNCH_CHECK();
+        if (!replace || hs.stop) break;
+        stage = next;
     }
     return SW_OK;
 }
@@ -411,23 +491,31 @@ int sw_project_vector_field(int dtype, int vec_dtype, void* fx, void* fy, void* 
     const double i2hx = 1.0 / (2.0 * dx), i2hy = 1.0 / (2.0 * dy), i2hz = 1.0 / (2.0 * dz);
     const double qx = 1.0 / (4.0 * dx * dx), qy = 1.0 / (4.0 * dy * dy), qz = 1.0 / (4.0 * dz * dz);
 
-    hipLaunchKernelGGL(cg_reset_kernel, dim3(1), dim3(1), 0, s, w.st, tol, (long long)max_iters);
+    // float32 vectors first stop at a looser stage level for a residual replacement (run_cg)
+    const double stage = (vec_dtype == SW_F32 && tol < kStageFactor) ? kStageFactor : tol;
+    hipLaunchKernelGGL(cg_reset_kernel, dim3(1), dim3(1), 0, s, w.st, tol, (long long)max_iters, stage);
     SW_LAUNCH_CHECK();
     CgState hs;
     if (vec_dtype == SW_F32) {
-        SW_DISPATCH(dtype, T,
-                    hipLaunchKernelGGL((cg_init_kernel<T, float>), dim3(nb), blk, 0, s, (const T*)fx, (const T*)fy,
-                                       (const T*)fz, w.phi, (float*)w.r, (float*)w.p0, g, i2hx, i2hy, i2hz,
-                                       w.part_sum, w.part_max, w.st));
-        SW_LAUNCH_CHECK();
-        if ((rc = run_cg<float>(w, g, march, cs, nb, qx, qy, qz, max_iters, hs, s))) return rc;
+        SW_DISPATCH(dtype, T, {
+            hipLaunchKernelGGL((cg_init_kernel<T, float>), dim3(nb), blk, 0, s, (const T*)fx, (const T*)fy,
+                               (const T*)fz, w.phi, (float*)w.r, (float*)w.p0, g, i2hx, i2hy, i2hz, w.part_sum,
+                               w.part_max, w.st);
+            SW_LAUNCH_CHECK();
+            if ((rc = run_cg<T, float>(w, g, march, cs, nb, (const T*)fx, (const T*)fy, (const T*)fz, i2hx, i2hy,
+                                       i2hz, qx, qy, qz, tol, max_iters, stage, hs, s)))
+                return rc;
+        });
     } else {
-        SW_DISPATCH(dtype, T,
-                    hipLaunchKernelGGL((cg_init_kernel<T, double>), dim3(nb), blk, 0, s, (const T*)fx, (const T*)fy,
-                                       (const T*)fz, w.phi, (double*)w.r, (double*)w.p0, g, i2hx, i2hy, i2hz,
-                                       w.part_sum, w.part_max, w.st));
-        SW_LAUNCH_CHECK();
-        if ((rc = run_cg<double>(w, g, march, cs, nb, qx, qy, qz, max_iters, hs, s))) return rc;
+        SW_DISPATCH(dtype, T, {
+            hipLaunchKernelGGL((cg_init_kernel<T, double>), dim3(nb), blk, 0, s, (const T*)fx, (const T*)fy,
+                               (const T*)fz, w.phi, (double*)w.r, (double*)w.p0, g, i2hx, i2hy, i2hz, w.part_sum,
+                               w.part_max, w.st);
+            SW_LAUNCH_CHECK();
+            if ((rc = run_cg<T, double>(w, g, march, cs, nb, (const T*)fx, (const T*)fy, (const T*)fz, i2hx, i2hy,
+                                        i2hz, qx, qy, qz, tol, max_iters, stage, hs, s)))
+                return rc;
+        });
     }
 
     SW_DISPATCH(dtype, T,
@@ -442,6 +530,7 @@ int sw_project_vector_field(int dtype, int vec_dtype, void* fx, void* fy, void* 
         info->max_div = hs.max_div;
         info->reason = hs.stop ? hs.reason : SW_CG_MAX_ITERS;
         info->pad = 0;
+        info->true_relres = hs.reason == SW_CG_EARLY_EXIT ? 0.0 : hs.true_relres;
     }
     return SW_OK;
 }

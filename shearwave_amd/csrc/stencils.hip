@@ -147,11 +147,16 @@ __global__ void __launch_bounds__(256) combine3_kernel(const T* __restrict__ a0,
     }
 }
 
+// out[k] = f[idx[k]]; an index outside [0, n) reads nothing and yields NaN.
 template <typename T>
-__global__ void __launch_bounds__(256) gather_kernel(const T* __restrict__ f, const int64_t* __restrict__ idx,
-                                                     int64_t count, T* __restrict__ out, int64_t stride) {
+__global__ void __launch_bounds__(256) gather_kernel(const T* __restrict__ f, int64_t n,
+                                                     const int64_t* __restrict__ idx, int64_t count,
+                                                     T* __restrict__ out, int64_t stride) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < count) out[k * stride] = f[idx[k]];
+    if (k < count) {
+        const int64_t j = idx[k];
+        out[k * stride] = (j >= 0 && j < n) ? f[j] : (T)NAN;
+    }
 }
 
 }  // namespace
@@ -294,16 +299,16 @@ int sw_combine3(int dtype, const void* const* a, const void* const* b, double al
     return SW_OK;
 }
 
-int sw_gather_traces(int dtype, const void* f, const int64_t* idx, int64_t count, void* out, int64_t out_stride,
-                     void* stream) {
+int sw_gather_traces(int dtype, const void* f, int64_t n, const int64_t* idx, int64_t count, void* out,
+                     int64_t out_stride, void* stream) {
     int rc = check_dtype(dtype);
     if (rc) return rc;
-    SW_REQUIRE(count >= 0 && out_stride >= 1, SW_ERR_ARG, "bad count/stride");
+    SW_REQUIRE(count >= 0 && out_stride >= 1 && n >= 0, SW_ERR_ARG, "bad count/stride/n");
     if (count == 0) return SW_OK;
     SW_REQUIRE(f && idx && out, SW_ERR_ARG, "null pointer");
     SW_DISPATCH(dtype, T,
                 hipLaunchKernelGGL(gather_kernel<T>, dim3((unsigned)((count + 255) / 256)), dim3(256), 0,
-                                   (hipStream_t)stream, (const T*)f, idx, count, (T*)out, out_stride));
+                                   (hipStream_t)stream, (const T*)f, n, idx, count, (T*)out, out_stride));
     SW_LAUNCH_CHECK();
     return SW_OK;
 }

@@ -95,8 +95,13 @@ _WORKSPACES: dict = {}
 
 
 def _workspace(kind, shape, nbytes_fn) -> torch.Tensor:
-    """Cached device workspace of one kind for a grid shape (one per device and shape)."""
-    key = (kind, torch.cuda.current_device(), tuple(shape))
+    """Cached device workspace of one kind for a grid shape.
+
+    One per device, shape and stream: calls on one stream are ordered, so a
+    workspace (which also holds the CG state the host polls) is never used by
+    two calls at once; calls on different streams get different workspaces.
+    """
+    key = (kind, torch.cuda.current_device(), tuple(shape), stream_ptr())
     ws = _WORKSPACES.get(key)
     if ws is None:
         nbytes = int(nbytes_fn(*shape))
@@ -124,7 +129,9 @@ def project_inplace(f, spacing, tol, max_iters, *, scale_out=None, inv_rho=None,
     ``vectors`` is the storage of the CG vectors r and p (``"float32"`` or
     ``"float64"``; default: f's dtype); phi, the dot products and the residual
     norm are float64 either way. Returns the CG report: iters, relres,
-    max_div, reason.
+    max_div, reason and true_relres (|b - A phi| / |b| of the returned phi,
+    b formed in float64; with float32 vectors the CG replaces its recursive
+    residual by it until it meets tol).
     """
     shape = tuple(f.shape[1:])
     ws = project_workspace(shape)
@@ -143,6 +150,7 @@ def project_inplace(f, spacing, tol, max_iters, *, scale_out=None, inv_rho=None,
         "relres": float(info.relres),
         "max_div": float(info.max_div),
         "reason": _abi.CG_REASONS.get(int(info.reason), str(info.reason)),
+        "true_relres": float(info.true_relres),
     }
 
 
@@ -187,6 +195,8 @@ class Solve:
                 self._by_ptr[t[c].data_ptr()] = t[c]
         self.force = None
         self.step_count = 0
+        # first global step after which a non-finite check found inf/NaN in u (-1: none)
+        self._nonfinite = torch.full((1,), -1, dtype=torch.int64, device=dev)
         s = _abi.FdtdState()
         s.dtype = self.code
         s.form = _FORMS[form]
@@ -239,8 +249,13 @@ class Solve:
 
     # ----------------------------------------------------------------- loop
     def run(self, n_steps, env=None, *, snap_stride=0, snap_out=None, inv_stride=0, inv_out=None,
-            trace_idx=None, trace_out=None):
-        """Advance n_steps steps; env (host, length n_steps) scales the force per step."""
+            trace_idx=None, trace_out=None, finite_stride=0):
+        """Advance n_steps steps; env (host, length n_steps) scales the force per step.
+
+        With ``finite_stride`` > 0, u is checked for inf/NaN every that many
+        global steps (after the pass that reaches the step), without
+        synchronising; :meth:`first_nonfinite_step` reads the result.
+        """
         n_steps = int(n_steps)
         if n_steps <= 0:
             return
@@ -261,8 +276,17 @@ class Solve:
             out.n_traces = trace_idx.numel()
             out.trace_out = trace_out.data_ptr()
             out.trace_ld = trace_out.shape[1]
+        if finite_stride:
+            out.finite_stride = int(finite_stride)
+            out.nonfinite_step = self._nonfinite.data_ptr()
         check(lib().sw_fdtd_run(C.byref(self.state), n_steps, env_arr, C.byref(out), stream_ptr()))
         self.step_count += n_steps
+
+    def first_nonfinite_step(self):
+        """The global step after which a check of :meth:`run` first found inf/NaN
+        in u, or None (synchronises with the stream)."""
+        k = int(self._nonfinite.item())
+        return None if k < 0 else k
 
     def step_into(self, u_src, d_src, u_dst, d_dst, env_value=None):
         """One step from caller buffers (u_src, d_src) into (u_dst, d_dst), each
@@ -322,23 +346,37 @@ class Solve:
         return out
 
     def reproject(self, tol, max_iters):
-        """Project u and u_prev to divergence-free (solver.py:236-239)."""
+        """Project u and u_prev to divergence-free (solver.py:236-239).
+
+        As the reference, u and u_prev are projected separately (each CG stops
+        on its own residual). In the increment form of float32 storage the
+        state is (u, d) with |d| << |u|, and d' = P(u) - P(u - d) cancels: so
+        u_prev = u - d (exact in float64), both projections (float64 fields
+        and CG vectors) and the difference all run in float64, and d' is
+        rounded to float32 once.
+        """
         u = self.u_components()
         b = self.b_components()
-        uvec = torch.stack(u)  # copies: the projection works in place
         if self.form == "increment":
-            prev = torch.empty_like(uvec)
-            # u_prev = u - d
-            check(lib().sw_combine3(self.code, ptrs(*u), ptrs(*b), 1.0, -1.0, ptrs(*prev), self.n, stream_ptr()))
-        else:
-            prev = torch.stack(b)
+            f64 = self.dtype == "float64"
+            u64 = torch.stack(u).to(torch.float64)  # copies: the projection works in place
+            b64 = torch.stack(b).to(torch.float64)
+            prev = torch.empty_like(u64)
+            check(lib().sw_combine3(_abi.SW_F64, ptrs(*u64), ptrs(*b64), 1.0, -1.0, ptrs(*prev), self.n,
+                                    stream_ptr()))
+            infos = [project_inplace(v, self.spacing, tol, max_iters, vectors=None if f64 else "float64")
+                     for v in (u64, prev)]
+            d64 = torch.empty_like(u64)
+            check(lib().sw_combine3(_abi.SW_F64, ptrs(*u64), ptrs(*prev), 1.0, -1.0, ptrs(*d64), self.n,
+                                    stream_ptr()))
+            for c in range(3):
+                u[c].copy_(u64[c])
+                b[c].copy_(d64[c])
+            return infos
+        uvec = torch.stack(u)
+        prev = torch.stack(b)
         infos = [project_inplace(v, self.spacing, tol, max_iters, vectors=self.cg_vectors) for v in (uvec, prev)]
         for c in range(3):
             u[c].copy_(uvec[c])
-        if self.form == "increment":
-            # d = u - u_prev
-            check(lib().sw_combine3(self.code, ptrs(*u), ptrs(*prev), 1.0, -1.0, ptrs(*b), self.n, stream_ptr()))
-        else:
-            for c in range(3):
-                b[c].copy_(prev[c])
+            b[c].copy_(prev[c])
         return infos

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
     missing = [s for s in declared_symbols() if not hasattr(lib, s)]
     assert not missing, missing
     lib.sw_abi_version.restype = ctypes.c_int
-    assert lib.sw_abi_version() == 3
+    assert lib.sw_abi_version() == 4
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="HIP library not built")
@@ -53,6 +53,30 @@ def test_python_binding_covers_header():
     rc = lib.sw_project_vector_field(_abi.SW_F32, _abi.SW_F32, None, None, None, 8, 8, 8, 1.0, 1.0, 1.0, 1e-8, 10,
                                      None, 0, None, None, 1.0, ctypes.byref(info), None)
     assert rc == 1 and b"null force" in lib.sw_last_error()
+    # the trace kernels take the field's size, so a bad index cannot reach outside it (checked in-kernel)
+    rc = lib.sw_gather_traces(_abi.SW_F32, None, -1, None, 4, None, 1, None)
+    assert rc == 1 and b"n" in lib.sw_last_error()
+    rc = lib.sw_inject_traces(_abi.SW_F32, None, -5, None, 4, None, 1, None)
+    assert rc == 1 and b"n" in lib.sw_last_error()
+    # a non-finite check needs its device flag
+    st = _abi.FdtdState()
+    st.dtype, st.form, st.nx, st.ny, st.nz = _abi.SW_F32, _abi.SW_FORM_INCREMENT, 8, 8, 8
+    st.dx = st.dy = st.dz = st.dt = 1.0
+    st.mu_over_rho = 1
+    for c in range(3):
+        st.u[c] = st.b[c] = st.u_next[c] = st.b_next[c] = 1
+    out = _abi.FdtdOutputs()
+    out.finite_stride = 10
+    rc = lib.sw_fdtd_run(ctypes.byref(st), 1, None, ctypes.byref(out), None)
+    assert rc == 1 and b"nonfinite_step" in lib.sw_last_error()
+
+
+def test_struct_layouts_match_header():
+    """The ctypes structures have the sizes the header's C layout gives them."""
+    from shearwave_amd import _abi
+
+    assert ctypes.sizeof(_abi.CgInfo) == 40  # int64, 2 double, 2 int32, double
+    assert ctypes.sizeof(_abi.FdtdOutputs) == 11 * 8
 
 
 def test_product_does_not_import_the_oracle():
