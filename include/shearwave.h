@@ -107,6 +107,20 @@ int sw_project_vector_field(int dtype, int vec_dtype, void* fx, void* fy, void* 
                             void* workspace, int64_t workspace_bytes, void* const* scale_out, const void* inv_rho,
                             double scale, sw_cg_info* info, void* stream);
 
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). sw_project_vector_field with the CG workspace taken from a
+ * static device array of the library instead of a caller's buffer, for grids
+ * of at most sw_project_noalloc_max_voxels() (2^20) voxels; larger grids are
+ * rejected with SW_ERR_ARG. It allocates nothing. Each GPU holds one such
+ * array (about 32 MiB) until the process ends, shared by every call on that
+ * GPU: calls that overlap on one GPU race. */
+int64_t sw_project_noalloc_max_voxels(void);
+int sw_project_vector_field_noalloc(int dtype, int vec_dtype, void* fx, void* fy, void* fz, int64_t nx, int64_t ny,
+                                    int64_t nz, double dx, double dy, double dz, double tol, int64_t max_iters,
+                                    void* const* scale_out, const void* inv_rho, double scale, sw_cg_info* info,
+                                    void* stream);
+
 /* ---------------------------------------------------------------- time loop */
 
 /* Material coefficients (solver.py:124-145, 1072-1079): from float64 rho, mu

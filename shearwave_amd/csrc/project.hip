@@ -23,6 +23,7 @@
 // stop flag once per chunk of iterations.
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
 #include <type_traits>
 
 #include "march_cg.h"
@@ -457,6 +458,46 @@ int run_cg(const Workspace& w, const Grid& g, bool march, const CgShape& cs, uns
 }  // namespace
 }  // namespace sw
 
+// ------------------------------------------------- no-allocation projection
+//
+// Scratch of sw_project_vector_field_noalloc: the CG workspace (phi, r, p,
+// p_old, the partial slots and the CG state) for grids whose workspace fits,
+// i.e. up to 2^20 voxels. Each GPU the library runs on holds its own copy
+// until the process ends. Calls on one GPU share it, so calls that overlap on
+// one GPU (different streams or threads) race. Nothing in it needs to start at
+// zero: every call resets the CG state and writes phi, r and p_old before any
+// read, and each slot is written before it is summed.
+constexpr int64_t kNoAllocScratchBytes = (int64_t(32) << 20) + (int64_t(64) << 10);
+constexpr int64_t kNoAllocMaxVoxels = int64_t(1) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace sw {
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices];  // device address of `scratch` per device ordinal (null until looked up)
+
+// The current device's address of `scratch`, looked up once per device. The
+// array's host-side name is not a device address and is never passed on.
+int noalloc_scratch(void** out) {
+    int dev = -1;
+    SW_TRY_HIP(hipGetDevice(&dev));
+    SW_REQUIRE(dev >= 0 && dev < kMaxDevices, SW_ERR_ARG, "device ordinal %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        SW_TRY_HIP(hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)));
+        SW_REQUIRE(p && ((uintptr_t)p & 255) == 0, SW_ERR_HIP, "scratch has no 256-byte aligned device address");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SW_OK;
+}
+
+}  // namespace
+}  // namespace sw
+
 using namespace sw;
 
 extern "C" {
@@ -538,6 +579,26 @@ int sw_project_vector_field(int dtype, int vec_dtype, void* fx, void* fy, void* 
         info->true_relres = hs.reason == SW_CG_EARLY_EXIT ? 0.0 : hs.true_relres;
     }
     return SW_OK;
+}
+
+int64_t sw_project_noalloc_max_voxels(void) { return kNoAllocMaxVoxels; }
+
+int sw_project_vector_field_noalloc(int dtype, int vec_dtype, void* fx, void* fy, void* fz, int64_t nx, int64_t ny,
+                                    int64_t nz, double dx, double dy, double dz, double tol, int64_t max_iters,
+                                    void* const* scale_out, const void* inv_rho, double scale, sw_cg_info* info,
+                                    void* stream) {
+    int rc = check_grid(nx, ny, nz);
+    if (rc) return rc;
+    const int64_t n = nx * ny * nz;
+    SW_REQUIRE(n <= kNoAllocMaxVoxels && workspace_bytes(nx, ny, nz) <= kNoAllocScratchBytes, SW_ERR_ARG,
+               "grid %lldx%lldx%lld (%lld voxels) exceeds the no-allocation projection's scratch (%lld voxels, "
+               "%lld bytes); use sw_project_vector_field with a workspace",
+               (long long)nx, (long long)ny, (long long)nz, (long long)n, (long long)kNoAllocMaxVoxels,
+               (long long)kNoAllocScratchBytes);
+    void* ws = nullptr;
+    if ((rc = noalloc_scratch(&ws))) return rc;
+    return sw_project_vector_field(dtype, vec_dtype, fx, fy, fz, nx, ny, nz, dx, dy, dz, tol, max_iters, ws,
+                                   kNoAllocScratchBytes, scale_out, inv_rho, scale, info, stream);
 }
 
 }  // extern "C"
