@@ -47,7 +47,8 @@ def parse():
     ap.add_argument("--dtype", default="float32", choices=["float32", "float64"])
     ap.add_argument("--no-fp64", action="store_true", help="skip the float64 solve")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=8,
+                    help="float32 oracle steps timed for cpu_baseline, after 2 warm-up steps (20 = the planned slice)")
     ap.add_argument("--kernel-steps", type=int, default=200)
     ap.add_argument("--kernel", default="auto")
     ap.add_argument("--tile-y", type=int, default=0)
@@ -56,7 +57,7 @@ def parse():
     ap.add_argument("--batch-solves", type=int, default=48)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
     ap.add_argument("--serve-rates", default="5,10,20,50,100,200")
-    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation run")
     ap.add_argument("--no-radforce", action="store_true", help="skip the radiation-force run")
     ap.add_argument("--no-static", action="store_true", help="skip the static elasticity solve run")
@@ -126,7 +127,10 @@ def time_step_kernel(solve, n_launch, steps_per_launch):
 
 
 def _oracle_steps(shape, h, m, e, dt, n_steps, seed=0):
-    """Seconds for `n_steps` force-free viscous steps of the float64 oracle loop (after one warm-up step)."""
+    """Seconds for `n_steps` force-free viscous steps of a float64 NumPy loop (after one warm-up step).
+
+    The per-process work of the CPU throughput estimate below (materials as scalars).
+    """
     from oracle import solver as O
 
     rng = np.random.default_rng(seed)
@@ -149,6 +153,30 @@ def _oracle_steps(shape, h, m, e, dt, n_steps, seed=0):
     for _ in range(n_steps):
         one()
     return time.perf_counter() - t0
+
+
+def oracle_loop_seconds(w, dtype, n_steps, warm):
+    """Seconds per step of the oracle's own time loop (oracle.solver.shear_fdtd) on the workload's grid and fields.
+
+    ``dtype`` float32 is the reference's arithmetic: the oracle's float32 mode
+    is bit-identical to solver.py (tests/golden/ref_fp32.npz). Force-free
+    steps: bz = 0 with env_t = 0 (the projection exits at once), a random
+    initial state so that every step does its full work. Timed from the
+    callback after step ``warm - 1`` to the one after step ``warm + n_steps -
+    1``: exactly ``n_steps`` whole loop iterations.
+    """
+    from oracle import solver as O
+
+    shape = tuple(w["shape"])
+    rng = np.random.default_rng(0)
+    u0 = (1e-9 * rng.standard_normal(shape + (3,))).astype(dtype)
+    v0 = (1e-6 * rng.standard_normal(shape + (3,))).astype(dtype)
+    total = warm + n_steps
+    opts = {"env_t": np.zeros(total), "eta": w["eta"], "initial_u": u0, "initial_v": v0}
+    stamps = []
+    O.shear_fdtd(None, None, np.zeros(shape, dtype=dtype), w["rho"], w["mu"], w["h"], w["h"], w["h"], w["dt"], total,
+                 opts=opts, callback=lambda it, u: stamps.append(time.perf_counter()), dtype=dtype)
+    return (stamps[-1] - stamps[warm - 1]) / n_steps
 
 
 def cpu_projection_sample(w, iters, full_iters):
@@ -175,28 +203,43 @@ def cpu_projection_sample(w, iters, full_iters):
             "extrapolated_s": round(t1 + per_iter * (full_iters - 1), 1), "full_iters": full_iters}
 
 
-def cpu_baseline(w, n_steps, proj_iters=0, full_iters=0):
-    """The float64 oracle on `n_steps` force-free steps of the same 256^3 loop (one core).
+def cpu_baseline(w, n_steps=8, warm=2, n_steps64=3, proj_iters=3, full_iters=0):
+    """The reference's CPU path on the same 256^3 workload, one core.
 
-    With proj_iters > 0 also a sample of the projection (setup), timed apart.
+    `value` is the oracle's float32 mode (the reference's own arithmetic) on a
+    slice of `n_steps` force-free steps after `warm` warm-up steps; the
+    float64 oracle is timed beside it on `n_steps64` steps; the projection
+    (float64 CG in both, as poisson_cg) is sampled at `proj_iters` CG
+    iterations and extrapolated to `full_iters`. NumPy ufuncs are
+    single-threaded here.
     """
-    shape = w["shape"]
-    t = _oracle_steps(shape, w["h"], w["mu"] / w["rho"], w["eta"] / w["rho"], w["dt"], n_steps)
-    vox = int(np.prod(shape))
+    vox = int(np.prod(w["shape"]))
+    t32 = oracle_loop_seconds(w, np.float32, n_steps, warm)
+    t64 = oracle_loop_seconds(w, np.float64, n_steps64, 1)
     proj = cpu_projection_sample(w, proj_iters, full_iters) if proj_iters > 0 else None
-    return {
-        "projection_sample": proj,
-        "value": vox * n_steps / t / 1e9,
+    n_total = w["n_steps"]
+    rec = {
+        "value": round(1.0 / t32 * vox / 1e9, 5),
         "unit": "Gvoxel-steps/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{n_steps} force-free viscous time steps of the {shape[0]}^3 loop in float64 (NumPy oracle "
-                  f"restating solver.py:194-234) after 1 warm-up step; {t:.1f} s; projection not included",
-        "seconds": t,
+        "sample": f"{n_steps} force-free viscous steps of the {w['shape'][0]}^3 loop after {warm} warm-up steps, "
+                  f"oracle.solver.shear_fdtd in float32 mode (bit-identical to solver.py's float32 arithmetic): "
+                  f"{t32:.3f} s per step; loop only, the projection is sampled apart",
+        "seconds_per_step": round(t32, 4),
+        "float64": {"gvox_steps_per_s": round(1.0 / t64 * vox / 1e9, 5), "seconds_per_step": round(t64, 4),
+                    "sample": f"{n_steps64} steps after 1 warm-up step, float64 oracle"},
+        "projection_sample": proj,
         "numpy": np.__version__,
         "host": platform.processor() or platform.machine(),
         "os_cpu_count": os.cpu_count(),
     }
+    if proj is not None:
+        solve_s = proj["extrapolated_s"] + n_total * t32
+        rec["solve_s_extrapolated"] = round(solve_s, 1)
+        rec["solve_note"] = (f"projection (extrapolated to {full_iters} CG iterations) + {n_total} float32 steps at "
+                             f"the sampled rate; in-loop strain not included")
+    return rec
 
 
 def _cpu_worker(args):
@@ -235,13 +278,13 @@ def bench_batch(n_solves=48, check=8):
     import shearwave_amd as sw
     from shearwave_amd import workloads as W
     from shearwave_amd.autotune import default_tuner
-    from shearwave_amd.batch import Batch
+    from shearwave_amd.batch import Batch, axis_order
 
     specs = W.batch_48(n_solves=n_solves)
     vox_steps = sum(sp.voxels * sp.n_steps for sp in specs)
     tuner = default_tuner()
-    for sp in specs:
-        tuner.xchunk(sp.shape)  # tune every grid class before timing
+    for sp in specs:  # tune every grid class (in its stored axis order) before timing
+        tuner.xchunk(tuple(sp.shape[i] for i in axis_order(sp.shape)))
 
     def plain(sp, dtype="float32"):
         opts = {"env_t": sp.env, "eta": sp.eta, "vel_damp": sp.vel_damp, "poisson_tol": sp.tol,
@@ -266,6 +309,7 @@ def bench_batch(n_solves=48, check=8):
     b.run()
     t_run = time.perf_counter() - t1
     batch_out = [b.result(k) for k in range(len(specs))]
+    b_perm, graph_steps = [s.perm for s in b.solves], b.graph_steps
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
 
@@ -288,6 +332,8 @@ def bench_batch(n_solves=48, check=8):
         "plain_s": round(t_plain, 3),
         "batched_phases_s": {"upload_setup": round(t_setup, 3), "projection": round(t_proj, 3), "loop": round(t_run, 3)},
         "batched_loop_gvox_steps_per_s": round(vox_steps / t_run / 1e9, 2),
+        "solves_stored_permuted": sum(1 for s in b_perm if s != (0, 1, 2)),
+        "graph_steps": graph_steps,
         "cg_iters_max": max(i["iters"] for i in info),
         "max_rel_err_vs_fp64": max(errs) if errs else None,
         "checked_solves": len(errs),
@@ -295,39 +341,49 @@ def bench_batch(n_solves=48, check=8):
     }
 
 
-def bench_serve(rates=(20.0, 100.0), n_requests=24, tick_passes=10):
-    """Config 4: Poisson arrivals, continuous batching; solves/s and p50/p99 latency per rate."""
+def bench_serve(rates=(5.0, 10.0, 20.0, 50.0, 100.0, 200.0), n_requests=200, tick_passes=10):
+    """Config 4: Poisson arrivals, continuous batching; solves/s and p50/p99 latency per rate.
+
+    The same seeded pool of requests (grids U{64..256}^3, nT U{300..1500}) is
+    served at every rate, each rate with its own seeded Poisson arrivals.
+    """
     import torch
 
     import shearwave_amd as sw
     from shearwave_amd.autotune import default_tuner
-    from shearwave_amd.serve import Server, poisson_trace
+    from shearwave_amd.batch import axis_order
+    from shearwave_amd.serve import Server, request_specs, with_arrivals
 
     tuner = default_tuner()
-    out = {"workload": f"{n_requests} requests per rate, grids U{{64..256}}^3 (multiples of 8), nT U{{300..1500}}, "
-                       f"float32, seed 4, tick of {tick_passes} passes (one or two steps each)", "rates": []}
-    plain_rate = None
-    for rate in rates:
-        reqs = poisson_trace(rate, n_requests, seed=4)
-        for r in reqs:
-            tuner.xchunk(r.spec.shape)
+    t0 = time.perf_counter()
+    pool = request_specs(n_requests, seed=4)
+    t_gen = time.perf_counter() - t0
+    for sp in pool:  # tune every grid class (in its stored axis order) before timing
+        tuner.xchunk(tuple(sp.shape[i] for i in axis_order(sp.shape)))
+    out = {"workload": f"{n_requests} requests per rate from one seeded pool (seed 4): grids U{{64..256}}^3 "
+                       f"(multiples of 8), nT U{{300..1500}}, float32, CG tol 1e-8; Poisson arrivals seeded per rate; "
+                       f"ticks of {tick_passes} passes (one or two steps each), 3 ticks in flight, hipGraph replay",
+           "pool_generation_s": round(t_gen, 1), "rates": []}
+    for k, rate in enumerate(rates):
+        reqs = with_arrivals(pool, rate, seed=100 + k)
         torch.cuda.synchronize()
         srv = Server(tick_passes=tick_passes, tuner=tuner)
         done, summ = srv.run(reqs)
-        summ = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in summ.items()}
+        summ = {kk: (round(v, 4) if isinstance(v, float) else v) for kk, v in summ.items()}
         summ["rate_req_per_s"] = rate
         out["rates"].append(summ)
+        for r in done:
+            r.u = r.v = None
         del srv, done
         torch.cuda.empty_cache()
-        if plain_rate is None:
-            # the plain drop-in on the same requests, one after another (arrivals ignored: best case)
-            t0 = time.perf_counter()
-            for r in reqs:
-                sp = r.spec
-                opts = {"env_t": sp.env, "eta": sp.eta, "poisson_tol": sp.tol, "poisson_max_iters": sp.max_iters}
-                sw.shear_fdtd(None, None, sp.force[2], sp.rho, sp.mu, *sp.h, sp.dt, sp.n_steps, opts)
-            torch.cuda.synchronize()
-            plain_rate = len(reqs) / (time.perf_counter() - t0)
+    # the plain drop-in on the same requests, one after another (arrivals ignored: its best case)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for sp in pool:
+        opts = {"env_t": sp.env, "eta": sp.eta, "poisson_tol": sp.tol, "poisson_max_iters": sp.max_iters}
+        sw.shear_fdtd(None, None, sp.force[2], sp.rho, sp.mu, *sp.h, sp.dt, sp.n_steps, opts)
+    torch.cuda.synchronize()
+    plain_rate = len(pool) / (time.perf_counter() - t0)
     out["plain_dropin_solves_per_s"] = round(plain_rate, 3)
     best = max(r["solves_per_s"] for r in out["rates"])
     out["best_served_over_plain"] = round(best / plain_rate, 3)
@@ -637,6 +693,35 @@ def bench_adjoint(n=128, n_steps=800, n_fd=32):
     return res
 
 
+def dropin_host_to_host(w, dtype="float32"):
+    """The same 256^3 solve through the drop-in API with host arrays in and out (PCIe included).
+
+    shearwave_amd.shear_fdtd uploads rho, mu, eta (float64) and bz, projects,
+    runs the loop and returns u and v as NumPy arrays (pinned downloads). No
+    in-loop strain (its 100 evaluations would be 27 GB of host output). One
+    warm-up call, then one timed call.
+    """
+    import torch
+
+    import shearwave_amd as sw
+
+    h = w["h"]
+    opts = {"env_t": w["env"], "eta": w["eta"], "poisson_tol": w["tol"], "poisson_max_iters": w["max_iters"],
+            "sw_dtype": dtype}
+    args = (None, None, w["bz"], w["rho"], w["mu"], h, h, h, w["dt"], w["n_steps"])
+    sw.shear_fdtd(*args, opts)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    u, v = sw.shear_fdtd(*args, opts)
+    t = time.perf_counter() - t0
+    vox = int(np.prod(w["shape"]))
+    item = 4 if dtype == "float32" else 8
+    return {"seconds": round(t, 4), "gvox_steps_per_s": round(vox * w["n_steps"] / t / 1e9, 3),
+            "upload_bytes": 3 * 8 * vox + item * vox, "download_bytes": 2 * 3 * item * vox,
+            "note": "host NumPy arrays in, host arrays out (u, v); projection and the 2000-step loop on the GPU; "
+                    "no in-loop strain"}
+
+
 def config1_cpu_vs_gpu():
     """Config 1 (the reference's CPU case, 64^3, nT=500) on CPU and GPU: the CPU-baseline leg for it.
 
@@ -779,6 +864,7 @@ def main():
             "projection_s": round(t_proj, 4),
             "cg_iters": bench.last_cg["iters"],
             "cg_relres": bench.last_cg["relres"],
+            "cg_true_relres": bench.last_cg["true_relres"],
             "loop_gvox_steps_per_s": round(spl * vox / kt / 1e9, 2),
         },
     }
@@ -824,10 +910,14 @@ def main():
         rec["adjoint"] = bench_adjoint()
     if not args.no_radforce:
         rec["radiation_force"] = bench_radforce()
+    if not args.no_cpu:
+        rec["dropin_host_to_host"] = dropin_host_to_host(w, args.dtype)
+        rec["config1"] = config1_cpu_vs_gpu()
     if not args.no_static:
         rec["static_solve"] = bench_static()
     if not args.no_cpu:
-        rec["cpu_baseline"] = cpu_baseline(w, args.cpu_steps, proj_iters=3, full_iters=int(rec["phases"]["cg_iters"]))
+        rec["cpu_baseline"] = cpu_baseline(w, args.cpu_steps, warm=2, n_steps64=3, proj_iters=3,
+                                           full_iters=int(rec["phases"]["cg_iters"]))
     else:
         rec["cpu_baseline"] = None
     print(json.dumps(rec), flush=True)

@@ -244,7 +244,11 @@ typedef struct sw_batch_solve {
 
 /* Advance the batch by n_passes global passes: per pass one persistent launch
  * over all `items` (tile_y = 8; blocks = 0 picks one resident wave) and one
- * increment of *gpass (device int64). */
+ * increment of gpass[0]. gpass is device memory of SW_BATCH_PASS_STATE int64:
+ * gpass[0] the global pass, gpass[1..2] the launch's work-claim counters,
+ * which the caller zeroes once and every launch leaves at zero. Blocks claim
+ * items dynamically in list order, so list the longest items first. */
+#define SW_BATCH_PASS_STATE 4
 int sw_batch_step(const sw_batch_solve* solves, const int32_t* items, int64_t n_items, int64_t* gpass, int32_t tile_y,
                   int32_t blocks, int64_t n_passes, void* stream);
 
@@ -272,8 +276,8 @@ typedef struct sw_batch_cg {
     float *r, *p0, *p1;              /* float32 CG vectors, nx*ny*nz each (dot products stay float64) */
     double* part;                    /* 2 * n_slots float64 partial slots, n_slots =
                                         ceil(nx/48) ceil(ny/16) ceil(nz/64) */
-    int32_t item0, n_items;          /* this solve's range in the CG item list (the caller's
-                                        bookkeeping; the kernels do not read it) */
+    int32_t item0, n_items;          /* the caller's bookkeeping of this solve's items (the
+                                        kernels do not read it) */
 } sw_batch_cg;
 
 /* Bytes of device CG state per solve. */
@@ -286,8 +290,10 @@ int64_t sw_cg_state_bytes(void);
  * solves of the batch. The descriptors and items are checked first (one
  * small device-to-host copy): SW_ERR_ARG unless every tile of every solve is
  * listed exactly once and each xchunk is valid. `states` is device
- * memory of n_solves * sw_cg_state_bytes() bytes. Runs until every solve has
- * stopped; info (host, n_solves entries) receives each solve's report. */
+ * memory of (n_solves + 1) * sw_cg_state_bytes() bytes: a CG state per solve,
+ * then the kernels' work-claim counters (blocks claim items in list order, so
+ * list the longest first). Runs until every solve has stopped; info (host,
+ * n_solves entries) receives each solve's report. */
 int sw_batch_project(const sw_batch_cg* solves, int32_t n_solves, const int32_t* items, int64_t n_items,
                      void* states, sw_cg_info* info, void* stream);
 

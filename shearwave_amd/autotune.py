@@ -2,12 +2,15 @@
 
 The batched step kernel marches x-chunks of a (8 x 64) tile of (y, z)
 columns; the chunk depth trades the chunk's prologue and halo re-reads
-(three planes per chunk for a two-step pass) against the number of work
-items available to balance the persistent grid. The tuner
-times the step kernel on a synthetic solve of each grid class with every
-candidate chunk depth, keeps the fastest, and caches the choice per device
-architecture in a JSON file (``$SW_AUTOTUNE_CACHE`` or
-``~/.cache/shearwave_amd/autotune.json``).
+(three planes per chunk for a two-step pass) and the per-item fixed cost
+against the number of work items available to balance the persistent grid.
+The tuner times the step kernel on a batch of copies of a grid class (in its
+stored axis order) with every candidate chunk depth, keeps the fastest, and
+caches the choice per device architecture in a JSON file
+(``$SW_AUTOTUNE_CACHE`` or ``~/.cache/shearwave_amd/autotune.json``). The
+tile's orientation (which grid axis is stored fastest, see
+``batch.axis_order``) is fixed by the lanes it fills, and a batch with too
+few items for the GPU shortens its chunks further (``batch.fill_xchunks``).
 """
 
 from __future__ import annotations
@@ -17,7 +20,7 @@ import math
 import os
 import threading
 
-CANDIDATES = (16, 32, 48, 64, 96, 128)
+CANDIDATES = (48, 64, 96, 128, 192, 256)
 
 
 def _pow2(v):
@@ -91,9 +94,7 @@ class Tuner:
         cls = grid_class(shape)
         rep = (min(cls[0], 512), 8 * cls[1], 64)  # same x length and tiles per plane
         times = {}
-        for xc in CANDIDATES:
-            if xc > rep[0]:
-                continue
+        for xc in [c for c in CANDIDATES if c <= rep[0]] or [rep[0]]:  # a short grid: whole columns
             times[xc] = float(timer(rep, xc))
         best = min(times, key=times.get)
         return {"xchunk": best, "shape": list(rep), "ps_per_voxel_step": {str(k): round(v * 1e12, 3) for k, v in times.items()}}
@@ -117,7 +118,7 @@ def _time_step_kernel(shape, xchunk, n_passes=6):
     n_steps = 2 * (n_passes + 3)  # force-free: every pass is two steps
     specs = [SolveSpec(shape=shape, h=h, dt=1e-6, rho=1000.0, mu=3000.0, eta=1.0, force=(None, None, None),
                        env=np.zeros(n_steps), n_steps=n_steps, xchunk=xchunk) for _ in range(copies)]
-    b = Batch(specs)
+    b = Batch(specs, axes="given")  # `shape` is already a stored axis order
     desc = b._desc_array(b.solves)
     items = step_items(b.solves)
     items_t = torch.from_numpy(items.reshape(-1).copy()).to(desc.device)

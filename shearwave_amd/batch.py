@@ -16,6 +16,7 @@ depend on the other solves in the batch.
 from __future__ import annotations
 
 import ctypes as C
+import itertools
 import math
 from dataclasses import dataclass, field
 
@@ -27,6 +28,7 @@ from . import engine as E
 from ._abi import check, lib
 
 STEP_BY = 8  # output rows of a batched step tile (blocks of 64 x 11 threads)
+PASS_STATE = 4  # int64 of the batch's pass state: the global pass and two work-claim counters
 MAX_PLANE = 1 << 28  # the kernels address a y-z plane with 32-bit offsets
 CG_BY = 16  # tile rows of the batched CG (fixed by the kernels)
 CG_SUB = 48  # x planes per partial sum of the batched CG (fixed by the kernels)
@@ -76,22 +78,78 @@ class SolveSpec:
         return int(np.prod(self.shape))
 
 
-class _Solve:
-    """Device buffers of one solve in a batch."""
+def lane_fill(shape, by=STEP_BY):
+    """Fraction of the step tiles' (by x 64) lanes that cover voxels of a (nx, ny, nz) grid."""
+    _, ny, nz = shape
+    return ny * nz / (math.ceil(nz / 64) * 64 * math.ceil(ny / by) * by)
 
-    def __init__(self, spec: SolveSpec, two_step=True):
+
+def axis_order(shape, by=STEP_BY):
+    """The order in which a solve's axes are stored in HBM: stored axis i is given axis order[i].
+
+    The stencil treats the three axes alike (each with its own spacing), so a
+    solve may be stored with any axis fastest. The tiles are 64 voxels wide
+    along the fastest axis and `by` rows along the middle one, so the order
+    that fills the most lanes is taken (ties: the given order, then the
+    longest marching axis). A 128 x 192 x 200 grid stored as given fills 78%
+    of its lanes, stored as (200, 128, 192) all of them.
+    """
+    shape = tuple(int(v) for v in shape)
+
+    def key(p):
+        s = tuple(shape[i] for i in p)
+        return (round(lane_fill(s, by), 9), p == (0, 1, 2), s[0])
+
+    return max(itertools.permutations(range(3)), key=key)
+
+
+def vec_to_given(v, perm):
+    """A (3, n0, n1, n2) SoA vector field stored in axis order `perm` back in the given order.
+
+    Stored axis i is given axis perm[i] and stored component i is the
+    component along it: stored = [given[perm[i]].permute(perm) for i]."""
+    if tuple(perm) == (0, 1, 2):
+        return v
+    inv = [int(i) for i in np.argsort(perm)]
+    return torch.stack([v[inv[m]].permute(*inv) for m in range(3)])
+
+
+def aos_to_given(a, perm):
+    """An (n0, n1, n2, 3) AoS vector field stored in axis order `perm` back in the given order."""
+    if tuple(perm) == (0, 1, 2):
+        return a
+    inv = [int(i) for i in np.argsort(perm)]
+    return a.permute(*inv, 3)[..., inv].contiguous()
+
+
+class _Solve:
+    """Device buffers of one solve in a batch, stored in the axis order `perm` (given axis perm[i] first)."""
+
+    def __init__(self, spec: SolveSpec, two_step=True, axes="auto"):
         dev = E.device()
         self.spec = spec
-        shape = tuple(int(s) for s in spec.shape)
-        if len(shape) != 3 or min(shape) < 1:
-            raise ValueError(f"solve shape {shape} must be three positive sizes")
+        given = tuple(int(s) for s in spec.shape)
+        if len(given) != 3 or min(given) < 1:
+            raise ValueError(f"solve shape {given} must be three positive sizes")
+        self.perm = axis_order(given) if axes == "auto" else (0, 1, 2)
+        self.inv = tuple(int(i) for i in np.argsort(self.perm))
+        shape = tuple(given[i] for i in self.perm)
         if shape[1] * shape[2] >= MAX_PLANE:
-            raise ValueError(f"solve shape {shape}: the y-z plane must hold fewer than {MAX_PLANE} voxels")
+            raise ValueError(f"solve shape {given}: the y-z plane must hold fewer than {MAX_PLANE} voxels")
+        self.given_shape = given
         self.shape = shape
+        self.h = tuple(float(spec.h[i]) for i in self.perm)
         st = E.stream_ptr()
-        rho64 = E.to_device(spec.rho, "float64", shape)
-        mu64 = E.to_device(spec.mu, "float64", shape)
-        eta64 = None if spec.eta is None else E.to_device(spec.eta, "float64", shape)
+
+        def up(x, dtype):
+            if np.isscalar(x):
+                return E.to_device(x, dtype, shape)
+            t = E.to_device(x, dtype, given)
+            return t if self.perm == (0, 1, 2) else t.permute(*self.perm).contiguous()
+
+        rho64 = up(spec.rho, "float64")
+        mu64 = up(spec.mu, "float64")
+        eta64 = None if spec.eta is None else up(spec.eta, "float64")
         self.m = torch.empty(shape, dtype=torch.float32, device=dev)
         self.e = None if eta64 is None else torch.empty(shape, dtype=torch.float32, device=dev)
         self.inv_rho = torch.empty(shape, dtype=torch.float32, device=dev)
@@ -103,11 +161,12 @@ class _Solve:
         self.u = torch.zeros((2, 3) + shape, dtype=torch.float32, device=dev)
         self.d = torch.zeros((2, 3) + shape, dtype=torch.float32, device=dev)
         self.f = torch.empty((3,) + shape, dtype=torch.float32, device=dev)
-        for c, comp in enumerate(spec.force):
+        for c in range(3):
+            comp = spec.force[self.perm[c]]  # the component along stored axis c
             if comp is None:
                 self.f[c].zero_()
             else:
-                self.f[c].copy_(E.to_device(comp, "float32", shape))
+                self.f[c].copy_(up(comp, "float32"))
         self.force = torch.empty_like(self.f)
         env = np.zeros(spec.n_steps) if spec.env is None else np.asarray(spec.env, dtype=np.float64)
         if env.shape[0] != spec.n_steps:
@@ -121,6 +180,21 @@ class _Solve:
     def footprint(self):
         n = int(np.prod(self.shape))
         return n * 4 * (3 + 12 + 6) + self.spec.n_steps * 8
+
+    def given_vec(self, v):
+        """A (3, ...) vector field of this solve's storage order in the given axis order."""
+        return vec_to_given(v, self.perm)
+
+    def export_aos(self, par):
+        """(u, v) of state parity `par` as AoS (nx, ny, nz, 3) device tensors in the given axis order."""
+        u, d = self.u[par], self.d[par]
+        aos = self.shape + (3,)
+        ua = torch.empty(aos, dtype=torch.float32, device=u.device)
+        va = torch.empty(aos, dtype=torch.float32, device=u.device)
+        check(lib().sw_export_state(_abi.SW_F32, _abi.SW_FORM_INCREMENT, _abi.ptrs(u[0], u[1], u[2]),
+                                    _abi.ptrs(d[0], d[1], d[2]), 1.0 / float(self.spec.dt), ua.data_ptr(),
+                                    va.data_ptr(), int(np.prod(self.shape)), E.stream_ptr()))
+        return aos_to_given(ua, self.perm), aos_to_given(va, self.perm)
 
 
 def pass_plan(n_steps, two_step=True):
@@ -152,17 +226,29 @@ def solve_bytes(spec: SolveSpec, with_cg=True) -> int:
     return b
 
 
+def item_cost(x_planes, z_width):
+    """Relative time of a step work item: its planes plus the two-step prologue (3), each costing a fixed
+    part and a part that grows with the lanes in use."""
+    return (x_planes + 3) * (32 + z_width)
+
+
 def step_items(solves, by=STEP_BY):
-    """Work items (solve, z tile, y tile, x chunk): z fastest, each solve's items contiguous."""
-    items = []
+    """Work items (solve, z tile, y tile, x chunk), longest first: the kernel's blocks claim them in list
+    order, so the short ones fill in at the end of a pass."""
+    items, cost = [], []
     for s_idx, s in enumerate(solves):
         nx, ny, nz = s.shape
         xc = s.xchunk
-        for tx in range(math.ceil(nx / xc)):
-            for ty in range(math.ceil(ny / by)):
-                for tz in range(math.ceil(nz / 64)):
-                    items.append((s_idx, tz, ty, tx))
-    return np.asarray(items, dtype=np.int32).reshape(-1, 4)
+        tx, ty, tz = np.meshgrid(np.arange(math.ceil(nx / xc)), np.arange(math.ceil(ny / by)),
+                                 np.arange(math.ceil(nz / 64)), indexing="ij")
+        tx, ty, tz = tx.ravel(), ty.ravel(), tz.ravel()
+        items.append(np.stack([np.full_like(tx, s_idx), tz, ty, tx], axis=1))
+        cost.append(item_cost(np.minimum(xc, nx - tx * xc), np.minimum(64, nz - 64 * tz)))
+    if not items:
+        return np.zeros((0, 4), dtype=np.int32)
+    items = np.concatenate(items).astype(np.int32)
+    order = np.argsort(-np.concatenate(cost).astype(np.int64), kind="stable")
+    return items[order]
 
 
 def default_xchunk(shape):
@@ -170,19 +256,52 @@ def default_xchunk(shape):
     return int(min(shape[0], 64))
 
 
+def n_step_items(solves, by=STEP_BY):
+    return sum(math.ceil(s.shape[0] / s.xchunk) * math.ceil(s.shape[1] / by) * math.ceil(s.shape[2] / 64)
+               for s in solves)
+
+
+def fill_xchunks(solves, n_blocks, per_block=2, floor=48):
+    """Shorten x-chunks until the work items give each of `n_blocks` persistent blocks `per_block` of them.
+
+    Long chunks pay fewer prologues (on the 48-solve workload whole columns
+    beat 64-plane chunks by 4%), but a set of few solves (a serving tick, the
+    tail of a batch) needs shorter items to occupy every CU: the longest
+    chunk is halved, down to `floor` planes, until there are enough.
+    """
+    while n_step_items(solves) < per_block * n_blocks:
+        s = max(solves, key=lambda s: min(s.xchunk, s.shape[0]), default=None)
+        if s is None or min(s.xchunk, s.shape[0]) <= floor:
+            return
+        s.xchunk = max(floor, min(s.xchunk, s.shape[0]) // 2)
+
+
+def n_compute_units():
+    return int(torch.cuda.get_device_properties(E.device()).multi_processor_count)
+
+
+def xchunk_for(s, tuner=None):
+    """x planes per step work item of a stored solve: its spec's choice, else the tuner's for its stored
+    shape, else the default."""
+    if s.spec.xchunk:
+        return int(s.spec.xchunk)
+    return tuner.xchunk(s.shape) if tuner else default_xchunk(s.shape)
+
+
 class Batch:
     """A set of solves advanced together on one GPU (float32 storage, increment form)."""
 
-    def __init__(self, specs, *, tile_y=STEP_BY, tuner=None, graph_steps=0, two_step=True):
+    def __init__(self, specs, *, tile_y=STEP_BY, tuner=None, graph_steps=16, two_step=True, axes="auto"):
         self.specs = list(specs)
         self.tile_y = int(tile_y)
-        self.graph_steps = int(graph_steps)  # passes per captured graph (0: no graph)
+        self.graph_steps = int(graph_steps)  # passes per captured graph, replayed (0: direct launches only)
         self.solves = []
         for sp in self.specs:
-            s = _Solve(sp, two_step)
-            s.xchunk = int(sp.xchunk) if sp.xchunk else (tuner.xchunk(sp.shape) if tuner else default_xchunk(sp.shape))
+            s = _Solve(sp, two_step, axes)
+            s.xchunk = s.base_xchunk = xchunk_for(s, tuner)
             self.solves.append(s)
-        self.gpass = torch.zeros(1, dtype=torch.int64, device=E.device())
+        # the global pass and the step kernel's work-claim counters (SW_BATCH_PASS_STATE int64)
+        self.gpass = torch.zeros(PASS_STATE, dtype=torch.int64, device=E.device())
         self.cg_info = None
 
     # --------------------------------------------------------- descriptors
@@ -193,7 +312,7 @@ class Batch:
             d = arr[k]
             d.nx, d.ny, d.nz = s.shape
             d.xchunk = s.xchunk
-            hx, hy, hz = (float(v) for v in sp.h)
+            hx, hy, hz = s.h
             dt = float(sp.dt)
             d.cx, d.cy, d.cz = dt * dt / (hx * hx), dt * dt / (hy * hy), dt * dt / (hz * hz)
             d.vx, d.vy, d.vz = dt / (hx * hx), dt / (hy * hy), dt / (hz * hz)
@@ -210,8 +329,7 @@ class Batch:
             d.env = s.env.data_ptr()
             d.plan = s.plan.data_ptr()
             d.n_passes = s.n_passes
-        raw = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8).to(E.device())
-        return raw
+        return E.upload_async(np.frombuffer(bytes(arr), dtype=np.uint8))
 
     # ---------------------------------------------------------- projection
     def project(self, solves=None, xchunk=None):
@@ -229,6 +347,7 @@ class Batch:
             xchunk = cg_xchunk([s.shape for s in solves], torch.cuda.get_device_properties(dev).multi_processor_count)
         elif xchunk < 1 or xchunk % CG_SUB:
             raise ValueError(f"xchunk must be a positive multiple of {CG_SUB}, got {xchunk}")
+        cost = []
         for k, s in enumerate(solves):
             sp = s.spec
             nx, ny, nz = s.shape
@@ -238,6 +357,7 @@ class Batch:
                 for ty in range(math.ceil(ny / CG_BY)):
                     for tz in range(math.ceil(nz / 64)):
                         items.append((k, tz, ty, tx))
+                        cost.append(item_cost(min(xc, nx - tx * xc), min(64, nz - 64 * tz)))
             phi = torch.empty(s.shape, dtype=torch.float64, device=dev)
             vec = torch.empty((3,) + s.shape, dtype=torch.float32, device=dev)  # r, p0, p1
             n_slots = math.ceil(nx / CG_SUB) * math.ceil(ny / CG_BY) * math.ceil(nz / 64)
@@ -246,7 +366,7 @@ class Batch:
             c = cgs[k]
             c.nx, c.ny, c.nz = s.shape
             c.xchunk = xc
-            c.dx, c.dy, c.dz = (float(v) for v in sp.h)
+            c.dx, c.dy, c.dz = s.h
             c.tol = float(sp.tol)
             c.max_iters = int(sp.max_iters)
             for j in range(3):
@@ -259,9 +379,12 @@ class Batch:
             c.part = part.data_ptr()
             c.item0 = n_items0
             c.n_items = len(items) - n_items0
-        items_t = torch.from_numpy(np.asarray(items, dtype=np.int32).reshape(-1)).to(dev)
+        # longest first: the dir and step kernels' blocks claim items in list order
+        order = np.argsort(-np.asarray(cost, dtype=np.int64), kind="stable")
+        items_t = torch.from_numpy(np.asarray(items, dtype=np.int32).reshape(-1, 4)[order].reshape(-1)).to(dev)
         desc = torch.frombuffer(bytearray(bytes(cgs)), dtype=torch.uint8).to(dev)
-        states = torch.empty(len(solves) * int(lib().sw_cg_state_bytes()), dtype=torch.uint8, device=dev)
+        # a CG state per solve, then the kernels' work-claim counters
+        states = torch.empty((len(solves) + 1) * int(lib().sw_cg_state_bytes()), dtype=torch.uint8, device=dev)
         info = (_abi.CgInfo * len(solves))()
         check(lib().sw_batch_project(desc.data_ptr(), len(solves), items_t.data_ptr(), len(items), states.data_ptr(),
                                      info, E.stream_ptr()))
@@ -272,7 +395,10 @@ class Batch:
         return self.cg_info
 
     # ---------------------------------------------------------------- loop
-    def _advance(self, desc, items_t, n_items, n_passes):
+    def _advance(self, desc, items_t, n_items, n_passes, keep=None):
+        """Queue n_passes global passes over the items: whole graphs of graph_steps passes, replayed, then
+        direct launches for the rest. A graph is destroyed after a synchronisation: at once, or, when a list
+        `keep` is given, by the caller (which also keeps desc and items alive until then)."""
         if n_passes <= 0:
             return
         if self.graph_steps and n_passes >= self.graph_steps:
@@ -285,8 +411,11 @@ class Batch:
                     check(lib().sw_graph_launch(g, E.stream_ptr()))
                 rest = n_passes - reps * self.graph_steps
             finally:
-                torch.cuda.current_stream().synchronize()
-                lib().sw_graph_destroy(g)
+                if keep is None:
+                    torch.cuda.current_stream().synchronize()
+                    lib().sw_graph_destroy(g)
+                else:
+                    keep.append((g, desc, items_t))
         else:
             rest = n_passes
         if rest:
@@ -294,41 +423,45 @@ class Batch:
                                       0, rest, E.stream_ptr()))
 
     def run(self):
-        """All solves start together; the item list shrinks as solves finish."""
+        """All solves start together; the item list shrinks as solves finish (x-chunks shorten when the
+        remaining solves are too few to occupy every CU with long ones)."""
         for s in self.solves:
             s.start = 0
         self.gpass.zero_()
-        desc = self._desc_array(self.solves)
         ends = sorted(set(s.n_passes for s in self.solves))
+        n_cu = n_compute_units()
         done = 0
+        graphs = []  # (graph, desc, items) of the segments, destroyed after the final synchronisation
         for end in ends:
             active = [k for k, s in enumerate(self.solves) if s.n_passes > done]
-            items = step_items([self.solves[k] for k in active], self.tile_y)
+            act = [self.solves[k] for k in active]
+            for s in act:
+                s.xchunk = s.base_xchunk
+            fill_xchunks(act, n_cu)
+            desc = self._desc_array(self.solves)
+            items = step_items(act, self.tile_y)
             # map local solve index back to the full descriptor array
             items[:, 0] = np.asarray(active, dtype=np.int32)[items[:, 0]]
-            items_t = torch.from_numpy(items.reshape(-1).copy()).to(E.device())
-            self._advance(desc, items_t, items.shape[0], end - done)
+            items_t = E.upload_async(items.reshape(-1))
+            self._advance(desc, items_t, items.shape[0], end - done, keep=graphs)
+            graphs.append((None, desc, items_t))  # direct launches of the segment read them too
             done = end
         torch.cuda.current_stream().synchronize()
+        for g, _, _ in graphs:
+            if g is not None:
+                lib().sw_graph_destroy(g)
 
     # ------------------------------------------------------------- results
     def state(self, k):
-        """(u, d) of solve k after its n_steps steps, as (3, nx, ny, nz) device tensors."""
+        """(u, d) of solve k after its n_steps steps, as (3, nx, ny, nz) device tensors (given axis order)."""
         s = self.solves[k]
         par = s.n_passes & 1
-        return s.u[par], s.d[par]
+        return s.given_vec(s.u[par]), s.given_vec(s.d[par])
 
     def result(self, k, to_host=True):
         """(u, v) of solve k in the reference's AoS (nx, ny, nz, 3) layout."""
         s = self.solves[k]
-        u, d = self.state(k)
-        n = int(np.prod(s.shape))
-        aos = s.shape + (3,)
-        ua = torch.empty(aos, dtype=torch.float32, device=u.device)
-        va = torch.empty(aos, dtype=torch.float32, device=u.device)
-        check(lib().sw_export_state(_abi.SW_F32, _abi.SW_FORM_INCREMENT, _abi.ptrs(u[0], u[1], u[2]),
-                                    _abi.ptrs(d[0], d[1], d[2]), 1.0 / float(s.spec.dt), ua.data_ptr(), va.data_ptr(),
-                                    n, E.stream_ptr()))
+        ua, va = s.export_aos(s.n_passes & 1)
         if to_host:
             return E.to_host(ua), E.to_host(va)
         return ua, va

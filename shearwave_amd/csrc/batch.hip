@@ -44,6 +44,34 @@ __device__ inline ItemWalk item_walk(int64_t n_items) {
     return w;
 }
 
+// Dynamic item claims: each block takes the next item with one atomic on
+// claim[0] and runs body(k) until the list is exhausted, so blocks that drew
+// short items take more of them and a launch ends when its work does (the host
+// lists the items longest first). The last block to finish resets claim[0]
+// and claim[1] to zero for the next launch, so no memset is needed (and none is
+// recorded into a graph). body must be uniform over the block.
+template <typename F>
+__device__ inline void claim_items(unsigned long long* claim, int64_t n_items, F&& body) {
+    __shared__ long long claimed;
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+    for (;;) {
+        if (tid == 0) claimed = (long long)atomicAdd(claim, 1ull);
+        __syncthreads();  // the claim is visible; the previous item's LDS use is done
+        const long long k = claimed;
+        __syncthreads();  // every thread has read the claim before thread 0 makes the next one
+        if (k >= n_items) break;
+        body(k);
+    }
+    if (tid == 0) {
+        __threadfence();
+        if (atomicAdd(claim + 1, 1ull) == (unsigned long long)gridDim.x - 1ull) {
+            claim[0] = 0ull;  // every block has made its last claim
+            claim[1] = 0ull;
+            __threadfence();
+        }
+    }
+}
+
 // Tile rows of the batched step: 8 output rows; the block has 8 + 3 rows of
 // threads so that a two-step pass (march2_step.h) fits, and single steps leave
 // the three extra rows to the barriers.
@@ -57,17 +85,17 @@ union BatchLds {
     float one[2][6][kBatchBY + 2][66];
 };
 
+// pass_state[0] is the global pass; items are claimed on pass_state[1..2].
 __global__ void __launch_bounds__(64 * kBatchRows) batch_step_kernel(const sw_batch_solve* __restrict__ solves,
                                                                      const int4* __restrict__ items, int64_t n_items,
-                                                                     const long long* __restrict__ gpass) {
+                                                                     long long* __restrict__ pass_state) {
     __shared__ BatchLds lds;
-    const long long gp = *gpass;
-    const ItemWalk w = item_walk(n_items);
-    for (int64_t k = w.first; k < w.hi; k += w.stride) {
+    const long long gp = pass_state[0];
+    claim_items((unsigned long long*)(pass_state + 1), n_items, [&](long long k) {
         const int4 it = items[k];
         const sw_batch_solve& S = solves[it.x];
         const long long pass = gp - S.start;
-        if (pass < 0 || pass >= S.n_passes) continue;  // uniform in the block
+        if (pass < 0 || pass >= S.n_passes) return;  // uniform in the block
         long long local = pass;
         bool two = false;
         if (S.plan) {
@@ -121,17 +149,25 @@ __global__ void __launch_bounds__(64 * kBatchRows) batch_step_kernel(const sw_ba
             else
                 march_tile<float, true, false, false, B, 64, 6, R>(a, x0, x1, y0, z0, lds.one);
         }
-        __syncthreads();  // the next item reuses the LDS
-    }
+    });
 }
 
 __global__ void advance_kernel(long long* counter) { *counter += 1; }
 
 // ------------------------------------------------------------ batched CG
 
+// states[n] holds the dir and step kernels' work-claim counters.
 __global__ void batch_cg_reset_kernel(const sw_batch_cg* solves, int n, CgState* states) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < n) cg_reset(states + s, solves[s].tol, (long long)solves[s].max_iters, solves[s].tol);
+    if (s == n) {
+        unsigned long long* claim = (unsigned long long*)(states + n);
+        claim[0] = claim[1] = 0ull;
+    }
+}
+
+__device__ inline unsigned long long* cg_claims(CgState* states, int n_solves) {
+    return (unsigned long long*)(states + n_solves);
 }
 
 __device__ inline CgTile batch_cg_tile(const sw_batch_cg& S, int4 it) {
@@ -194,15 +230,15 @@ __global__ void __launch_bounds__(64 * kCgBY) batch_cg_init_kernel(const sw_batc
 // p ping-pongs between p0 and p1 by the solve's iteration parity. The batch
 // keeps r and p in float32 (phi, dot products and residual norm in float64).
 __global__ void __launch_bounds__(64 * kCgBY) batch_cg_dir_kernel(const sw_batch_cg* __restrict__ solves, CgState* states,
-                                                                  const int4* __restrict__ items, int64_t n_items) {
+                                                                  int n_solves, const int4* __restrict__ items,
+                                                                  int64_t n_items) {
     __shared__ CgLdsT<float> lds;
     __shared__ double red[16];
-    const ItemWalk w = item_walk(n_items);
-    for (int64_t k = w.first; k < w.hi; k += w.stride) {
+    claim_items(cg_claims(states, n_solves), n_items, [&](long long k) {
         const int4 it = items[k];
         const sw_batch_cg& S = solves[it.x];
         CgState* st = states + it.x;
-        if (st->stop) continue;
+        if (st->stop) return;
         const Grid g{S.nx, S.ny, S.nz};
         const CgTile t = batch_cg_tile(S, it);
         const bool odd = st->iters & 1;
@@ -224,20 +260,19 @@ __global__ void __launch_bounds__(64 * kCgBY) batch_cg_dir_kernel(const sw_batch
             reduce_slots(S.part, nullptr, n_slots, red, &denom, &unused);
             if (threadIdx.x == 0 && threadIdx.y == 0) cg_after_dir(st, denom);
         }
-        __syncthreads();
-    }
+    });
 }
 
 __global__ void __launch_bounds__(64 * kCgBY) batch_cg_step_kernel(const sw_batch_cg* __restrict__ solves, CgState* states,
-                                                                   const int4* __restrict__ items, int64_t n_items) {
+                                                                   int n_solves, const int4* __restrict__ items,
+                                                                   int64_t n_items) {
     __shared__ CgLdsT<float> lds;
     __shared__ double red[16];
-    const ItemWalk w = item_walk(n_items);
-    for (int64_t k = w.first; k < w.hi; k += w.stride) {
+    claim_items(cg_claims(states, n_solves), n_items, [&](long long k) {
         const int4 it = items[k];
         const sw_batch_cg& S = solves[it.x];
         CgState* st = states + it.x;
-        if (st->stop) continue;
+        if (st->stop) return;
         const Grid g{S.nx, S.ny, S.nz};
         const CgTile t = batch_cg_tile(S, it);
         // iteration k = iters + 1 wrote p into p1 (k odd) or p0 (k even); phi is
@@ -262,8 +297,7 @@ __global__ void __launch_bounds__(64 * kCgBY) batch_cg_step_kernel(const sw_batc
             reduce_slots(S.part, nullptr, n_slots, red, &rs_new, &unused);
             if (threadIdx.x == 0 && threadIdx.y == 0) cg_after_step(st, rs_new);
         }
-        __syncthreads();
-    }
+    });
 }
 
 // The deferred phi term of each solve that stopped after an odd iteration.
@@ -309,7 +343,7 @@ int launch_batch_passes(const sw_batch_solve* solves, const int32_t* items, int6
     const unsigned nb = persistent_step_blocks(blocks);
     for (int64_t k = 0; k < n_passes; ++k) {
         hipLaunchKernelGGL(batch_step_kernel, dim3(nb), dim3(64, kBatchRows), 0, s, solves, (const int4*)items, n_items,
-                           (const long long*)gpass);
+                           (long long*)gpass);
         hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(1), 0, s, (long long*)gpass);
     }
     SW_LAUNCH_CHECK();
@@ -451,7 +485,7 @@ int sw_batch_project(const sw_batch_cg* solves, int32_t n_solves, const int32_t*
         SW_TRY_HIP(hipStreamSynchronize(s));
         return SW_OK;
     };
-    hipLaunchKernelGGL(batch_cg_reset_kernel, dim3((n_solves + 63) / 64), dim3(64), 0, s, solves, n_solves, st);
+    hipLaunchKernelGGL(batch_cg_reset_kernel, dim3((n_solves + 64) / 64), dim3(64), 0, s, solves, n_solves, st);
     hipLaunchKernelGGL(batch_cg_init_kernel, dim3(nb), blk, 0, s, solves, st, it, n_items);
     SW_LAUNCH_CHECK();
     int rc = poll();
@@ -461,8 +495,8 @@ int sw_batch_project(const sw_batch_cg* solves, int32_t n_solves, const int32_t*
         for (int k = 0; k < n_solves; ++k) running |= (h[k].stop == 0);
         if (!running) break;
         for (int64_t j = 0; j < chunk; ++j) {
-            hipLaunchKernelGGL(batch_cg_dir_kernel, dim3(nb), blk, 0, s, solves, st, it, n_items);
-            hipLaunchKernelGGL(batch_cg_step_kernel, dim3(nb), blk, 0, s, solves, st, it, n_items);
+            hipLaunchKernelGGL(batch_cg_dir_kernel, dim3(nb), blk, 0, s, solves, st, n_solves, it, n_items);
+            hipLaunchKernelGGL(batch_cg_step_kernel, dim3(nb), blk, 0, s, solves, st, n_solves, it, n_items);
         }
         SW_LAUNCH_CHECK();
         rc = poll();

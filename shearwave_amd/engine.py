@@ -84,11 +84,25 @@ def to_device(x, dtype, shape=None) -> torch.Tensor:
             raise ValueError("scalar needs a shape")
         t = torch.full(tuple(shape), float(x), dtype=tdt, device=dev)
     else:
-        a = np.ascontiguousarray(np.asarray(x), dtype=_DTYPES[dtype_name(dtype)][2])
-        t = torch.from_numpy(a).to(dev)
+        a = np.asarray(x)
+        if a.dtype in (np.float32, np.float64):
+            # upload in the source type, convert on the device (the same IEEE rounding as a host cast)
+            t = torch.from_numpy(np.ascontiguousarray(a)).to(dev).to(tdt)
+        else:
+            t = torch.from_numpy(np.ascontiguousarray(a, dtype=_DTYPES[dtype_name(dtype)][2])).to(dev)
     if shape is not None and tuple(t.shape) != tuple(shape):
         raise ValueError(f"expected shape {tuple(shape)}, got {tuple(t.shape)}")
     return t
+
+
+def upload_async(a: np.ndarray) -> torch.Tensor:
+    """A host array on the device through pinned memory, queued on the current stream without
+    waiting for it (a pageable copy would wait for the stream's queued work first)."""
+    a = np.ascontiguousarray(a)
+    if not a.flags.writeable:
+        a = a.copy()
+    h = torch.from_numpy(a).pin_memory()
+    return h.to(device(), non_blocking=True)
 
 
 _WORKSPACES: dict = {}
