@@ -386,9 +386,8 @@ int run_steps(sw_fdtd_state* st, int64_t n_steps, const double* env, const sw_fd
         if (out->inv_stride > 0 && (step + 1) % out->inv_stride == 0) {
             const int64_t k = (step + 1) / out->inv_stride - 1;
             T* base = (T*)out->inv_out + k * 4 * n;
-            rc = sw_strain_invariants_from_u(DType<T>::code, st->u[0], st->u[1], st->u[2], base, base + n,
-                                             base + 2 * n, base + 3 * n, g.nx, g.ny, g.nz, st->dx, st->dy, st->dz,
-                                             s);
+            rc = strain_invariants_in_loop(DType<T>::code, st->u[0], st->u[1], st->u[2], base, g.nx, g.ny, g.nz,
+                                           st->dx, st->dy, st->dz, s);
             if (rc) return rc;
         }
         if (out->n_traces > 0) {

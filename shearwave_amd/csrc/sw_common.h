@@ -214,6 +214,12 @@ inline unsigned flat_grid(int64_t n) {
     return (unsigned)(b < 1 ? 1 : b);
 }
 
+// The four strain invariants of u every few steps of sw_fdtd_run into out
+// (SoA: vol, dev, max_shear, von_mises, n each, in dtype), x-marching; one
+// rounding apart from sw_strain_invariants_from_u (strain.hip).
+int strain_invariants_in_loop(int dtype, const void* ux, const void* uy, const void* uz, void* out, int64_t nx,
+                              int64_t ny, int64_t nz, double dx, double dy, double dz, hipStream_t s);
+
 inline int check_dtype(int dtype) {
     SW_REQUIRE(dtype == SW_F32 || dtype == SW_F64, SW_ERR_ARG, "dtype %d is not SW_F32 or SW_F64", dtype);
     return SW_OK;

@@ -310,3 +310,26 @@ def test_trace_kernels_bounds_checked(sw):
     want = np.arange(n, dtype=np.float32)
     want[[0, 999, 5]] += 1.0
     np.testing.assert_array_equal(f.cpu().numpy(), want)
+
+
+# ----------------------------------------------------------- in-loop strain
+
+
+@pytest.mark.parametrize("dtype,tol", [("float32", 2e-6), ("float64", 1e-12)])
+def test_in_loop_strain_matches_numpy_strain_of_the_snapshots(sw, dtype, tol):
+    """The invariants computed inside the loop (x-marching kernel, partials times 1/(2h)) against
+    compute_strain_invariants(compute_strain_tensor(u)) (strain.py:16-135, NumPy in the same precision) of
+    the snapshot taken at the same step: within a few roundings (rel L2 per invariant)."""
+    from oracle import strain as S
+
+    p = _problem(shape=(34, 40, 70), n_steps=40, seed=23)
+    args = (None, None, p["bz"], p["rho"], p["mu"], *p["h"], p["dt"], p["n_steps"])
+    opts = {**p["opts"], "sw_dtype": dtype, "sw_strain_stride": 10}
+    u, v, snaps, strain = sw.shear_fdtd(*args, opts, snapshot_stride=10, return_strain=True)
+    assert snaps.shape[0] == 4 and strain["volumetric"].shape == (4,) + p["shape"]
+    for k in range(4):
+        s = snaps[k]
+        want = S.compute_strain_invariants(S.compute_strain_tensor(s[..., 0], s[..., 1], s[..., 2], *p["h"]))
+        for key in ("volumetric", "deviatoric", "max_shear", "von_mises"):
+            assert float(np.abs(want[key]).max()) > 0
+            assert rel(strain[key][k], want[key]) < tol, (k, key, rel(strain[key][k], want[key]))
