@@ -279,6 +279,22 @@ int launch_step_any(const StepArgs<T>& a, bool incr, bool visc, bool force, int 
     return SW_ERR_ARG;
 }
 
+// Traces of a two-step pass (increment form), whose state is u_{n+2}, d_{n+2}:
+// column c+1 gets u_z after step n+2, column c gets u_z - d_z, u_z after step
+// n+1 (u_n + d_{n+1} up to one rounding of u). Indices outside [0, n) give NaN.
+template <typename T>
+__global__ void __launch_bounds__(256) gather2_kernel(const T* __restrict__ uz, const T* __restrict__ dz, int64_t n,
+                                                      const int64_t* __restrict__ idx, int64_t count,
+                                                      T* __restrict__ out, int64_t stride) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const int64_t j = idx[k];
+    const bool ok = j >= 0 && j < n;
+    const T u = ok ? uz[j] : (T)NAN;
+    out[k * stride] = ok ? u - dz[j] : (T)NAN;
+    out[k * stride + 1] = u;
+}
+
 // Non-finite detection: if any of the three fields holds an inf or NaN and
 // *flag is still -1, *flag = step. One compare-and-swap per wave that saw one.
 template <typename T>
@@ -313,13 +329,14 @@ int run_steps(sw_fdtd_state* st, int64_t n_steps, const double* env, const sw_fd
     // tile_y names the two-step tile in a two-step run; its single steps use the default tile
     const int one_tile_y = kernel == SW_KERNEL_MARCH2 ? 0 : st->tile_y;
     const int64_t step0 = out ? out->step0 : 0;
-    // whether an output reads the state after global step `step`
+    // whether an output reads the whole state after global step `step` (a pass ends there); traces
+    // do not end passes: a two-step pass gathers both of its steps' traces afterwards
     auto output_after = [&](int64_t step) {
         if (!out) return false;
         if (out->snap_stride > 0 && (step + 1) % out->snap_stride == 0) return true;
-        if (out->inv_stride > 0 && (step + 1) % out->inv_stride == 0) return true;
-        return out->n_traces > 0;
+        return out->inv_stride > 0 && (step + 1) % out->inv_stride == 0;
     };
+    const bool traces = out && out->n_traces > 0;
 
     StepArgs<T> a{};
     a.m = (const T*)st->mu_over_rho;
@@ -338,7 +355,7 @@ int run_steps(sw_fdtd_state* st, int64_t n_steps, const double* env, const sw_fd
     for (int64_t it = 0; it < n_steps;) {
         const double fs = env ? env[it] : 0.0;
         // three force-free steps with no output after the first two
-        const bool three = triples && it + 2 < n_steps && !output_after(step0 + it) &&
+        const bool three = triples && !traces && it + 2 < n_steps && !output_after(step0 + it) &&
                            !output_after(step0 + it + 1) &&
                            (!env || (env[it] == 0.0 && env[it + 1] == 0.0 && env[it + 2] == 0.0));
         const bool two = !three && pairs && it + 1 < n_steps && !output_after(step0 + it);
@@ -390,7 +407,12 @@ int run_steps(sw_fdtd_state* st, int64_t n_steps, const double* env, const sw_fd
                                            st->dx, st->dy, st->dz, s);
             if (rc) return rc;
         }
-        if (out->n_traces > 0) {
+        if (traces && two) {
+            hipLaunchKernelGGL(gather2_kernel<T>, dim3((unsigned)((out->n_traces + 255) / 256)), dim3(256), 0, s,
+                               (const T*)st->u[2], (const T*)st->b[2], n, out->trace_idx, out->n_traces,
+                               (T*)out->trace_out + step - 1, out->trace_ld);
+            SW_LAUNCH_CHECK();
+        } else if (traces) {
             rc = sw_gather_traces(DType<T>::code, st->u[2], n, out->trace_idx, out->n_traces,
                                   (T*)out->trace_out + step, out->trace_ld, s);
             if (rc) return rc;

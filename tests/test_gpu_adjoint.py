@@ -138,3 +138,25 @@ def test_march_adjoint_close_to_fp64(sw):
     _, gmu32, geta32 = loss_and_grad(prob32, mu, eta, target)
     for a, b in ((gmu32, gmu), (geta32, geta)):
         assert float(torch.linalg.norm(a - b) / torch.linalg.norm(b)) < 1e-3
+
+
+def test_two_step_forward_traces_and_gradient(sw):
+    """On a grid the x-marching kernels take, the forward pairs steps and gathers both steps' traces after each
+    pass (step n+1 as u - d): the traces agree with single-step passes to float64 rounding, and the gradient
+    agrees across checkpoint spacings to rounding (the checkpointed states round as the pairs fall, and the
+    backward recomputes single steps from them)."""
+    from shearwave_amd import _abi
+
+    shape = (20, 16, 40)
+    prob, mu, eta, target = setup("float64", shape=shape)
+    tr2, _ = prob.forward(mu, eta, keep_checkpoints=False)
+    prob.solve.state.kernel = _abi.SW_KERNEL_MARCH  # one step per pass
+    tr1, _ = prob.forward(mu, eta, keep_checkpoints=False)
+    prob.solve.state.kernel = _abi.SW_KERNEL_AUTO
+    assert float(tr1.abs().max()) > 0
+    assert float(torch.linalg.norm(tr2 - tr1) / torch.linalg.norm(tr1)) < 1e-13
+    _, gmu, geta = loss_and_grad(prob, mu, eta, target)
+    prob1, _, _, _ = setup("float64", checkpoint_every=1, shape=shape)
+    _, gmu1, geta1 = loss_and_grad(prob1, mu, eta, target)
+    for a, b in ((gmu, gmu1), (geta, geta1)):
+        assert float(torch.linalg.norm(a - b) / torch.linalg.norm(b)) < 1e-11

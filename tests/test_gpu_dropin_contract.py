@@ -79,7 +79,7 @@ def test_return_device_equals_host_arrays(sw, dtype):
     np.testing.assert_array_equal(ud.cpu().numpy(), u)
     # traces
     idx = (np.array([3, 17, 30]), np.array([5, 16, 2]), np.array([18, 18, 33]))
-    # (traces read u after every step, so these runs take single steps: compare them with each other)
+    # (the trace runs are compared with each other: traces of a two-step pass are gathered after it)
     u_t, v_t, tr = sw.shear_fdtd_jax(*args, opts, trace_indices=idx, return_traces=True)
     ud, vd, trd = sw.shear_fdtd_jax(*args, opts, trace_indices=idx, return_traces=True, return_device=True)
     assert isinstance(trd, torch.Tensor) and tuple(trd.shape) == (3, p["n_steps"])
