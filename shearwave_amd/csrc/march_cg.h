@@ -24,6 +24,9 @@ struct CgState {
     long long max_iters;   // stop after this many phi updates (solver.py:1033)
     double true_relres;    // sqrt(|b - A phi|^2 / rs0) from the last true-residual pass
     double final_tol;      // the caller's tol; `tol` may be a looser stage before a residual replacement
+    double rz;             // r.z of the preconditioned CG (0 before the first application)
+    int precond;           // 1: preconditioned (z = M r; alpha and beta from r.z, the stop from r.r)
+    int pad3;
 };
 static_assert(sizeof(CgState) <= 128, "CgState too large");
 
@@ -106,7 +109,19 @@ __device__ inline void cg_after_dir(CgState* st, double denom) {
         st->reason = SW_CG_DENOM;
     } else {
         st->alpha_prev = st->alpha;
-        st->alpha = st->rs_old / denom;
+        st->alpha = (st->precond ? st->rz : st->rs_old) / denom;
+    }
+}
+
+// After z = M r and r.z of the preconditioned CG: beta = r.z / previous r.z
+// (0 for the first application, after which p = z).
+__device__ inline void cg_after_precond(CgState* st, double rz) {
+    st->counter = 0u;
+    st->beta = st->rz > 0.0 ? rz / st->rz : 0.0;
+    st->rz = rz;
+    if (!isfinite(rz)) {
+        st->stop = 1;
+        st->reason = SW_CG_NONFINITE;
     }
 }
 
@@ -125,7 +140,7 @@ __device__ inline void cg_after_step(CgState* st, double rs_new) {
         st->stop = 1;
         st->reason = SW_CG_CONVERGED;
     } else {
-        st->beta = rs_new / fmax(st->rs_old, kEps64);
+        if (!st->precond) st->beta = rs_new / fmax(st->rs_old, kEps64);  // else from r.z after z = M r
         st->rs_old = rs_new;
         if (st->iters >= st->max_iters) {
             st->stop = 1;
@@ -155,7 +170,7 @@ __device__ inline void cg_after_true_residual(CgState* st, double rs, int replac
         st->stop = 1;
         st->reason = SW_CG_MAX_ITERS;
     } else {
-        st->beta = rs / fmax(st->rs_old, kEps64);
+        if (!st->precond) st->beta = rs / fmax(st->rs_old, kEps64);  // else from r.z after z = M r
         st->rs_old = rs;
         st->tol = next_tol;
         st->stop = 0;
@@ -169,6 +184,9 @@ __device__ inline void cg_after_true_residual(CgState* st, double rs, int replac
 __device__ inline void cg_reset(CgState* st, double tol, long long max_iters, double stage_tol) {
     st->rs_old = st->rs0 = st->alpha = st->beta = st->relres = st->max_div = st->alpha_prev = 0.0;
     st->true_relres = -1.0;
+    st->rz = 0.0;
+    st->precond = 0;
+    st->pad3 = 0;
     st->final_tol = tol;
     st->tol = stage_tol;
     st->iters = 0;

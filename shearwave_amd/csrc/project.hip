@@ -27,6 +27,7 @@
 #include <type_traits>
 
 #include "march_cg.h"
+#include "spectral.h"
 #include "sw_common.h"
 
 namespace sw {
@@ -34,8 +35,9 @@ namespace {
 
 constexpr int kMaxRedBlocks = 2048;
 
-__global__ void cg_reset_kernel(CgState* st, double tol, long long max_iters, double stage_tol) {
+__global__ void cg_reset_kernel(CgState* st, double tol, long long max_iters, double stage_tol, int precond) {
     cg_reset(st, tol, max_iters, stage_tol);
+    st->precond = precond;
 }
 
 // r = -div_c(mask f) on the interior, p_old = 0, phi = 0; reduces r.r and max|div|.
@@ -323,6 +325,7 @@ struct Workspace {
     void *r, *p0, *p1;
     double *part_sum, *part_max;
     CgState* st;
+    float* tables;  // spectral preconditioner tables (when the workspace holds them), else null
 };
 
 int64_t align256(int64_t b) { return (b + 255) & ~(int64_t)255; }
@@ -332,9 +335,16 @@ int64_t partial_capacity(int64_t nx, int64_t ny, int64_t nz) {
     return m > kMaxRedBlocks ? m : kMaxRedBlocks;
 }
 
-int64_t workspace_bytes(int64_t nx, int64_t ny, int64_t nz) {
+// Without the preconditioner's tables.
+int64_t plain_workspace_bytes(int64_t nx, int64_t ny, int64_t nz) {
     const int64_t n = nx * ny * nz;
     return 4 * align256(n * 8) + 2 * align256(partial_capacity(nx, ny, nz) * 8) + 256;
+}
+
+// With them (grids whose chains have tables).
+int64_t workspace_bytes(int64_t nx, int64_t ny, int64_t nz) {
+    const int64_t b = plain_workspace_bytes(nx, ny, nz);
+    return spectral_shape(nx, ny, nz) ? b + align256(spectral_table_bytes(nx, ny, nz)) : b;
 }
 
 Workspace carve(void* base, int64_t nx, int64_t ny, int64_t nz) {
@@ -355,6 +365,8 @@ Workspace carve(void* base, int64_t nx, int64_t ny, int64_t nz) {
     w.part_max = (double*)p;
     p += align256(pc * 8);
     w.st = (CgState*)p;
+    p += 256;
+    w.tables = spectral_shape(nx, ny, nz) ? (float*)p : nullptr;
     return w;
 }
 
@@ -362,6 +374,18 @@ unsigned red_grid(const Grid& g) {
     int64_t cap = (int64_t)device_cu_count() * 8;
     if (cap > kMaxRedBlocks) cap = kMaxRedBlocks;
     return row_grid(g, cap);
+}
+
+// The preconditioned CG is used only at tolerances this tight or tighter.
+constexpr double kPcgMaxTol = 1e-7;
+
+// Iterations the plain CG takes to tol on a grid, from measurements (256^3 at
+// 1e-8: 599; 48^3 at 1e-8: 113; 40x21x70 at 1e-9: 179), rounded up: the
+// preconditioned CG stands in for it only where max_iters would not stop it.
+inline int64_t plain_cg_iters(const Grid& g, double tol) {
+    if (!(tol > 0.0)) return INT64_MAX;
+    const int nmax = g.nx > g.ny ? (g.nx > g.nz ? g.nx : g.nz) : (g.ny > g.nz ? g.ny : g.nz);
+    return (int64_t)ceil(nmax * (0.25 * log10(1.0 / tol) + 0.5));
 }
 
 // Stop levels of the float32-vector CG before its true-residual checks: each
@@ -390,16 +414,25 @@ constexpr int kMaxReplacements = 24;
 template <typename T, typename V>
 int run_cg(const Workspace& w, const Grid& g, bool march, const CgShape& cs, unsigned nb, const T* fx, const T* fy,
            const T* fz, double i2hx, double i2hy, double i2hz, double qx, double qy, double qz, double tol,
-           int64_t max_iters, double stage, CgState& hs, hipStream_t s) {
+           int64_t max_iters, double stage, const SubLat* pcg, CgState& hs, hipStream_t s) {
     const dim3 blk = row_block();
     const bool replacing = sizeof(V) == 4;
-    SW_TRY_HIP(hipMemcpyAsync(&hs, w.st, sizeof(CgState), hipMemcpyDeviceToHost, s));
-    SW_TRY_HIP(hipStreamSynchronize(s));
     V* const p_odd = (V*)w.p1;   // iteration k (counted from 1) writes its p here when k is odd
     V* const p_even = (V*)w.p0;  // and here when k is even
     V* r = (V*)w.r;
+    // preconditioned: z = M r in the upper half of r's slot, the transforms' work in those of p0 and p1
+    V* const z = pcg ? r + g.size() : r;
+    float* const W1 = (float*)w.p0 + g.size();
+    float* const W2 = (float*)w.p1 + g.size();
+    auto precondition = [&]() {
+        if (pcg) spectral_apply<V>(r, z, W1, W2, g, *pcg, nb, w.part_sum, w.st, s);
+    };
+    precondition();  // z0 = M r0 and r0.z0 (p1 = z0: p_old starts at zero)
+    SW_LAUNCH_CHECK();
+    SW_TRY_HIP(hipMemcpyAsync(&hs, w.st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+    SW_TRY_HIP(hipStreamSynchronize(s));
     int64_t base = 0;  // iterations whose phi terms are all in phi (the pairing restarts after a flush)
-    int64_t chunk = 8;  // iterations launched between two polls of the stop flag
+    int64_t chunk = pcg ? 2 : 8;  // iterations launched between two polls of the stop flag
     for (int round = 0;; ++round) {
         int64_t launched = hs.iters;
         while (!hs.stop && launched < max_iters) {
@@ -410,7 +443,7 @@ int run_cg(const Workspace& w, const Grid& g, bool march, const CgShape& cs, uns
                 V* p_old = (it & 1) ? p_even : p_odd;
                 if (march) {
                     const bool skip = ((it - base) & 1) == 1;
-                    hipLaunchKernelGGL(cg_dir_march_kernel<V>, dim3(cs.blocks()), dim3(64, kCgBY), 0, s, r, p_old,
+                    hipLaunchKernelGGL(cg_dir_march_kernel<V>, dim3(cs.blocks()), dim3(64, kCgBY), 0, s, z, p_old,
                                        p_new, g, qx, qy, qz, cs.xchunk, cs.nty, cs.ntz, w.part_sum, w.st);
                     if (skip)
                         hipLaunchKernelGGL((cg_step_march_kernel<V, kPhiSkip>), dim3(cs.blocks()), dim3(64, kCgBY), 0,
@@ -421,11 +454,12 @@ int run_cg(const Workspace& w, const Grid& g, bool march, const CgShape& cs, uns
                                            s, w.phi, r, p_new, p_old, g, qx, qy, qz, cs.xchunk, cs.nty, cs.ntz,
                                            w.part_sum, w.st);
                 } else {
-                    hipLaunchKernelGGL(cg_dir_kernel<V>, dim3(nb), blk, 0, s, r, p_old, p_new, g, qx, qy, qz,
+                    hipLaunchKernelGGL(cg_dir_kernel<V>, dim3(nb), blk, 0, s, z, p_old, p_new, g, qx, qy, qz,
                                        w.part_sum, w.st);
                     hipLaunchKernelGGL(cg_step_kernel<V>, dim3(nb), blk, 0, s, w.phi, r, p_new, g, qx, qy, qz,
                                        w.part_sum, w.st);
                 }
+                precondition();
             }
             SW_LAUNCH_CHECK();
             launched += k;
@@ -451,6 +485,8 @@ int run_cg(const Workspace& w, const Grid& g, bool march, const CgShape& cs, uns
         SW_TRY_HIP(hipStreamSynchronize(s));
         if (!replace || hs.stop) break;
         stage = next;
+        precondition();  // z = M r of the replaced residual; beta from r.z
+        SW_LAUNCH_CHECK();
     }
     return SW_OK;
 }
@@ -467,7 +503,7 @@ int run_cg(const Workspace& w, const Grid& g, bool march, const CgShape& cs, uns
 // one GPU (different streams or threads) race. Nothing in it needs to start at
 // zero: every call resets the CG state and writes phi, r and p_old before any
 // read, and each slot is written before it is summed.
-constexpr int64_t kNoAllocScratchBytes = (int64_t(32) << 20) + (int64_t(64) << 10);
+constexpr int64_t kNoAllocScratchBytes = (int64_t(33) << 20) + (int64_t(320) << 10);  // incl. spectral tables
 constexpr int64_t kNoAllocMaxVoxels = int64_t(1) << 20;
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
@@ -518,7 +554,7 @@ int sw_project_vector_field(int dtype, int vec_dtype, void* fx, void* fy, void* 
     SW_REQUIRE(fx && fy && fz, SW_ERR_ARG, "null force component");
     SW_REQUIRE(max_iters >= 0, SW_ERR_ARG, "max_iters must be >= 0");
     SW_REQUIRE(dx > 0 && dy > 0 && dz > 0, SW_ERR_ARG, "grid spacing must be positive");
-    SW_REQUIRE(workspace && workspace_bytes_given >= workspace_bytes(nx, ny, nz), SW_ERR_WORKSPACE,
+    SW_REQUIRE(workspace && workspace_bytes_given >= plain_workspace_bytes(nx, ny, nz), SW_ERR_WORKSPACE,
                "workspace of %lld bytes is smaller than the %lld needed", (long long)workspace_bytes_given,
                (long long)workspace_bytes(nx, ny, nz));
     SW_REQUIRE(((uintptr_t)workspace & 255) == 0, SW_ERR_ARG, "workspace must be 256-byte aligned");
@@ -537,9 +573,21 @@ int sw_project_vector_field(int dtype, int vec_dtype, void* fx, void* fy, void* 
     const double i2hx = 1.0 / (2.0 * dx), i2hy = 1.0 / (2.0 * dy), i2hz = 1.0 / (2.0 * dz);
     const double qx = 1.0 / (4.0 * dx * dx), qy = 1.0 / (4.0 * dy * dy), qz = 1.0 / (4.0 * dz * dz);
 
-    // float32 vectors first stop at a looser stage level for a residual replacement (run_cg)
-    const double stage = (vec_dtype == SW_F32 && tol < kStageFactor) ? kStageFactor : tol;
-    hipLaunchKernelGGL(cg_reset_kernel, dim3(1), dim3(1), 0, s, w.st, tol, (long long)max_iters, stage);
+    // the spectral preconditioner (float32 vectors): where the workspace holds its tables and the plain CG
+    // would meet tol too (tol <= 1e-7, max_iters at least its iteration estimate), so that the result is the
+    // reference's to well within the float32 form's tolerance; SW_CG_PRECOND=0 / 1 turns it off / on
+    const char* env_p = getenv("SW_CG_PRECOND");
+    const bool pcg_possible = vec_dtype == SW_F32 && spectral_shape(nx, ny, nz) &&
+                              workspace_bytes_given >= workspace_bytes(nx, ny, nz);
+    const bool pcg = pcg_possible && !(env_p && env_p[0] == '0') &&
+                     ((env_p && env_p[0] == '1') || (tol <= kPcgMaxTol && max_iters >= plain_cg_iters(g, tol)));
+    if (!pcg) w.tables = nullptr;
+    SubLat S{};
+    if (pcg && (rc = spectral_setup(nx, ny, nz, dx, dy, dz, w.tables, S, s))) return rc;
+    // float32 vectors first stop at a looser stage level for a residual replacement (run_cg); the
+    // preconditioned CG converges in a few iterations and checks its true residual at tol
+    const double stage = (vec_dtype == SW_F32 && tol < kStageFactor && !pcg) ? kStageFactor : tol;
+    hipLaunchKernelGGL(cg_reset_kernel, dim3(1), dim3(1), 0, s, w.st, tol, (long long)max_iters, stage, pcg ? 1 : 0);
     SW_LAUNCH_CHECK();
     CgState hs;
     if (vec_dtype == SW_F32) {
@@ -549,7 +597,7 @@ int sw_project_vector_field(int dtype, int vec_dtype, void* fx, void* fy, void* 
                                w.part_max, w.st);
             SW_LAUNCH_CHECK();
             if ((rc = run_cg<T, float>(w, g, march, cs, nb, (const T*)fx, (const T*)fy, (const T*)fz, i2hx, i2hy,
-                                       i2hz, qx, qy, qz, tol, max_iters, stage, hs, s)))
+                                       i2hz, qx, qy, qz, tol, max_iters, stage, pcg ? &S : nullptr, hs, s)))
                 return rc;
         });
     } else {
@@ -559,7 +607,7 @@ int sw_project_vector_field(int dtype, int vec_dtype, void* fx, void* fy, void* 
                                w.part_max, w.st);
             SW_LAUNCH_CHECK();
             if ((rc = run_cg<T, double>(w, g, march, cs, nb, (const T*)fx, (const T*)fy, (const T*)fz, i2hx, i2hy,
-                                        i2hz, qx, qy, qz, tol, max_iters, stage, hs, s)))
+                                        i2hz, qx, qy, qz, tol, max_iters, stage, nullptr, hs, s)))
                 return rc;
         });
     }

@@ -333,3 +333,51 @@ def test_in_loop_strain_matches_numpy_strain_of_the_snapshots(sw, dtype, tol):
         for key in ("volumetric", "deviatoric", "max_shear", "von_mises"):
             assert float(np.abs(want[key]).max()) > 0
             assert rel(strain[key][k], want[key]) < tol, (k, key, rel(strain[key][k], want[key]))
+
+
+# ------------------------------------------------- spectral preconditioner
+
+
+@pytest.mark.parametrize("shape", [(40, 21, 70), (33, 34, 35), (64, 64, 64), (9, 70, 12)])
+def test_preconditioned_projection(sw, shape, monkeypatch):
+    """The float32-vector CG with the spectral preconditioner (forced on: SW_CG_PRECOND=1) converges in a few
+    iterations to a TRUE residual below tol, and the projection agrees with the float64 oracle's converged one
+    (odd and even axis lengths: every pairing of flux and value chain ends; unequal spacings)."""
+    monkeypatch.setenv("SW_CG_PRECOND", "1")
+    h = (0.3e-3, 0.25e-3, 0.35e-3)
+    rng = np.random.default_rng(sum(shape))
+    f = [(rng.standard_normal(shape) * taper(shape)) for _ in range(2)]
+    f.append(300.0 * gauss(shape, 4.0) + rng.standard_normal(shape) * taper(shape))
+    m = O.interior_mask(shape, np.float64)
+    d_in = np.linalg.norm(O.divergence_center(*(a * m for a in f), *h) * m)
+    info = {}
+    out = sw.project_vector_field(*f, *h, 1e-10, 5000, dtype="float64", info=info, cg_vectors="float32")
+    assert info["reason"] == "converged" and info["iters"] <= 12, info
+    ratio = np.linalg.norm(O.divergence_center(*out, *h) * m) / d_in
+    assert ratio < 1e-10 and abs(info["true_relres"] - ratio) < 1e-3 * ratio, (ratio, info)
+    want = O.project_vector_field(*f, *h, 1e-13, 20000, dtype=np.float64)
+    for a, b in zip(out, want):
+        assert rel(a, b) < 1e-9
+    info32 = {}
+    out32 = sw.project_vector_field(*(a.astype(np.float32) for a in f), *h, 1e-9, 5000, info=info32)
+    assert info32["reason"] == "converged" and info32["iters"] <= 12 and info32["true_relres"] < 1e-9, info32
+    for a, b in zip(out32, want):
+        assert rel(a, b) < 1e-6
+
+
+def test_preconditioner_only_where_the_plain_cg_converges(sw):
+    """By default the preconditioned CG stands in for the plain one only at tol <= 1e-7 with max_iters at least
+    the plain CG's iteration estimate: the reference's defaults (tol 1e-5, 200 iterations) keep the plain CG and
+    its iterates, tight tolerances take a few preconditioned iterations."""
+    shape = (48, 40, 72)
+    h = (0.25e-3,) * 3
+    rng = np.random.default_rng(6)
+    f = [(rng.standard_normal(shape) * taper(shape)).astype(np.float32) for _ in range(3)]
+    runs = {}
+    for tol, max_iters in ((1e-5, 200), (1e-8, 20000), (1e-8, 40)):
+        info = {}
+        sw.project_vector_field(*f, *h, tol, max_iters, info=info)
+        runs[(tol, max_iters)] = info
+    assert runs[(1e-5, 200)]["iters"] > 30  # plain CG
+    assert runs[(1e-8, 20000)]["iters"] <= 12 and runs[(1e-8, 20000)]["true_relres"] < 1e-8
+    assert runs[(1e-8, 40)]["reason"] == "max_iters" and runs[(1e-8, 40)]["iters"] == 40  # plain, stopped early

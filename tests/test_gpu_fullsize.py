@@ -59,7 +59,7 @@ def test_projection_256_divergence_free(sw, workload):
     info = {}
     out = sw.project_vector_field(*(f_in[c].float() for c in range(3)), h, h, h, w["tol"], w["max_iters"],
                                   info=info)
-    assert info["reason"] == "converged" and info["iters"] > 100, info
+    assert info["reason"] == "converged" and info["iters"] >= 1, info  # spectral-preconditioned at 256^3
     assert 0 < info["true_relres"] < w["tol"], info
     print(f"256^3 projection, float32 storage and CG vectors: {info}")
     d_in = sw.divergence_center(*(f_in[c] for c in range(3)), h, h, h)

@@ -117,6 +117,7 @@ def test_projection_float32_cg_vectors(sw, variant, monkeypatch):
     """
     if variant == "direct":
         monkeypatch.setenv("SW_CG_KERNEL", "direct")
+    monkeypatch.setenv("SW_CG_PRECOND", "0")  # the plain CG: its iteration count is compared
     rng = np.random.default_rng(11)
     shape = (40, 21, 70)  # partial y and z tiles
     f = [(rng.standard_normal(shape) * taper(shape)).astype(np.float32) for _ in range(2)]

@@ -172,7 +172,13 @@ def run_with_outputs_against_rule(sw, kernel, dtype, snap_stride, inv_stride, th
     torch.cuda.synchronize()
     _assert_equal_states(got, ref)
     assert torch.equal(snaps, snaps_ref)
-    assert torch.equal(inv, inv_ref)
+    # the in-loop invariants multiply by 1/(2h) where sw_strain_invariants_from_u divides (NumPy's order):
+    # equal to a few roundings
+    tol = 1e-6 if dtype == "float32" else 1e-14
+    for k in range(inv.shape[0]):
+        for j in range(4):
+            a, b = inv[k, j].double(), inv_ref[k, j].double()
+            assert float(torch.linalg.norm(a - b) / torch.linalg.norm(b)) < tol, (k, j)
     assert float(snaps.abs().max()) > 0 and float(inv.abs().max()) > 0
 
 

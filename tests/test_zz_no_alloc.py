@@ -71,7 +71,8 @@ def test_noalloc_projection_equals_workspace_projection(sw, dtype, vec, tol):
         ws = torch.empty(lib.sw_project_workspace_bytes(*shape), dtype=torch.uint8, device="cuda")
         want = _project(lib, f, inv_rho, dtype, vec, tol, ws)
         got = _project(lib, f, inv_rho, dtype, vec, tol, None)
-        assert want[2][3] == 1 and want[2][0] > 10, want[2]
+        # converged (float32 vectors: the spectral-preconditioned CG, a few iterations; float64: the plain CG)
+        assert want[2][3] == 1 and want[2][0] >= 1 and want[2][4] < tol, want[2]
         np.testing.assert_array_equal(got[0], want[0])
         np.testing.assert_array_equal(got[1], want[1])
         assert got[2] == want[2]
