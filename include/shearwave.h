@@ -107,13 +107,27 @@ int sw_project_vector_field(int dtype, int vec_dtype, void* fx, void* fy, void* 
                             void* workspace, int64_t workspace_bytes, void* const* scale_out, const void* inv_rho,
                             double scale, sw_cg_info* info, void* stream);
 
+/* Whether sw_project_vector_field, given a workspace of
+ * sw_project_workspace_bytes(nx, ny, nz), runs the spectral-preconditioned CG
+ * (1) or the plain one (0). The preconditioner (an exact solve of the
+ * interior operator -div_c grad_c on each of its 8 parity sub-lattices, by
+ * closed-form sine/cosine bases along each axis) is taken for float32 CG
+ * vectors on grids whose parity chains have at most 256 points, when tol <=
+ * 1e-7 and max_iters is at least the plain CG's iteration estimate, i.e. where
+ * the plain CG of poisson_cg (solver.py:1010-1050) meets tol as well: both
+ * stop on sqrt(r.r / r0.r0) < tol, and the preconditioned one checks its true
+ * residual. Environment SW_CG_PRECOND=0 / 1 turns it off / on wherever
+ * possible. No reference counterpart. */
+int sw_project_preconditioned(int vec_dtype, int64_t nx, int64_t ny, int64_t nz, double tol, int64_t max_iters);
+
 /* No reference counterpart: for callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc). sw_project_vector_field with the CG workspace taken from a
  * static device array of the library instead of a caller's buffer, for grids
  * of at most sw_project_noalloc_max_voxels() (2^20) voxels; larger grids are
  * rejected with SW_ERR_ARG. It allocates nothing. Each GPU holds one such
- * array (about 32 MiB) until the process ends, shared by every call on that
+ * array (33.3 MiB: the CG vectors and the preconditioner's tables) until the
+ * process ends, shared by every call on that
  * GPU: calls that overlap on one GPU race. */
 int64_t sw_project_noalloc_max_voxels(void);
 int sw_project_vector_field_noalloc(int dtype, int vec_dtype, void* fx, void* fy, void* fz, int64_t nx, int64_t ny,

@@ -180,6 +180,7 @@ SIGNATURES = {
     "sw_export_state": (_i32, [_i32, _i32, C.POINTER(_p), C.POINTER(_p), _d, _p, _p, _i64, _p]),
     "sw_import_state": (_i32, [_i32, _i32, _p, _p, _d, C.POINTER(_p), C.POINTER(_p), _i64, _i64, _i64, _p]),
     "sw_combine3": (_i32, [_i32, C.POINTER(_p), C.POINTER(_p), _d, _d, C.POINTER(_p), _i64, _p]),
+    "sw_project_preconditioned": (_i32, [_i32, _i64, _i64, _i64, _d, _i64]),
     "sw_project_noalloc_max_voxels": (_i64, []),
     "sw_project_vector_field_noalloc": (_i32, [_i32, _i32, _p, _p, _p, _i64, _i64, _i64, _d, _d, _d, _d, _i64,
                                                 C.POINTER(_p), _p, _d, C.POINTER(CgInfo), _p]),

@@ -332,13 +332,26 @@ class Batch:
         return E.upload_async(np.frombuffer(bytes(arr), dtype=np.uint8))
 
     # ---------------------------------------------------------- projection
-    def project(self, solves=None, xchunk=None):
-        """Project every solve's force with one batched CG; keeps dT^2 f_S / rho for the loop.
+    def project(self, solves=None, xchunk=None, method="auto"):
+        """Project every solve's force; keeps dT^2 f_S / rho for the loop.
 
-        `xchunk` (a multiple of CG_SUB) overrides :func:`cg_xchunk`; it changes
-        the speed only.
+        ``method``: ``"each"`` projects the solves one after another with the
+        single-solve projection (the spectral-preconditioned CG converges in a
+        few iterations); ``"batched"`` runs one batched plain CG over all of
+        them; ``"auto"`` takes ``"each"`` when every solve's projection would be
+        preconditioned (``sw_project_preconditioned``). `xchunk` (a multiple of
+        CG_SUB) overrides :func:`cg_xchunk` for the batched CG; it changes the
+        speed only.
         """
         solves = self.solves if solves is None else solves
+        if method == "auto":
+            method = "each" if solves and all(
+                lib().sw_project_preconditioned(_abi.SW_F32, *s.shape, float(s.spec.tol), int(s.spec.max_iters))
+                for s in solves) else "batched"
+        if method == "each":
+            return self._project_each(solves)
+        if method != "batched":
+            raise ValueError(f"unknown projection method {method!r}")
         dev = E.device()
         cgs = (_abi.BatchCg * len(solves))()
         items = []
@@ -393,6 +406,24 @@ class Batch:
         self.cg_info = [{"iters": int(i.iters), "relres": float(i.relres), "max_div": float(i.max_div),
                          "reason": _abi.CG_REASONS.get(int(i.reason), str(i.reason))} for i in info]
         return self.cg_info
+
+    def _project_each(self, solves):
+        """The single-solve projection of each solve in turn, with one workspace sized for the largest."""
+        nbytes = max(int(lib().sw_project_workspace_bytes(*s.shape)) for s in solves)
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=E.device())
+        st = E.stream_ptr()
+        info = []
+        for s in solves:
+            ci = _abi.CgInfo()
+            check(lib().sw_project_vector_field(
+                _abi.SW_F32, _abi.SW_F32, s.f[0].data_ptr(), s.f[1].data_ptr(), s.f[2].data_ptr(), *s.shape, *s.h,
+                float(s.spec.tol), int(s.spec.max_iters), ws.data_ptr(), ws.numel(), _abi.ptrs(*s.force),
+                s.inv_rho.data_ptr(), float(s.spec.dt) ** 2, C.byref(ci), st))
+            info.append({"iters": int(ci.iters), "relres": float(ci.relres), "max_div": float(ci.max_div),
+                         "reason": _abi.CG_REASONS.get(int(ci.reason), str(ci.reason)),
+                         "true_relres": float(ci.true_relres)})
+        self.cg_info = info
+        return info
 
     # ---------------------------------------------------------------- loop
     def _advance(self, desc, items_t, n_items, n_passes, keep=None):

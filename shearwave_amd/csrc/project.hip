@@ -388,6 +388,18 @@ inline int64_t plain_cg_iters(const Grid& g, double tol) {
     return (int64_t)ceil(nmax * (0.25 * log10(1.0 / tol) + 0.5));
 }
 
+// Whether a projection (with a workspace of workspace_bytes) takes the spectral
+// preconditioner: float32 vectors, a grid with tables, and tol / max_iters at
+// which the plain CG converges too; SW_CG_PRECOND=0 / 1 turns it off / on
+// wherever possible.
+inline bool uses_preconditioner(int vec_dtype, int64_t nx, int64_t ny, int64_t nz, double tol, int64_t max_iters) {
+    if (vec_dtype != SW_F32 || !spectral_shape(nx, ny, nz)) return false;
+    const char* env = getenv("SW_CG_PRECOND");
+    if (env && env[0] == '0') return false;
+    if (env && env[0] == '1') return true;
+    return tol <= kPcgMaxTol && max_iters >= plain_cg_iters(Grid{(int)nx, (int)ny, (int)nz}, tol);
+}
+
 // Stop levels of the float32-vector CG before its true-residual checks: each
 // level is this factor below the one before, down to the caller's tol. One
 // replacement per decade keeps each replacement's change to r (the drift since
@@ -576,11 +588,8 @@ int sw_project_vector_field(int dtype, int vec_dtype, void* fx, void* fy, void* 
     // the spectral preconditioner (float32 vectors): where the workspace holds its tables and the plain CG
     // would meet tol too (tol <= 1e-7, max_iters at least its iteration estimate), so that the result is the
     // reference's to well within the float32 form's tolerance; SW_CG_PRECOND=0 / 1 turns it off / on
-    const char* env_p = getenv("SW_CG_PRECOND");
-    const bool pcg_possible = vec_dtype == SW_F32 && spectral_shape(nx, ny, nz) &&
-                              workspace_bytes_given >= workspace_bytes(nx, ny, nz);
-    const bool pcg = pcg_possible && !(env_p && env_p[0] == '0') &&
-                     ((env_p && env_p[0] == '1') || (tol <= kPcgMaxTol && max_iters >= plain_cg_iters(g, tol)));
+    const bool pcg = workspace_bytes_given >= workspace_bytes(nx, ny, nz) &&
+                     uses_preconditioner(vec_dtype, nx, ny, nz, tol, max_iters);
     if (!pcg) w.tables = nullptr;
     SubLat S{};
     if (pcg && (rc = spectral_setup(nx, ny, nz, dx, dy, dz, w.tables, S, s))) return rc;
@@ -627,6 +636,11 @@ int sw_project_vector_field(int dtype, int vec_dtype, void* fx, void* fy, void* 
         info->true_relres = hs.reason == SW_CG_EARLY_EXIT ? 0.0 : hs.true_relres;
     }
     return SW_OK;
+}
+
+int sw_project_preconditioned(int vec_dtype, int64_t nx, int64_t ny, int64_t nz, double tol, int64_t max_iters) {
+    if (check_grid(nx, ny, nz)) return 0;
+    return uses_preconditioner(vec_dtype, nx, ny, nz, tol, max_iters) ? 1 : 0;
 }
 
 int64_t sw_project_noalloc_max_voxels(void) { return kNoAllocMaxVoxels; }

@@ -179,7 +179,7 @@ def test_batched_cg_stops_after_odd_and_even_iterations(sw):
     for sp, k in zip(sps, (5, 6)):
         sp.tol, sp.max_iters = 1e-30, k
     b = Batch(sps)
-    info = b.project()
+    info = b.project(method="batched")
     for k, sp in enumerate(sps):
         assert info[k]["reason"] == "max_iters" and info[k]["iters"] == sp.max_iters, info[k]
         z = np.zeros(sp.shape, np.float32)
@@ -201,7 +201,7 @@ def test_batched_cg_item_length_changes_nothing(sw):
     got = []
     for xc in (48, 96, 192):
         b = Batch(sps)
-        info = b.project(xchunk=xc)
+        info = b.project(xchunk=xc, method="batched")
         assert all(i["reason"] == "converged" for i in info), info
         got.append(([i["iters"] for i in info], [[t.cpu().numpy() for t in s.force] for s in b.solves]))
     for iters, forces in got[1:]:
@@ -211,7 +211,7 @@ def test_batched_cg_item_length_changes_nothing(sw):
                 np.testing.assert_array_equal(a, c)
     assert any(np.any(f) for f in got[0][1][0])
     with pytest.raises(ValueError, match="multiple of 48"):
-        Batch(sps[:1]).project(xchunk=100)
+        Batch(sps[:1]).project(xchunk=100, method="batched")
 
 
 def test_batched_cg_rejects_bad_descriptors(sw, monkeypatch):
@@ -222,4 +222,20 @@ def test_batched_cg_rejects_bad_descriptors(sw, monkeypatch):
     b = B.Batch([make_spec((150, 20, 70), 313, 4)], axes="given")  # x stays the 150-plane axis
     monkeypatch.setattr(B, "cg_xchunk", lambda shapes, n_cu: 100)
     with pytest.raises(RuntimeError, match="multiple of 48"):
-        b.project()
+        b.project(method="batched")
+
+
+def test_projection_methods_agree(sw, specs):
+    """Each solve projected in turn (spectral-preconditioned CG) and all of them by one batched plain CG: the
+    projected loop forces agree to the CG tolerance (1e-10)."""
+    from shearwave_amd.batch import Batch
+
+    got = {}
+    for method in ("each", "batched"):
+        b = Batch(specs)
+        info = b.project(method=method)
+        assert all(i["reason"] == "converged" for i in info), (method, info)
+        got[method] = ([i["iters"] for i in info], [s.given_vec(s.force).cpu().numpy() for s in b.solves])
+    assert max(got["each"][0]) <= 12 < min(got["batched"][0])
+    for a, c in zip(got["each"][1], got["batched"][1]):
+        assert rel(a, c) < 1e-7
