@@ -259,10 +259,12 @@ typedef struct sw_batch_solve {
 /* Advance the batch by n_passes global passes: per pass one persistent launch
  * over all `items` (tile_y = 8; blocks = 0 picks one resident wave) and one
  * increment of gpass[0]. gpass is device memory of SW_BATCH_PASS_STATE int64:
- * gpass[0] the global pass, gpass[1..2] the launch's work-claim counters,
- * which the caller zeroes once and every launch leaves at zero. Blocks claim
- * items dynamically in list order, so list the longest items first. */
-#define SW_BATCH_PASS_STATE 4
+ * gpass[0] the global pass, gpass[1..9] the launch's work-claim counters,
+ * which the caller zeroes once and every launch leaves at zero. The item list
+ * is cut into 8 equal ranges, one per XCD; blocks claim items of their XCD's
+ * range in list order, then of the others': list neighbouring tiles next to
+ * each other. */
+#define SW_BATCH_PASS_STATE 16
 int sw_batch_step(const sw_batch_solve* solves, const int32_t* items, int64_t n_items, int64_t* gpass, int32_t tile_y,
                   int32_t blocks, int64_t n_passes, void* stream);
 

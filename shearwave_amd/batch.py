@@ -28,7 +28,7 @@ from . import engine as E
 from ._abi import check, lib
 
 STEP_BY = 8  # output rows of a batched step tile (blocks of 64 x 11 threads)
-PASS_STATE = 4  # int64 of the batch's pass state: the global pass and two work-claim counters
+PASS_STATE = 16  # int64 of the batch's pass state: the global pass and the work-claim counters
 MAX_PLANE = 1 << 28  # the kernels address a y-z plane with 32-bit offsets
 CG_BY = 16  # tile rows of the batched CG (fixed by the kernels)
 CG_SUB = 48  # x planes per partial sum of the batched CG (fixed by the kernels)
@@ -232,9 +232,12 @@ def item_cost(x_planes, z_width):
     return (x_planes + 3) * (32 + z_width)
 
 
-def step_items(solves, by=STEP_BY):
-    """Work items (solve, z tile, y tile, x chunk), longest first: the kernel's blocks claim them in list
-    order, so the short ones fill in at the end of a pass."""
+def step_items(solves, by=STEP_BY, order="spatial"):
+    """Work items (solve, z tile, y tile, x chunk).
+
+    ``"spatial"``: each solve's items with z fastest, then y, then x chunk,
+    so that the step kernel's per-XCD ranges hold neighbouring tiles;
+    ``"longest"``: longest first (short ones fill in at the end of a pass)."""
     items, cost = [], []
     for s_idx, s in enumerate(solves):
         nx, ny, nz = s.shape
@@ -247,8 +250,9 @@ def step_items(solves, by=STEP_BY):
     if not items:
         return np.zeros((0, 4), dtype=np.int32)
     items = np.concatenate(items).astype(np.int32)
-    order = np.argsort(-np.concatenate(cost).astype(np.int64), kind="stable")
-    return items[order]
+    if order == "spatial":
+        return items
+    return items[np.argsort(-np.concatenate(cost).astype(np.int64), kind="stable")]
 
 
 def default_xchunk(shape):

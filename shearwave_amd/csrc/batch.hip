@@ -72,6 +72,41 @@ __device__ inline void claim_items(unsigned long long* claim, int64_t n_items, F
     }
 }
 
+// Claims per XCD: the item list is cut into 8 equal ranges, one per XCD (a
+// block's XCD is blockIdx % 8: the hardware deals blocks round-robin), and a
+// block claims items of its own XCD's range first, so tiles listed next to
+// each other (neighbours, when the list is in spatial order) run on one XCD
+// at about the same time and share the lines at their edges in its L2. A
+// block whose range is exhausted claims from the others'. ctr[0..7] are the
+// ranges' counters and ctr[8] counts finished blocks; the last block resets
+// them all.
+template <typename F>
+__device__ inline void claim_items_xcd(unsigned long long* ctr, int64_t n_items, F&& body) {
+    __shared__ long long claimed;
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+    const int nr = (gridDim.x & 7u) == 0u ? 8 : 1;
+    const int own = nr == 8 ? (int)(blockIdx.x & 7u) : 0;
+    for (int t = 0; t < nr; ++t) {
+        const int j = (own + t) % nr;
+        const long long lo = (long long)j * n_items / nr, hi = (long long)(j + 1) * n_items / nr;
+        for (;;) {
+            if (tid == 0) claimed = lo + (long long)atomicAdd(ctr + j, 1ull);
+            __syncthreads();  // the claim is visible; the previous item's LDS use is done
+            const long long k = claimed;
+            __syncthreads();  // every thread has read the claim before thread 0 makes the next one
+            if (k >= hi) break;
+            body(k);
+        }
+    }
+    if (tid == 0) {
+        __threadfence();
+        if (atomicAdd(ctr + 8, 1ull) == (unsigned long long)gridDim.x - 1ull) {
+            for (int j = 0; j < 9; ++j) ctr[j] = 0ull;  // every block has made its last claim
+            __threadfence();
+        }
+    }
+}
+
 // Tile rows of the batched step: 8 output rows; the block has 8 + 3 rows of
 // threads so that a two-step pass (march2_step.h) fits, and single steps leave
 // the three extra rows to the barriers.
@@ -85,13 +120,13 @@ union BatchLds {
     float one[2][6][kBatchBY + 2][66];
 };
 
-// pass_state[0] is the global pass; items are claimed on pass_state[1..2].
+// pass_state[0] is the global pass; items are claimed on pass_state[1..9].
 __global__ void __launch_bounds__(64 * kBatchRows) batch_step_kernel(const sw_batch_solve* __restrict__ solves,
                                                                      const int4* __restrict__ items, int64_t n_items,
                                                                      long long* __restrict__ pass_state) {
     __shared__ BatchLds lds;
     const long long gp = pass_state[0];
-    claim_items((unsigned long long*)(pass_state + 1), n_items, [&](long long k) {
+    claim_items_xcd((unsigned long long*)(pass_state + 1), n_items, [&](long long k) {
         const int4 it = items[k];
         const sw_batch_solve& S = solves[it.x];
         const long long pass = gp - S.start;

@@ -56,9 +56,14 @@ def test_fill_xchunks_shortens_only_when_needed():
     assert n_step_items(two) >= 128 and max(s.xchunk for s in two) <= 96
 
 
-def test_step_items_longest_first():
+def test_step_items_orders():
     s = [SimpleNamespace(shape=(100, 16, 70), xchunk=64), SimpleNamespace(shape=(30, 8, 64), xchunk=64)]
-    items = step_items(s)
+    # spatial (the default): per solve, z tile fastest, then y tile, then x chunk
+    sp = step_items(s)
+    assert [tuple(r) for r in sp[:4]] == [(0, 0, 0, 0), (0, 1, 0, 0), (0, 0, 1, 0), (0, 1, 1, 0)]
+    assert tuple(sp[-1]) == (1, 0, 0, 0)
+    items = step_items(s, order="longest")
+    assert sorted(map(tuple, items)) == sorted(map(tuple, sp))
     assert items.shape == (2 * 2 * 2 + 1, 4)
     # the first item is a full 64-plane, 64-wide tile; the last a short, narrow one
     assert tuple(items[0][[0, 3]]) == (0, 0) and items[0][1] == 0
