@@ -47,8 +47,8 @@ def parse():
     ap.add_argument("--dtype", default="float32", choices=["float32", "float64"])
     ap.add_argument("--no-fp64", action="store_true", help="skip the float64 solve")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-steps", type=int, default=8,
-                    help="float32 oracle steps timed for cpu_baseline, after 2 warm-up steps (20 = the planned slice)")
+    ap.add_argument("--cpu-steps", type=int, default=20,
+                    help="float32 oracle steps timed for cpu_baseline, after 2 warm-up steps")
     ap.add_argument("--kernel-steps", type=int, default=200)
     ap.add_argument("--kernel", default="auto")
     ap.add_argument("--tile-y", type=int, default=0)
@@ -203,7 +203,19 @@ def cpu_projection_sample(w, iters, full_iters):
             "extrapolated_s": round(t1 + per_iter * (full_iters - 1), 1), "full_iters": full_iters}
 
 
-def cpu_baseline(w, n_steps=8, warm=2, n_steps64=3, proj_iters=3, full_iters=0):
+def cpu_model():
+    """The host CPU's model name (what lscpu prints), or the platform string."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def cpu_baseline(w, n_steps=20, warm=2, n_steps64=3, proj_iters=3, full_iters=0):
     """The reference's CPU path on the same 256^3 workload, one core.
 
     `value` is the oracle's float32 mode (the reference's own arithmetic) on a
@@ -231,7 +243,8 @@ def cpu_baseline(w, n_steps=8, warm=2, n_steps64=3, proj_iters=3, full_iters=0):
                     "sample": f"{n_steps64} steps after 1 warm-up step, float64 oracle"},
         "projection_sample": proj,
         "numpy": np.__version__,
-        "host": platform.processor() or platform.machine(),
+        "host": platform.machine(),
+        "cpu_model": cpu_model(),
         "os_cpu_count": os.cpu_count(),
     }
     if proj is not None:
@@ -744,7 +757,7 @@ def config1_cpu_vs_gpu():
     rec = {"workload": "plain shear_fdtd 64^3, nT=500, eta=1, CG tol 1e-10, snapshot stride 10",
            "dt": w["dt"], "dt_cfl_unstable": w["dt_cfl"],
            "cpu_oracle_fp64": {"seconds": round(t_cpu, 3), "gvox_steps_per_s": vox_steps / t_cpu / 1e9,
-                               "cores": 1, "numpy": np.__version__}}
+                               "cores": 1, "numpy": np.__version__, "cpu_model": cpu_model()}}
 
     def rel(a, b):
         return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))

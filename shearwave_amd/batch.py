@@ -1,10 +1,11 @@
 """Batched solves: many shear_fdtd problems of uneven sizes in one persistent launch per pass.
 
 :class:`Batch` keeps every solve's fields resident in HBM (float32 storage,
-increment form), projects all their forces with one batched CG
-(``sw_batch_project``), and advances all active solves with one persistent
-kernel launch per pass (``sw_batch_step``), optionally replayed from a
-captured hipGraph (``sw_batch_graph_create``). A solve's pass is two time
+increment form, each solve in the axis order that fills the tiles), projects
+each force with the spectral-preconditioned CG where its tolerance admits it
+and the rest with one batched CG (``sw_batch_project``), and advances all
+active solves with one persistent kernel launch per pass (``sw_batch_step``),
+replayed from a captured hipGraph (``sw_batch_graph_create``). A solve's pass is two time
 steps in one read of its state (the two-step tile), or one step, as its pass
 plan (:func:`pass_plan`) says. Work items are (solve, z tile,
 y tile, x chunk) quadruples; when solves finish, the item list is rebuilt so
