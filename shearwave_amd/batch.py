@@ -17,8 +17,10 @@ depend on the other solves in the batch.
 from __future__ import annotations
 
 import ctypes as C
+import functools
 import itertools
 import math
+import time
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -27,6 +29,7 @@ import torch
 from . import _abi
 from . import engine as E
 from ._abi import check, lib
+from .autotune import DEFAULT_CAP, balanced_chunk
 
 STEP_BY = 8  # output rows of a batched step tile (blocks of 64 x 11 threads)
 PASS_STATE = 16  # int64 of the batch's pass state: the global pass and the work-claim counters
@@ -233,32 +236,50 @@ def item_cost(x_planes, z_width):
     return (x_planes + 3) * (32 + z_width)
 
 
+@functools.lru_cache(maxsize=4096)
+def _tiles(nx, ny, nz, xc, by):
+    """(z tile, y tile, x chunk) of one grid, z fastest, then y, then x chunk (int32, read-only)."""
+    tx, ty, tz = np.meshgrid(np.arange(math.ceil(nx / xc)), np.arange(math.ceil(ny / by)),
+                             np.arange(math.ceil(nz / 64)), indexing="ij")
+    t = np.stack([tz.ravel(), ty.ravel(), tx.ravel()], axis=1).astype(np.int32)
+    t.flags.writeable = False
+    return t
+
+
 def step_items(solves, by=STEP_BY, order="spatial"):
     """Work items (solve, z tile, y tile, x chunk).
 
     ``"spatial"``: each solve's items with z fastest, then y, then x chunk,
     so that the step kernel's per-XCD ranges hold neighbouring tiles;
-    ``"longest"``: longest first (short ones fill in at the end of a pass)."""
-    items, cost = [], []
-    for s_idx, s in enumerate(solves):
-        nx, ny, nz = s.shape
-        xc = s.xchunk
-        tx, ty, tz = np.meshgrid(np.arange(math.ceil(nx / xc)), np.arange(math.ceil(ny / by)),
-                                 np.arange(math.ceil(nz / 64)), indexing="ij")
-        tx, ty, tz = tx.ravel(), ty.ravel(), tz.ravel()
-        items.append(np.stack([np.full_like(tx, s_idx), tz, ty, tx], axis=1))
-        cost.append(item_cost(np.minimum(xc, nx - tx * xc), np.minimum(64, nz - 64 * tz)))
-    if not items:
+    ``"longest"``: longest first (short ones fill in at the end of a pass);
+    ``"spatial_lpt"``: the spatial list cut into the kernel's 8 per-XCD
+    ranges, each range longest first (a stable sort, so equal items of a
+    solve keep their spatial order)."""
+    tiles = [_tiles(*s.shape, int(s.xchunk), int(by)) for s in solves]
+    if not tiles:
         return np.zeros((0, 4), dtype=np.int32)
-    items = np.concatenate(items).astype(np.int32)
+    items = np.empty((sum(t.shape[0] for t in tiles), 4), dtype=np.int32)
+    items[:, 0] = np.repeat(np.arange(len(tiles), dtype=np.int32), [t.shape[0] for t in tiles])
+    items[:, 1:] = np.concatenate(tiles)
     if order == "spatial":
         return items
-    return items[np.argsort(-np.concatenate(cost).astype(np.int64), kind="stable")]
+    xc = np.repeat(np.asarray([int(s.xchunk) for s in solves]), [t.shape[0] for t in tiles])
+    nx = np.repeat(np.asarray([s.shape[0] for s in solves]), [t.shape[0] for t in tiles])
+    nz = np.repeat(np.asarray([s.shape[2] for s in solves]), [t.shape[0] for t in tiles])
+    cost = item_cost(np.minimum(xc, nx - items[:, 3] * xc), np.minimum(64, nz - 64 * items[:, 1])).astype(np.int64)
+    if order == "spatial_lpt":
+        n = items.shape[0]
+        out = np.empty_like(items)
+        for j in range(8):
+            lo, hi = j * n // 8, (j + 1) * n // 8
+            out[lo:hi] = items[lo + np.argsort(-cost[lo:hi], kind="stable")]
+        return out
+    return items[np.argsort(-cost, kind="stable")]
 
 
 def default_xchunk(shape):
-    """x planes per batched work item when the autotuner has no entry for the grid class."""
-    return int(min(shape[0], 64))
+    """x planes per batched work item without a tuner: balanced chunks under the default cap."""
+    return balanced_chunk(shape[0], DEFAULT_CAP)
 
 
 def n_step_items(solves, by=STEP_BY):
@@ -286,8 +307,8 @@ def n_compute_units():
 
 
 def xchunk_for(s, tuner=None):
-    """x planes per step work item of a stored solve: its spec's choice, else the tuner's for its stored
-    shape, else the default."""
+    """x planes per step work item of a stored solve: its spec's choice, else balanced chunks under the
+    tuner's cap, else under the default cap."""
     if s.spec.xchunk:
         return int(s.spec.xchunk)
     return tuner.xchunk(s.shape) if tuner else default_xchunk(s.shape)
@@ -311,6 +332,10 @@ class Batch:
 
     # --------------------------------------------------------- descriptors
     def _desc_array(self, solves):
+        return E.upload_async(self._desc_bytes(solves))
+
+    def _desc_bytes(self, solves):
+        """The sw_batch_solve descriptors of `solves` as a host byte array."""
         arr = (_abi.BatchSolve * len(solves))()
         for k, s in enumerate(solves):
             sp = s.spec
@@ -334,7 +359,7 @@ class Batch:
             d.env = s.env.data_ptr()
             d.plan = s.plan.data_ptr()
             d.n_passes = s.n_passes
-        return E.upload_async(np.frombuffer(bytes(arr), dtype=np.uint8))
+        return np.frombuffer(bytes(arr), dtype=np.uint8)
 
     # ---------------------------------------------------------- projection
     def project(self, solves=None, xchunk=None, method="auto"):
@@ -431,61 +456,177 @@ class Batch:
         return info
 
     # ---------------------------------------------------------------- loop
-    def _advance(self, desc, items_t, n_items, n_passes, keep=None):
-        """Queue n_passes global passes over the items: whole graphs of graph_steps passes, replayed, then
-        direct launches for the rest. A graph is destroyed after a synchronisation: at once, or, when a list
-        `keep` is given, by the caller (which also keeps desc and items alive until then)."""
+    def _graph(self, desc_ptr, items_ptr, n_items):
+        """A captured graph of graph_steps passes over the items (the caller destroys it)."""
+        g = C.c_void_p()
+        check(lib().sw_batch_graph_create(desc_ptr, items_ptr, n_items, self.gpass.data_ptr(), self.tile_y, 0,
+                                          self.graph_steps, C.byref(g)))
+        return g
+
+    def _queue(self, desc_ptr, items_ptr, n_items, n_passes, g=None):
+        """Queue n_passes global passes over the items: replays of graph `g` (graph_steps passes each), then
+        direct launches for the rest."""
+        rest = n_passes
+        if g is not None:
+            reps = n_passes // self.graph_steps
+            for _ in range(reps):
+                check(lib().sw_graph_launch(g, E.stream_ptr()))
+            rest -= reps * self.graph_steps
+        if rest:
+            check(lib().sw_batch_step(desc_ptr, items_ptr, n_items, self.gpass.data_ptr(), self.tile_y, 0, rest,
+                                      E.stream_ptr()))
+
+    def _advance(self, desc, items_t, n_items, n_passes):
+        """Queue and finish n_passes global passes over the items (descriptor and item tensors)."""
         if n_passes <= 0:
             return
-        if self.graph_steps and n_passes >= self.graph_steps:
-            g = C.c_void_p()
-            check(lib().sw_batch_graph_create(desc.data_ptr(), items_t.data_ptr(), n_items, self.gpass.data_ptr(),
-                                              self.tile_y, 0, self.graph_steps, C.byref(g)))
-            try:
-                reps = n_passes // self.graph_steps
-                for _ in range(reps):
-                    check(lib().sw_graph_launch(g, E.stream_ptr()))
-                rest = n_passes - reps * self.graph_steps
-            finally:
-                if keep is None:
-                    torch.cuda.current_stream().synchronize()
-                    lib().sw_graph_destroy(g)
-                else:
-                    keep.append((g, desc, items_t))
-        else:
-            rest = n_passes
-        if rest:
-            check(lib().sw_batch_step(desc.data_ptr(), items_t.data_ptr(), n_items, self.gpass.data_ptr(), self.tile_y,
-                                      0, rest, E.stream_ptr()))
+        g = self._graph(desc.data_ptr(), items_t.data_ptr(), n_items) \
+            if self.graph_steps and n_passes >= self.graph_steps else None
+        try:
+            self._queue(desc.data_ptr(), items_t.data_ptr(), n_items, n_passes, g)
+        finally:
+            torch.cuda.current_stream().synchronize()
+            if g is not None:
+                lib().sw_graph_destroy(g)
 
-    def run(self):
+    def run(self, collect=False):
         """All solves start together; the item list shrinks as solves finish (x-chunks shorten when the
-        remaining solves are too few to occupy every CU with long ones)."""
+        remaining solves are too few to occupy every CU with long ones).
+
+        Every segment's descriptors, item list and graph are made before the
+        first pass is queued, in one upload: made segment by segment while
+        the persistent step kernel ran, these uploads and graph instantiations
+        slowed single passes 3.5x.
+
+        With ``collect`` every solve's (u, v) is exported after the last pass
+        and copied to pinned host memory in one stream of copies with a single
+        synchronisation; :meth:`result` then returns those arrays. The export
+        buffers (one device and one pinned allocation) are made while the
+        passes run.
+        """
+        self._collected = None
+        t0 = time.perf_counter()
+        t_buffers = 0.0
         for s in self.solves:
             s.start = 0
         self.gpass.zero_()
         ends = sorted(set(s.n_passes for s in self.solves))
         n_cu = n_compute_units()
         done = 0
-        graphs = []  # (graph, desc, items) of the segments, destroyed after the final synchronisation
+        segs, parts, off = [], [], 0  # (passes, descriptor offset, item offset, items) per segment
+
+        def put(a):
+            nonlocal off
+            b = np.frombuffer(np.ascontiguousarray(a).tobytes(), dtype=np.uint8)
+            pad = (-b.size) % 256
+            parts.append(np.concatenate([b, np.zeros(pad, np.uint8)]) if pad else b)
+            o, off = off, off + b.size + pad
+            return o
+
+        # the descriptors differ between segments only in their x chunks
+        base_desc = self._desc_bytes(self.solves)
+        size = C.sizeof(_abi.BatchSolve)
+        xoff = _abi.BatchSolve.xchunk.offset
         for end in ends:
             active = [k for k, s in enumerate(self.solves) if s.n_passes > done]
             act = [self.solves[k] for k in active]
             for s in act:
                 s.xchunk = s.base_xchunk
             fill_xchunks(act, n_cu)
-            desc = self._desc_array(self.solves)
             items = step_items(act, self.tile_y)
             # map local solve index back to the full descriptor array
             items[:, 0] = np.asarray(active, dtype=np.int32)[items[:, 0]]
-            items_t = E.upload_async(items.reshape(-1))
-            self._advance(desc, items_t, items.shape[0], end - done, keep=graphs)
-            graphs.append((None, desc, items_t))  # direct launches of the segment read them too
+            desc = base_desc.copy()
+            xc = desc.reshape(-1, size)[:, xoff:xoff + 4].view(np.int32)
+            xc[:, 0] = [s.xchunk for s in self.solves]
+            segs.append((end - done, put(desc), put(items.reshape(-1)), items.shape[0]))
             done = end
-        torch.cuda.current_stream().synchronize()
-        for g, _, _ in graphs:
-            if g is not None:
-                lib().sw_graph_destroy(g)
+        blob = E.upload_async(np.concatenate(parts))
+        base = blob.data_ptr()
+        graphs = [self._graph(base + d, base + i, n) if self.graph_steps and p >= self.graph_steps else None
+                  for p, d, i, n in segs]
+        main = torch.cuda.current_stream()
+        t_start, t_end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t_start.record(main)
+        try:
+            for (p, d, i, n), g in zip(segs, graphs):
+                self._queue(base + d, base + i, n, p, g)
+            t_end.record(main)
+            t_queued = time.perf_counter() - t0
+            if collect:
+                # allocated while the passes run (the host would otherwise idle): pinning host memory
+                # beside the persistent kernel measured no slower passes
+                exports = self._export_buffers()
+                t_buffers = time.perf_counter() - t0 - t_queued
+                # after the last pass: copies to the host running beside the persistent kernel slowed its
+                # passes up to 2.8x, more than they save
+                for k in range(len(self.solves)):
+                    self._export_into(k, exports[k])
+                for buf in exports:
+                    self._copy_out(buf)
+        finally:
+            main.synchronize()
+            for g in graphs:
+                if g is not None:
+                    lib().sw_graph_destroy(g)
+            del blob
+        self.last_loop_s = t_start.elapsed_time(t_end) / 1e3  # the passes on the main stream
+        self.last_run_timing = {"export_buffers_s": t_buffers, "queued_s": t_queued,
+                                "total_s": time.perf_counter() - t0}
+        if collect:
+            self._collected = [(e[4].numpy(), e[5].numpy()) for e in exports]
+
+    def _export_buffers(self):
+        """Per solve: stored-order AoS u and v (device), given-order copies when the solve is stored
+        permuted (device, else None), and pinned host arrays for the results; carved from one device and
+        one pinned allocation."""
+        sizes = [3 * int(np.prod(s.shape)) for s in self.solves]
+        n_dev = sum(2 * n * (2 if tuple(s.perm) != (0, 1, 2) else 1) for n, s in zip(sizes, self.solves))
+        dev_all = torch.empty(n_dev, dtype=torch.float32, device=E.device())
+        host_all = torch.empty(2 * sum(sizes), dtype=torch.float32, pin_memory=True)
+        out, od, oh = [], 0, 0
+        for n, s in zip(sizes, self.solves):
+            aos = s.shape + (3,)
+            given = tuple(s.given_shape) + (3,)
+
+            def take(buf, off, shape):
+                return buf[off:off + n].view(shape), off + n
+
+            ua, od = take(dev_all, od, aos)
+            va, od = take(dev_all, od, aos)
+            gu = gv = None
+            if tuple(s.perm) != (0, 1, 2):
+                gu, od = take(dev_all, od, given)
+                gv, od = take(dev_all, od, given)
+            hu, oh = take(host_all, oh, given)
+            hv, oh = take(host_all, oh, given)
+            out.append((ua, va, gu, gv, hu, hv))
+        return out
+
+    @staticmethod
+    def _copy_out(buf):
+        """Queue the copies of an exported (u, v) (of _export_buffers) to its pinned host arrays."""
+        ua, va, gu, gv, hu, hv = buf
+        if gu is not None:
+            ua, va = gu, gv
+        hu.copy_(ua, non_blocking=True)
+        hv.copy_(va, non_blocking=True)
+
+    def _export_into(self, k, buf):
+        """Queue the export of solve k's final (u, v) into the device arrays of `buf` (of _export_buffers)."""
+        s = self.solves[k]
+        ua, va, gu, gv, hu, hv = buf
+        par = s.n_passes & 1
+        u, d = s.u[par], s.d[par]
+        check(lib().sw_export_state(_abi.SW_F32, _abi.SW_FORM_INCREMENT, _abi.ptrs(u[0], u[1], u[2]),
+                                    _abi.ptrs(d[0], d[1], d[2]), 1.0 / float(s.spec.dt), ua.data_ptr(),
+                                    va.data_ptr(), int(np.prod(s.shape)), E.stream_ptr()))
+        if gu is not None:
+            inv = [int(i) for i in np.argsort(s.perm)]
+            for src, dst in ((ua, gu), (va, gv)):
+                view = src.permute(*inv, 3)
+                for m in range(3):
+                    dst[..., m].copy_(view[..., inv[m]])
 
     # ------------------------------------------------------------- results
     def state(self, k):
@@ -495,7 +636,10 @@ class Batch:
         return s.given_vec(s.u[par]), s.given_vec(s.d[par])
 
     def result(self, k, to_host=True):
-        """(u, v) of solve k in the reference's AoS (nx, ny, nz, 3) layout."""
+        """(u, v) of solve k in the reference's AoS (nx, ny, nz, 3) layout (host arrays collected by
+        ``run(collect=True)`` when there are any)."""
+        if to_host and getattr(self, "_collected", None) is not None:
+            return self._collected[k]
         s = self.solves[k]
         ua, va = s.export_aos(s.n_passes & 1)
         if to_host:

@@ -7,37 +7,49 @@ import textwrap
 
 import pytest
 
-from shearwave_amd.autotune import CANDIDATES, Tuner, grid_class
+from shearwave_amd.autotune import CAPS, Tuner, balanced_chunk, tune_mix_shapes
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def test_grid_class_buckets():
-    assert grid_class((64, 64, 64)) == (64, 8)
-    assert grid_class((65, 64, 64)) == (128, 8)
-    assert grid_class((192, 192, 256)) == (256, 128)
-    assert grid_class((100, 20, 70)) == (128, 8)
+def test_balanced_chunk():
+    assert balanced_chunk(96, 96) == 96
+    assert balanced_chunk(97, 96) == 49  # two equal chunks, not 96 + 1
+    assert balanced_chunk(150, 96) == 75
+    assert balanced_chunk(256, 96) == 86  # three chunks
+    assert balanced_chunk(40, 96) == 40  # never deeper than the grid
+    for nx in range(1, 300):
+        for cap in CAPS:
+            c = balanced_chunk(nx, cap)
+            n = -(-nx // c)
+            assert c <= max(cap, 1) and n == -(-nx // cap) and c * (n - 1) < nx  # fewest chunks, none empty
 
 
-def test_tuner_picks_fastest_and_caches(tmp_path):
+def test_tune_mix_is_fixed():
+    shapes = tune_mix_shapes()
+    assert shapes == tune_mix_shapes() and len(shapes) == 24
+    assert all(64 <= v <= 256 and v % 8 == 0 for sh in shapes for v in sh)
+    assert len(set(shapes)) > 20  # uneven grids
+
+
+def test_tuner_picks_fastest_cap_and_caches(tmp_path):
     calls = []
 
-    def fake(shape, xc):
-        calls.append((tuple(shape), xc))
-        return abs(xc - 48) + 1.0  # fastest at 48
+    def fake(caps):
+        calls.append(tuple(caps))
+        return {c: abs(c - 128) + 1.0 for c in caps}  # fastest at 128
 
     path = str(tmp_path / "t.json")
     t = Tuner(path=path, timer=fake, arch="gfx950:256")
-    assert t.xchunk((128, 96, 128)) == 48
-    n = len(calls)
-    assert n == len([c for c in CANDIDATES if c <= 128])
-    assert t.xchunk((120, 80, 110)) == 48  # same class: no new timing
-    assert len(calls) == n
+    assert t.cap() == 128 and calls == [CAPS]
+    assert t.xchunk((200, 64, 64)) == 100 and t.xchunk((100, 64, 64)) == 100
+    assert len(calls) == 1  # measured once
     t2 = Tuner(path=path, timer=fake, arch="gfx950:256")
-    assert t2.xchunk((128, 96, 128)) == 48 and len(calls) == n  # reloaded from disk
+    assert t2.cap() == 128 and len(calls) == 1  # reloaded from disk
     table = json.load(open(path))
-    assert any(k.startswith("gfx950:256|") for k in table)
-    assert t.xchunk((20, 20, 20)) <= 20  # never deeper than the grid
+    assert table["gfx950:256|batch_cap"]["cap"] == 128
+    t3 = Tuner(path=path, timer=fake, arch="gfx950:304")  # another device is tuned anew
+    assert t3.cap() == 128 and len(calls) == 2
 
 
 def test_c_struct_layouts_match_ctypes(tmp_path):

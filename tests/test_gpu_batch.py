@@ -96,17 +96,25 @@ def test_alone_equals_in_mix(sw, specs, mixed):
         np.testing.assert_array_equal(v, mixed[k][1])
 
 
-@pytest.mark.parametrize("graph_steps", [7, 16])
-def test_graph_replay_equals_direct(sw, specs, mixed, graph_steps):
+@pytest.mark.parametrize("graph_steps,collect", [(7, False), (16, True)])
+def test_graph_replay_equals_direct(sw, specs, mixed, graph_steps, collect):
+    """Graph replays equal direct launches bit for bit; with collect the results exported and copied to the
+    host in one go after the loop (permuted back to the given axis order there) are the same."""
     from shearwave_amd.batch import Batch
 
     b = Batch(specs, graph_steps=graph_steps)
     b.project()
-    b.run()
+    b.run(collect=collect)
     for k in range(len(specs)):
         u, v = b.result(k)
+        assert u.shape == specs[k].shape + (3,) and u.dtype == np.float32
         np.testing.assert_array_equal(u, mixed[k][0])
         np.testing.assert_array_equal(v, mixed[k][1])
+    if collect:
+        for k in range(len(specs)):
+            ud, vd = b.result(k, to_host=False)  # exported again, on the main stream
+            np.testing.assert_array_equal(ud.cpu().numpy(), b.result(k)[0])
+            np.testing.assert_array_equal(vd.cpu().numpy(), b.result(k)[1])
 
 
 def test_axis_order_changes_only_rounding(sw, specs, mixed):
@@ -239,3 +247,15 @@ def test_projection_methods_agree(sw, specs):
     assert max(got["each"][0]) <= 12 < min(got["batched"][0])
     for a, c in zip(got["each"][1], got["batched"][1]):
         assert rel(a, c) < 1e-7
+
+
+def test_tuner_times_the_mix(sw, tmp_path):
+    """The autotuner's measurement on the device: every cap timed on the tuning mix, the fastest cached."""
+    from shearwave_amd.autotune import CAPS, Tuner
+
+    t = Tuner(path=str(tmp_path / "tune.json"))
+    cap = t.cap()
+    entry = t._table[t.key()]
+    assert cap in CAPS and set(entry["ps_per_voxel_step"]) == {str(c) for c in CAPS}
+    times = {int(k): v for k, v in entry["ps_per_voxel_step"].items()}
+    assert all(v > 0 for v in times.values()) and times[cap] == min(times.values())
