@@ -335,6 +335,17 @@ int sw_adjoint_step(int dtype, const void* const* u, const void* const* d, const
 int sw_inject_traces(int dtype, void* f, int64_t n, const int64_t* idx, int64_t count, const void* grad,
                      int64_t stride, void* stream);
 
+/* One two-step pass of sw_fdtd_run's update (increment form; steps with
+ * env2[0] and env2[1], a host array of two doubles or NULL for zero) from
+ * (u, b) into (u_next, b_next), that also writes the state after the first
+ * step into (u_mid, d_mid): the values a single step writes, bit for bit. The
+ * state's buffers are not rotated. For the adjoint's recomputation of the
+ * forward states of a segment (the forward loop, solver.py:194-234, runs such
+ * passes). No reference counterpart. SW_ERR_UNSUPPORTED on grids the
+ * x-marching pass does not take (nz < 32, ny < 8, nx < 8). */
+int sw_fdtd_step2_mid(const sw_fdtd_state* st, const double* env2, void* const* u_mid, void* const* d_mid,
+                      void* stream);
+
 /* ------------------------------------------------------------------ strain */
 
 /* compute_strain_tensor (strain.py:16-70): Voigt [exx,eyy,ezz,gyz,gxz,gxy],

@@ -16,6 +16,7 @@ LIB_PATH = os.environ.get("SHEARWAVE_LIB") or os.path.join(os.path.dirname(os.pa
                                                              "libshearwave_hip.so")
 
 SW_OK = 0
+SW_ERR_UNSUPPORTED = 4
 SW_F32, SW_F64 = 0, 1
 SW_FORM_INCREMENT, SW_FORM_LEAPFROG = 0, 1
 SW_KERNEL_AUTO, SW_KERNEL_DIRECT, SW_KERNEL_MARCH, SW_KERNEL_MARCH2, SW_KERNEL_MARCH3 = 0, 1, 2, 3, 4
@@ -194,6 +195,7 @@ SIGNATURES = {
     "sw_adjoint_step": (_i32, [_i32, C.POINTER(_p), C.POINTER(_p), C.POINTER(_p), C.POINTER(_p), C.POINTER(_p),
                                C.POINTER(_p), _p, _p, _p, _p, _i64, _i64, _i64, _d, _d, _d, _d, _d, _p]),
     "sw_inject_traces": (_i32, [_i32, _p, _i64, _p, _i64, _p, _i64, _p]),
+    "sw_fdtd_step2_mid": (_i32, [C.POINTER(FdtdState), C.POINTER(_d), _p, _p, _p]),
 }
 
 _lock = threading.Lock()

@@ -5,9 +5,11 @@ projected force; its outputs are u_z after every step at a set of sensor
 voxels (a plane for elastography). The adjoint is a hand-written kernel
 (``sw_adjoint_step``); memory is bounded by sqrt(nT) checkpointing: the
 forward pass keeps (u, d) every K ~ sqrt(nT) steps, and the backward pass
-recomputes each segment from its checkpoint (bit-identical, the step kernel is
-deterministic) before sweeping the adjoint back through it. The gradient sums
-accumulate in float64.
+recomputes each segment from its checkpoint before sweeping the adjoint back
+through it. The recomputation pairs steps into two-step passes as the forward
+run does (``sw_fdtd_step2_mid`` also writes each pass's intermediate state),
+so every recomputed state equals the forward run's bit for bit. The gradient
+sums accumulate in float64.
 
 :class:`SensorTraces` is a ``torch.autograd.Function``: ``traces =
 SensorTraces.apply(mu, eta, problem)``, then any loss of ``traces`` can be
@@ -117,12 +119,19 @@ class Problem:
         for seg in range(len(ckpts) - 1, -1, -1):
             n0 = seg * self.K
             n1 = min(n0 + self.K, N)
-            # recompute the forward states u_n, d_n for n = n0 .. n1-1
+            # recompute the forward states u_n, d_n for n = n0 .. n1-1, in two-step passes paired as the
+            # forward run paired them (from the checkpoint); each pass also writes its intermediate state
             seg_u[0].copy_(ckpts[seg][0])
             seg_d[0].copy_(ckpts[seg][1])
-            for n in range(n0, n1 - 1):
+            n = n0
+            while n < n1 - 1:
                 k = n - n0
-                s.step_into(seg_u[k], seg_d[k], seg_u[k + 1], seg_d[k + 1], self.env[n])
+                if n + 2 <= n1 - 1 and s.step2_into(seg_u[k], seg_d[k], seg_u[k + 1], seg_d[k + 1], seg_u[k + 2],
+                                                    seg_d[k + 2], self.env[n:n + 2]):
+                    n += 2
+                else:
+                    s.step_into(seg_u[k], seg_d[k], seg_u[k + 1], seg_d[k + 1], self.env[n])
+                    n += 1
             for n in range(n1 - 1, n0 - 1, -1):
                 u_n, d_n = seg_u[n - n0], seg_d[n - n0]
                 nxt = 1 - cur

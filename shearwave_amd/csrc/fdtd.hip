@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(BZ * BY) step_march_kernel(StepArgs<T> a, int 
     march_tile<T, INCR, VISC, FORCE, BY, BZ>(a, x0, min(x0 + xchunk, a.g.nx), tyi * BY, tzi * BZ, lds);
 }
 
-template <typename T, bool VISC, bool FORCE, int BY>
+template <typename T, bool VISC, bool FORCE, int BY, bool MID = false>
 __global__ void __launch_bounds__(64 * (BY + 3)) step2_march_kernel(StepArgs<T> a, int xchunk, int nty, int ntz) {
     __shared__ Step2Lds<T, VISC, BY> lds;
     const unsigned nb = gridDim.x;
@@ -101,7 +101,7 @@ __global__ void __launch_bounds__(64 * (BY + 3)) step2_march_kernel(StepArgs<T> 
     const int tyi = (int)(rest % (unsigned)nty);
     const int txi = (int)(rest / (unsigned)nty);
     const int x0 = txi * xchunk;
-    march2_tile<T, VISC, FORCE, BY>(a, x0, min(x0 + xchunk, a.g.nx), tyi * BY, tzi * 64, lds);
+    march2_tile<T, VISC, FORCE, BY, MID>(a, x0, min(x0 + xchunk, a.g.nx), tyi * BY, tzi * 64, lds);
 }
 
 // SW_STEP3_BY: rows of the three-step tile (block of 64 x (BY + 5) threads).
@@ -194,9 +194,9 @@ int launch_step(const StepArgs<T>& a, int kernel, int tile_y, int tile_z, int x_
     return SW_OK;
 }
 
-template <typename T, bool VISC, bool FORCE, int BY>
+template <typename T, bool VISC, bool FORCE, int BY, bool MID = false>
 void launch_march2(const StepArgs<T>& a, int x_chunk, hipStream_t s) {
-    auto k = step2_march_kernel<T, VISC, FORCE, BY>;
+    auto k = step2_march_kernel<T, VISC, FORCE, BY, MID>;
     static const int res = resident_blocks(k, 64 * (BY + 3));
     const MarchShape ms = march_shape(a.g, BY, 64, x_chunk, res);
     hipLaunchKernelGGL(k, dim3(ms.blocks()), dim3(64, BY + 3), 0, s, a, ms.xchunk, ms.nty, ms.ntz);
@@ -206,14 +206,14 @@ void launch_march2(const StepArgs<T>& a, int x_chunk, hipStream_t s) {
 // force, when either step applies it, is scaled by a.fs and a.fs2. Tile rows:
 // float32 8 (default) or 12 (force-free passes; forced ones use 8); float64 4
 // (a 7-wave block leaves each wave the registers the float64 queues need).
-template <typename T, bool VISC, bool FORCE>
+template <typename T, bool VISC, bool FORCE, bool MID = false>
 int launch_step2_rows(const StepArgs<T>& a, int tile_y, int x_chunk, hipStream_t s) {
     if constexpr (DType<T>::code == SW_F32) {
         const int by = tile_y > 0 ? tile_y : 8;
-        if (by == 8 || (by == 12 && FORCE)) {
-            launch_march2<T, VISC, FORCE, 8>(a, x_chunk, s);
+        if (by == 8 || (by == 12 && (FORCE || MID))) {
+            launch_march2<T, VISC, FORCE, 8, MID>(a, x_chunk, s);
         } else if (by == 12) {
-            if constexpr (!FORCE) launch_march2<T, VISC, false, 12>(a, x_chunk, s);
+            if constexpr (!FORCE && !MID) launch_march2<T, VISC, false, 12>(a, x_chunk, s);
         } else {
             sw::set_error("two-step pass: float32 tile_y %d not supported (8 or 12)", by);
             return SW_ERR_ARG;
@@ -224,7 +224,7 @@ int launch_step2_rows(const StepArgs<T>& a, int tile_y, int x_chunk, hipStream_t
             sw::set_error("two-step pass: float64 tile_y %d not supported (4)", by);
             return SW_ERR_ARG;
         }
-        launch_march2<T, VISC, FORCE, 4>(a, x_chunk, s);
+        launch_march2<T, VISC, FORCE, 4, MID>(a, x_chunk, s);
     }
     SW_LAUNCH_CHECK();
     return SW_OK;
@@ -254,12 +254,12 @@ int launch_step3(const StepArgs<T>& a, bool visc, int x_chunk, hipStream_t s) {
     }
 }
 
-template <typename T>
+template <typename T, bool MID = false>
 int launch_step2(const StepArgs<T>& a, bool visc, bool force, int tile_y, int x_chunk, hipStream_t s) {
-    if (visc) return force ? launch_step2_rows<T, true, true>(a, tile_y, x_chunk, s)
-                           : launch_step2_rows<T, true, false>(a, tile_y, x_chunk, s);
-    return force ? launch_step2_rows<T, false, true>(a, tile_y, x_chunk, s)
-                 : launch_step2_rows<T, false, false>(a, tile_y, x_chunk, s);
+    if (visc) return force ? launch_step2_rows<T, true, true, MID>(a, tile_y, x_chunk, s)
+                           : launch_step2_rows<T, true, false, MID>(a, tile_y, x_chunk, s);
+    return force ? launch_step2_rows<T, false, true, MID>(a, tile_y, x_chunk, s)
+                 : launch_step2_rows<T, false, false, MID>(a, tile_y, x_chunk, s);
 }
 
 template <typename T>
@@ -307,6 +307,25 @@ __global__ void __launch_bounds__(256) finite_check_kernel(const T* __restrict__
     if (__any(bad) && (threadIdx.x & (kWave - 1)) == 0) atomicCAS((unsigned long long*)flag, ~0ull, (unsigned long long)step);
 }
 
+// The coefficients, force and step factors of a state (the buffers are set per pass).
+template <typename T>
+StepArgs<T> step_args(const sw_fdtd_state* st) {
+    StepArgs<T> a{};
+    a.m = (const T*)st->mu_over_rho;
+    a.e = (const T*)st->eta_over_rho;
+    for (int c = 0; c < 3; ++c) a.f[c] = (const T*)st->force[c];
+    a.g = Grid{(int)st->nx, (int)st->ny, (int)st->nz};
+    const double dt = st->dt;
+    a.cx = (T)(dt * dt / (st->dx * st->dx));
+    a.cy = (T)(dt * dt / (st->dy * st->dy));
+    a.cz = (T)(dt * dt / (st->dz * st->dz));
+    a.vx = (T)(dt / (st->dx * st->dx));
+    a.vy = (T)(dt / (st->dy * st->dy));
+    a.vz = (T)(dt / (st->dz * st->dz));
+    a.damp = (T)(1.0 - st->vel_damp);
+    return a;
+}
+
 template <typename T>
 int run_steps(sw_fdtd_state* st, int64_t n_steps, const double* env, const sw_fdtd_outputs* out, hipStream_t s) {
     const Grid g{(int)st->nx, (int)st->ny, (int)st->nz};
@@ -338,19 +357,7 @@ int run_steps(sw_fdtd_state* st, int64_t n_steps, const double* env, const sw_fd
     };
     const bool traces = out && out->n_traces > 0;
 
-    StepArgs<T> a{};
-    a.m = (const T*)st->mu_over_rho;
-    a.e = (const T*)st->eta_over_rho;
-    for (int c = 0; c < 3; ++c) a.f[c] = (const T*)st->force[c];
-    a.g = g;
-    const double dt = st->dt;
-    a.cx = (T)(dt * dt / (st->dx * st->dx));
-    a.cy = (T)(dt * dt / (st->dy * st->dy));
-    a.cz = (T)(dt * dt / (st->dz * st->dz));
-    a.vx = (T)(dt / (st->dx * st->dx));
-    a.vy = (T)(dt / (st->dy * st->dy));
-    a.vz = (T)(dt / (st->dz * st->dz));
-    a.damp = (T)(1.0 - st->vel_damp);
+    StepArgs<T> a = step_args<T>(st);
 
     for (int64_t it = 0; it < n_steps;) {
         const double fs = env ? env[it] : 0.0;
@@ -421,10 +428,55 @@ int run_steps(sw_fdtd_state* st, int64_t n_steps, const double* env, const sw_fd
     return SW_OK;
 }
 
+// One two-step pass from (u, b) into (u_next, b_next) that also stores its
+// intermediate state into (u_mid, d_mid); the state's buffers are not rotated.
+template <typename T>
+int run_step2_mid(const sw_fdtd_state* st, const double* env2, void* const* u_mid, void* const* d_mid,
+                  hipStream_t s) {
+    StepArgs<T> a = step_args<T>(st);
+    const double fs = env2 ? env2[0] : 0.0, fs2 = env2 ? env2[1] : 0.0;
+    const bool force = fs != 0.0 || fs2 != 0.0;
+    SW_REQUIRE(!force || (st->force[0] && st->force[1] && st->force[2]), SW_ERR_ARG,
+               "env != 0 but the state has no force");
+    a.fs = (T)fs;
+    a.fs2 = (T)fs2;
+    for (int c = 0; c < 3; ++c) {
+        a.A[c] = (const T*)st->u[c];
+        a.B[c] = (const T*)st->b[c];
+        a.Ao[c] = (T*)st->u_next[c];
+        a.Bo[c] = (T*)st->b_next[c];
+        a.Am[c] = (T*)u_mid[c];
+        a.Bm[c] = (T*)d_mid[c];
+    }
+    return launch_step2<T, true>(a, st->eta_over_rho != nullptr, force, st->tile_y, st->x_chunk, s);
+}
+
 }  // namespace
 }  // namespace sw
 
 using namespace sw;
+
+extern "C" int sw_fdtd_step2_mid(const sw_fdtd_state* st, const double* env2, void* const* u_mid,
+                                 void* const* d_mid, void* stream) {
+    SW_REQUIRE(st, SW_ERR_ARG, "null state");
+    int rc = check_dtype(st->dtype);
+    if (rc) return rc;
+    if ((rc = check_grid(st->nx, st->ny, st->nz))) return rc;
+    SW_REQUIRE(st->form == SW_FORM_INCREMENT, SW_ERR_ARG, "the two-step pass needs the increment form");
+    SW_REQUIRE(st->dt > 0 && st->dx > 0 && st->dy > 0 && st->dz > 0, SW_ERR_ARG, "dt and spacing must be > 0");
+    SW_REQUIRE(st->mu_over_rho, SW_ERR_ARG, "mu_over_rho is null");
+    SW_REQUIRE(u_mid && d_mid, SW_ERR_ARG, "null intermediate state");
+    for (int c = 0; c < 3; ++c)
+        SW_REQUIRE(st->u[c] && st->b[c] && st->u_next[c] && st->b_next[c] && u_mid[c] && d_mid[c], SW_ERR_ARG,
+                   "null state buffer");
+    const Grid g{(int)st->nx, (int)st->ny, (int)st->nz};
+    SW_REQUIRE(g.nz >= 32 && g.ny >= 8 && g.nx >= 8 && g.plane() < kMaxMarchPlane, SW_ERR_UNSUPPORTED,
+               "the two-step pass needs nz >= 32, ny >= 8, nx >= 8 and a y-z plane of fewer than %lld voxels",
+               (long long)kMaxMarchPlane);
+    hipStream_t s = (hipStream_t)stream;
+    if (st->dtype == SW_F32) return run_step2_mid<float>(st, env2, u_mid, d_mid, s);
+    return run_step2_mid<double>(st, env2, u_mid, d_mid, s);
+}
 
 extern "C" int sw_fdtd_run(sw_fdtd_state* st, int64_t n_steps, const double* env, const sw_fdtd_outputs* out,
                            void* stream) {

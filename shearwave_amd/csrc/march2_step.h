@@ -95,7 +95,9 @@ __device__ __forceinline__ void stage(Plane& L, int r, int q, const Col<T>& c) {
     }
 }
 
-template <typename T, bool VISC, bool FORCE, int BY>
+// With MID the pass also stores step n+1 of its own planes (u_{n+1} = u_n +
+// d_{n+1} and d_{n+1}, the values two single steps store) into a.Am, a.Bm.
+template <typename T, bool VISC, bool FORCE, int BY, bool MID = false>
 __device__ __forceinline__ void march2_tile(const StepArgs<T>& a, int x0, int x1, int y0, int z0,
                                             Step2Lds<T, VISC, BY>& lds) {
     static_assert(2 * BY + 4 <= 64 && BY + 3 <= 16, "tile rows");
@@ -195,6 +197,17 @@ __device__ __forceinline__ void march2_tile(const StepArgs<T>& a, int x0, int x1
             n.d2 = in ? i2 : T(0);
             s[K & 3] = n;
             if (prim) stage<T, VISC>(S1, r1, q1, n);
+            if constexpr (MID) {
+                if (out_thread && p_in && x >= x0 && x < x1) {
+                    const int64_t pxm = (int64_t)x * P;
+                    store_out(&at_byte(a.Am[0] + pxm, pyz), n.u0);
+                    store_out(&at_byte(a.Am[1] + pxm, pyz), n.u1);
+                    store_out(&at_byte(a.Am[2] + pxm, pyz), n.u2);
+                    store_out(&at_byte(a.Bm[0] + pxm, pyz), n.d0);
+                    store_out(&at_byte(a.Bm[1] + pxm, pyz), n.d1);
+                    store_out(&at_byte(a.Bm[2] + pxm, pyz), n.d2);
+                }
+            }
         }
         // step n+2 at plane x-1 (staged in the other step-n+1 buffer one plane ago)
         if (out_thread && x - 1 >= x0 && p_in) {

@@ -24,6 +24,8 @@ struct StepArgs {
     T damp;        // 1 - vel_damp
     T fs;          // env[it]
     T fs2;         // env[it+1] (the second step of a two-step pass)
+    T* Am[3];      // u_{n+1} of a two-step pass (its intermediate state, when asked for)
+    T* Bm[3];      // d_{n+1} of a two-step pass
 };
 
 // One (y, z) column's state at one x plane: u and d = u - u_prev.

@@ -328,6 +328,34 @@ class Solve:
                 for c in range(3):
                     arr[c] = vals[c]
 
+    def step2_into(self, u_src, d_src, u_mid, d_mid, u_dst, d_dst, env_pair=None):
+        """Two steps from (u_src, d_src) into (u_dst, d_dst) in one two-step pass that also writes the state
+        after the first step into (u_mid, d_mid), bit for bit what step_into writes (the adjoint's
+        recomputation). Returns False, doing nothing, on grids the two-step pass does not take."""
+        assert self.form == "increment"
+        st = self.state
+        keep = [list(getattr(st, f)) for f in ("u", "b", "u_next", "b_next")]
+        try:
+            for c in range(3):
+                st.u[c] = u_src[c].data_ptr()
+                st.b[c] = d_src[c].data_ptr()
+                st.u_next[c] = u_dst[c].data_ptr()
+                st.b_next[c] = d_dst[c].data_ptr()
+            env_arr = None
+            if env_pair is not None:
+                env_np = np.asarray(env_pair, dtype=np.float64).reshape(2)
+                env_arr = env_np.ctypes.data_as(C.POINTER(C.c_double))
+            rc = lib().sw_fdtd_step2_mid(C.byref(st), env_arr, _abi.ptrs(*u_mid), _abi.ptrs(*d_mid), stream_ptr())
+            if rc == _abi.SW_ERR_UNSUPPORTED:
+                return False
+            check(rc)
+            return True
+        finally:
+            for f, vals in zip(("u", "b", "u_next", "b_next"), keep):
+                arr = getattr(st, f)
+                for c in range(3):
+                    arr[c] = vals[c]
+
     # --------------------------------------------------------------- access
     def _vec(self, ptr_array):
         return [self._by_ptr[ptr_array[c]] for c in range(3)]

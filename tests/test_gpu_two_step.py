@@ -196,3 +196,33 @@ def test_two_step_around_force_and_outputs(sw, dtype):
     assert len(got) == 4  # u, v, snapshots, strain dict
     for a, b in zip(_flat(got), _flat(auto)):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("dtype,shape,viscous,force", [("float32", (19, 13, 70), True, False),
+                                                        ("float32", (24, 16, 64), False, True),
+                                                        ("float64", (17, 11, 45), True, True)])
+def test_pass_with_intermediate_state(sw, dtype, shape, viscous, force):
+    """sw_fdtd_step2_mid: the pass's final state equals a two-step pass of sw_fdtd_run bit for bit, and the
+    intermediate state it writes equals one single step (the adjoint's recomputation relies on both)."""
+    from shearwave_amd import engine as E
+
+    s = _solve(sw, shape, 7, "auto", viscous, dtype=dtype)
+    if force:
+        rng = np.random.default_rng(3)
+        s.set_force(None, None, rng.standard_normal(shape), 1e-10, 5000)
+    env = np.array([0.7, 0.3]) if force else np.zeros(2)
+    u0, d0 = torch.stack(s.u_components()).clone(), torch.stack(s.b_components()).clone()
+    um, dm, u2, d2 = (torch.full_like(u0, float("nan")) for _ in range(4))
+    assert s.step2_into(u0, d0, um, dm, u2, d2, env)
+    u1, d1 = torch.empty_like(u0), torch.empty_like(u0)
+    s.step_into(u0, d0, u1, d1, env[0])  # one single step (kernel "march" for single steps of the auto kernel)
+    assert torch.equal(um, u1) and torch.equal(dm, d1)
+    s.run(2, env)  # one two-step pass of the loop from the same state
+    assert torch.equal(u2, torch.stack(s.u_components())) and torch.equal(d2, torch.stack(s.b_components()))
+    assert float(d2.abs().max()) > 0 and float(dm.abs().max()) > 0
+    # grids the pass does not take: nothing is written
+    small = _solve(sw, (10, 6, 20), 7, "auto", viscous, dtype=dtype)
+    v0 = torch.stack(small.u_components())
+    z = torch.zeros_like(v0)
+    assert not small.step2_into(v0, v0, z, z, z, z)
+    assert float(z.abs().max()) == 0
