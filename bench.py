@@ -11,10 +11,17 @@ The JSON line also carries the roofline of the dominant kernel (in float32
 the two-step pass, which advances two force-free viscous steps per launch;
 timed with HIP events on the stream it runs on; algorithmic bytes per launch
 = 44 B per voxel-step, the reference's (u, u_prev) traffic of one step, x 2
-steps x voxels) and a CPU baseline (the NumPy oracle in float64 on a bounded
-sample of the same loop, one core).
+steps x voxels) and a CPU baseline (the NumPy oracle's float32 mode, the
+reference's own arithmetic, on a bounded sample of the same loop, one core,
+with the float64 oracle beside it). Further records: the projection by the
+preconditioned and the plain CG, the float64 form of the solve, the solve
+through the drop-in API from host arrays, the 64^3 CPU-runnable case, the
+48-solve batch, serving at six arrival rates, the gradient workload, the
+radiation force and acoustic strain, and the static solve.
 
-    python bench.py [--steps K] [--warmup W] [--n 256] [--nt 2000] [--no-cpu] [--fp64]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--nt 2000] [--no-cpu] [--no-fp64]
+                    [--no-batch] [--no-serve] [--no-adjoint] [--no-radforce] [--no-static]
+                    [--only radforce|static|batch|serve|adjoint] [--config1]
 """
 
 from __future__ import annotations
