@@ -154,24 +154,27 @@ class _Solve:
         rho64 = up(spec.rho, "float64")
         mu64 = up(spec.mu, "float64")
         eta64 = None if spec.eta is None else up(spec.eta, "float64")
-        self.m = torch.empty(shape, dtype=torch.float32, device=dev)
-        self.e = None if eta64 is None else torch.empty(shape, dtype=torch.float32, device=dev)
-        self.inv_rho = torch.empty(shape, dtype=torch.float32, device=dev)
+        # every per-voxel field in one allocation, staggered across the L2 channels (engine.FieldArena):
+        # mu/rho, eta/rho, 1/rho, the two (u, d) parities, the force pattern and the loop force
+        arena = E.FieldArena(shape, torch.float32, 3 + 12 + 6)
+        self.m = arena.take()
+        self.e = None if eta64 is None else arena.take()
+        self.inv_rho = arena.take()
         check(lib().sw_prepare_coefficients(
             _abi.SW_F32, rho64.data_ptr(), mu64.data_ptr(), None if eta64 is None else eta64.data_ptr(),
             self.m.data_ptr(), None if self.e is None else self.e.data_ptr(), self.inv_rho.data_ptr(),
             int(np.prod(shape)), st))
         del rho64, mu64, eta64
-        self.u = torch.zeros((2, 3) + shape, dtype=torch.float32, device=dev)
-        self.d = torch.zeros((2, 3) + shape, dtype=torch.float32, device=dev)
-        self.f = torch.empty((3,) + shape, dtype=torch.float32, device=dev)
+        self.u = arena.take((2, 3))
+        self.d = arena.take((2, 3))
+        self.f = arena.take((3,))
         for c in range(3):
             comp = spec.force[self.perm[c]]  # the component along stored axis c
             if comp is None:
                 self.f[c].zero_()
             else:
                 self.f[c].copy_(up(comp, "float32"))
-        self.force = torch.empty_like(self.f)
+        self.force = arena.take((3,))
         env = np.zeros(spec.n_steps) if spec.env is None else np.asarray(spec.env, dtype=np.float64)
         if env.shape[0] != spec.n_steps:
             raise ValueError("env length must equal n_steps")
