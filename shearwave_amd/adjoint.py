@@ -96,11 +96,14 @@ class Problem:
         dev = self.rho.device
         tdt = E.torch_dtype(self.dtype)
         g = torch.as_tensor(gtraces, device=dev).to(tdt).contiguous()
-        vec = (3,) + self.shape
-        ub = [torch.zeros(vec, dtype=tdt, device=dev) for _ in range(2)]
-        db = [torch.zeros(vec, dtype=tdt, device=dev) for _ in range(2)]
-        gm = torch.zeros(self.shape, dtype=torch.float64, device=dev)
-        ge = torch.zeros(self.shape, dtype=torch.float64, device=dev) if self.viscous else None
+        # the adjoint states, the recomputed forward states of a segment and the gradient sums, each set
+        # carved from one allocation with its fields staggered across the L2 channels (E.FieldArena)
+        adj = E.FieldArena(self.shape, tdt, 12)
+        ub = [adj.take((3,)) for _ in range(2)]
+        db = [adj.take((3,)) for _ in range(2)]
+        sums = E.FieldArena(self.shape, torch.float64, 2)
+        gm = sums.take()
+        ge = sums.take() if self.viscous else None
         st = E.stream_ptr()
         n_sens = self.sensors.numel()
         N = self.n_steps
@@ -114,8 +117,9 @@ class Problem:
         cur = 0
         inject(ub[cur], N)
         # the forward states of one segment, each step written into its own slot
-        seg_u = torch.empty((self.K,) + vec, dtype=tdt, device=dev)
-        seg_d = torch.empty((self.K,) + vec, dtype=tdt, device=dev)
+        segs = E.FieldArena(self.shape, tdt, 6 * self.K)
+        seg_u = segs.take((self.K, 3))
+        seg_d = segs.take((self.K, 3))
         for seg in range(len(ckpts) - 1, -1, -1):
             n0 = seg * self.K
             n1 = min(n0 + self.K, N)
