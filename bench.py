@@ -44,6 +44,14 @@ METRIC = "Gvoxel-steps/s & %HBM roofline; batched solves/s + p50/p99 at 1e-5 rel
 HBM_PEAK_GBS = 8000.0
 
 
+_T0 = time.perf_counter()
+
+
+def progress(msg):
+    """One line on stderr as a section of the run ends (the JSON record goes to stdout at the end)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,11 +106,22 @@ class DropIn:
         self.E = E
         self.solve = E.Solve(w["shape"], (w["h"],) * 3, w["dt"], w["rho"], w["mu"], w["eta"], dtype=dtype,
                              form="increment", kernel=kernel, tile_y=tile_y, x_chunk=x_chunk)
+        # as the drop-in API does: with no tile shape given, the autotuner's (measured here, before timing)
+        self.tuned = None
+        if kernel == "auto" and not tile_y and not x_chunk and self.solve.two_step_fits():
+            self.tuned = self.solve.autotune()
         self.bz = E.to_device(w["bz"], dtype, w["shape"])
         self.n_inv = w["n_steps"] // w["strain_stride"]
         self.inv = torch.empty((self.n_inv, 4) + tuple(w["shape"]), dtype=E.torch_dtype(dtype), device=E.device())
         self.env = w["env"]
         self.last_cg = None
+
+    def solve_tuner_entry(self):
+        from shearwave_amd.autotune import default_tuner
+
+        t = default_tuner()
+        s = self.solve
+        return t.entry(t.pass_key(s.dtype, s.e is not None, s.shape))
 
     def run(self):
         s = self.solve
@@ -111,6 +130,26 @@ class DropIn:
         info, _ = s.set_force(None, None, self.bz, self.w["tol"], self.w["max_iters"])
         self.last_cg = info
         s.run(self.w["n_steps"], self.env, inv_stride=self.w["strain_stride"], inv_out=self.inv)
+
+
+def pretune_dropin(specs, dtype="float32"):
+    """The drop-in autotuner's launch shape for every grid class of `specs` that shear_fdtd would tune,
+    measured before a timed run (as the batch tuner is), so that no timed plain solve pays a first
+    measurement. Materials do not change the timing; the tuner keys on storage type, viscosity and shape."""
+    from shearwave_amd import api
+    from shearwave_amd import engine as E
+    from shearwave_amd.autotune import default_tuner
+
+    tuner = default_tuner()
+    for k, sp in enumerate(specs):
+        if sp.n_steps * sp.voxels >= api.AUTOTUNE_MIN_VOXEL_STEPS:
+            inviscid = sp.eta is None or (np.isscalar(sp.eta) and float(sp.eta) == 0.0)
+            s = E.Solve(sp.shape, sp.h, sp.dt, 1000.0, 3000.0, None if inviscid else 1.0, dtype=dtype)
+            if s.two_step_fits():
+                s.autotune(tuner)
+            del s
+        if (k + 1) % 25 == 0:
+            progress(f"drop-in launch shapes: {k + 1} of {len(specs)} solves")
 
 
 def time_step_kernel(solve, n_launch, steps_per_launch):
@@ -303,7 +342,9 @@ def bench_batch(n_solves=48, check=8):
     specs = W.batch_48(n_solves=n_solves)
     vox_steps = sum(sp.voxels * sp.n_steps for sp in specs)
     tuner = default_tuner()
-    cap = tuner.cap()  # tuned (or read from the cache) before timing
+    tile_y, cap = tuner.batch_shape()  # tuned (or read from the cache) before timing
+    pretune_dropin(specs)
+    progress("batch: launch shapes tuned")
 
     def plain(sp, dtype="float32"):
         opts = {"env_t": sp.env, "eta": sp.eta, "vel_damp": sp.vel_damp, "poisson_tol": sp.tol,
@@ -316,6 +357,7 @@ def bench_batch(n_solves=48, check=8):
     plain_out = [plain(sp) for sp in specs]
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
+    progress("batch: plain drop-in solves done")
 
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -356,7 +398,9 @@ def bench_batch(n_solves=48, check=8):
         "batched_loop_gvox_steps_per_s": round(vox_steps / t_loop / 1e9, 2),
         "solves_stored_permuted": sum(1 for s in b_perm if s != (0, 1, 2)),
         "graph_steps": graph_steps,
+        "tile_y": tile_y,
         "x_chunk_cap": cap,
+        "tuner": tuner.entry(tuner.key()),
         "run_timing_s": run_timing,
         "cg_iters_max": max(i["iters"] for i in info),
         "max_rel_err_vs_fp64": max(errs) if errs else None,
@@ -381,7 +425,9 @@ def bench_serve(rates=(5.0, 10.0, 20.0, 50.0, 100.0, 200.0), n_requests=200, tic
     t0 = time.perf_counter()
     pool = request_specs(n_requests, seed=4)
     t_gen = time.perf_counter() - t0
-    tuner.cap()  # tuned (or read from the cache) before timing
+    tuner.batch_shape()  # tuned (or read from the cache) before timing
+    pretune_dropin(pool)
+    progress("serve: launch shapes tuned")
     out = {"workload": f"{n_requests} requests per rate from one seeded pool (seed 4): grids U{{64..256}}^3 "
                        f"(multiples of 8), nT U{{300..1500}}, float32, CG tol 1e-8; Poisson arrivals seeded per rate; "
                        f"ticks of {tick_passes} passes (one or two steps each), 3 ticks in flight, hipGraph replay",
@@ -394,6 +440,7 @@ def bench_serve(rates=(5.0, 10.0, 20.0, 50.0, 100.0, 200.0), n_requests=200, tic
         summ = {kk: (round(v, 4) if isinstance(v, float) else v) for kk, v in summ.items()}
         summ["rate_req_per_s"] = rate
         out["rates"].append(summ)
+        progress(f"serve: {rate:g} req/s done ({summ['solves_per_s']} solves/s)")
         for r in done:
             r.u = r.v = None
         del srv, done
@@ -788,13 +835,14 @@ def config1_cpu_vs_gpu():
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported by this build "
-              "(a multi-GPU layout is described in DESIGN.md, not implemented)", file=sys.stderr)
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the multi-GPU layout is "
+              "a design, not implemented)", file=sys.stderr)
         sys.exit(2)
     # before anything touches the GPU: the CPU workers are spawned processes
     cpu_par = None
     if not args.no_cpu and not (args.no_batch and args.no_serve) and not args.only and not args.config1:
         cpu_par = cpu_parallel_throughput()
+        progress("CPU process-parallel baseline done")
     import torch
 
     from shearwave_amd import workloads as W
@@ -813,9 +861,11 @@ def main():
     item = 4 if args.dtype == "float32" else 8
     vox = int(np.prod(w["shape"]))
     bench = DropIn(w, args.dtype, args.kernel, args.tile_y, args.x_chunk)
+    progress(f"drop-in solve set up (two-step tile {bench.tuned})")
     for _ in range(args.warmup):
         bench.run()
     torch.cuda.synchronize()
+    progress("warm-up solves done")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         bench.run()
@@ -823,6 +873,7 @@ def main():
     t = time.perf_counter() - t0
     per_solve = t / args.steps
     value = vox * w["n_steps"] / per_solve / 1e9
+    progress(f"timed solves done: {value:.1f} Gvoxel-steps/s")
     # the final state of the timed solve, kept for the float32-against-float64 check below
     final32 = (torch.stack(bench.solve.u_components()).double(), torch.stack(bench.solve.b_components()).double())
 
@@ -878,6 +929,9 @@ def main():
             "dt_cfl": w["dt_cfl"],
             "seed": 2,
             "cg_vectors": args.dtype,
+            "two_step_tile": None if bench.tuned is None else {
+                "tile_y": bench.tuned[0], "x_chunk": bench.tuned[1], "chosen_by": "autotuner",
+                "candidates_ps_per_voxel_step": (bench.solve_tuner_entry() or {}).get("ps_per_voxel_step")},
         },
         "roofline": {
             "bound": "hbm",
@@ -928,30 +982,38 @@ def main():
         rec["fp64"] = {"value": round(vox * w["n_steps"] / t64 / 1e9, 3), "ms_per_solve": round(t64 * 1e3, 2),
                        "step_kernel_GBs": round(ab64 / kt64 / 1e9, 1), "frac": round(ab64 / kt64 / 1e9 / HBM_PEAK_GBS, 4)}
         del b64
+        progress("float64 solve done")
     del final32
     if not args.no_batch:
         del bench
         torch.cuda.empty_cache()
         rec["batch"] = bench_batch(args.batch_solves)
+        progress("batching done")
         if cpu_par:
             per_solve = rec["batch"]["voxel_steps"] / args.batch_solves
             rec["batch"]["cpu_parallel"] = {**cpu_par, "solves_per_s_est": round(cpu_par["gvox_steps_per_s"] * 1e9 /
                                                                                    per_solve, 5)}
     if not args.no_serve:
         rec["serve"] = bench_serve(tuple(float(r) for r in args.serve_rates.split(",")), args.serve_requests)
+        progress("serving done")
         if cpu_par:
             per_req = rec["serve"]["rates"][0]["voxel_steps"] / args.serve_requests
             rec["serve"]["cpu_parallel"] = {**cpu_par, "solves_per_s_est": round(cpu_par["gvox_steps_per_s"] * 1e9 /
                                                                                    per_req, 5)}
     if not args.no_adjoint:
         rec["adjoint"] = bench_adjoint()
+        progress("differentiation done")
     if not args.no_radforce:
         rec["radiation_force"] = bench_radforce()
+        progress("radiation force and acoustic strain done")
     if not args.no_cpu:
         rec["dropin_host_to_host"] = dropin_host_to_host(w, args.dtype)
+        progress("host-to-host drop-in done")
         rec["config1"] = config1_cpu_vs_gpu()
+        progress("64^3 CPU-runnable case done")
     if not args.no_static:
         rec["static_solve"] = bench_static()
+        progress("static solve done")
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(w, args.cpu_steps, warm=2, n_steps64=3, proj_iters=3,
                                            full_iters=int(rec["phases"]["plain_cg_iters"]))

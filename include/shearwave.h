@@ -227,8 +227,8 @@ int sw_gather_traces(int dtype, const void* f, int64_t n, const int64_t* idx, in
 
 /* One solve of a batch (float32 storage, increment form), as device memory.
  * A batch advances all its solves with ONE persistent launch per pass: work
- * items are (solve, z tile, y tile, x chunk) quadruples of int32 over 8-row,
- * 64-column tiles, and a device-side global pass counter gives each solve its
+ * items are (solve, z tile, y tile, x chunk) quadruples of int32 over
+ * tile_y-row, 64-column tiles (tile_y 4, 8 or 12, one per launch), and a device-side global pass counter gives each solve its
  * pass k = gpass - start (active while 0 <= k < n_passes; its state is read
  * from parity k&1 and written to the other parity). A pass is one step, or
  * two steps when plan says so: plan[k] = (first local step << 1)
@@ -257,7 +257,8 @@ typedef struct sw_batch_solve {
 } sw_batch_solve;
 
 /* Advance the batch by n_passes global passes: per pass one persistent launch
- * over all `items` (tile_y = 8; blocks = 0 picks one resident wave) and one
+ * over all `items` (tiles of tile_y = 4, 8 or 12 rows, as the items were
+ * cut; blocks = 0 picks one resident wave) and one
  * increment of gpass[0]. gpass is device memory of SW_BATCH_PASS_STATE int64:
  * gpass[0] the global pass, gpass[1..9] the launch's work-claim counters,
  * which the caller zeroes once and every launch leaves at zero. The item list
@@ -345,6 +346,14 @@ int sw_inject_traces(int dtype, void* f, int64_t n, const int64_t* idx, int64_t 
  * x-marching pass does not take (nz < 32, ny < 8, nx < 8). */
 int sw_fdtd_step2_mid(const sw_fdtd_state* st, const double* env2, void* const* u_mid, void* const* d_mid,
                       void* stream);
+
+/* n_passes force-free two-step passes from (u, b) into (u_next, b_next), with
+ * the state's tile_y and x_chunk, each reading the same state: the buffers
+ * are not rotated, so the state is unchanged and u_next / b_next receive one
+ * pass's result. For the launch-shape autotuner, which times candidate tile
+ * shapes on a solve's own buffers before its loop. No reference counterpart.
+ * SW_ERR_UNSUPPORTED where sw_fdtd_step2_mid is. */
+int sw_fdtd_trial_passes(const sw_fdtd_state* st, int64_t n_passes, void* stream);
 
 /* ------------------------------------------------------------------ strain */
 

@@ -196,6 +196,7 @@ SIGNATURES = {
                                C.POINTER(_p), _p, _p, _p, _p, _i64, _i64, _i64, _d, _d, _d, _d, _d, _p]),
     "sw_inject_traces": (_i32, [_i32, _p, _i64, _p, _i64, _p, _i64, _p]),
     "sw_fdtd_step2_mid": (_i32, [C.POINTER(FdtdState), C.POINTER(_d), _p, _p, _p]),
+    "sw_fdtd_trial_passes": (_i32, [C.POINTER(FdtdState), _i64, _p]),
 }
 
 _lock = threading.Lock()

@@ -17,6 +17,12 @@ Build-only options (namespaced ``sw_``):
     ``"march2"`` on gfx950), ``"march"`` (one step per pass) or ``"direct"``
     time-step kernel; ``"auto"`` is ``"march2"`` on grids the march tiles fit;
   * ``sw_tile_y``, ``sw_tile_z``, ``sw_x_chunk``: march-kernel tile shape (0 = automatic);
+  * ``sw_autotune``: with ``sw_kernel`` ``"auto"`` and no tile shape given,
+    the two-step pass's tile rows and x chunk come from the autotuner
+    (``shearwave_amd.autotune``), which times the candidates once per grid
+    class on the solve's own buffers and caches the fastest (default True
+    for loops of at least ``AUTOTUNE_MIN_VOXEL_STEPS`` voxel-steps); the
+    results do not depend on the tile shape;
   * ``sw_cg_vectors``: storage of the projection's CG vectors r and p,
     ``"float32"`` or ``"float64"`` (default: ``sw_dtype``); phi, the dot
     products and the residual norm are float64 in both, as poisson_cg keeps
@@ -37,6 +43,12 @@ import numpy as np
 import torch
 
 from . import engine as E
+
+
+# Loops shorter than this many voxel-steps keep the default launch shape: the
+# autotuner's first measurement of a grid class (about 30 passes per
+# candidate) would cost more than it saves.
+AUTOTUNE_MIN_VOXEL_STEPS = 1 << 30
 
 
 def _is_dev(x):
@@ -188,6 +200,10 @@ def shear_fdtd(  # noqa: PLR0912, PLR0915
                     vel_damp=float(vel_damp), kernel=opts.get("sw_kernel", "auto"),
                     tile_y=int(opts.get("sw_tile_y", 0)), x_chunk=int(opts.get("sw_x_chunk", 0)),
                     tile_z=int(opts.get("sw_tile_z", 0)), cg_vectors=opts.get("sw_cg_vectors"))
+    if (opts.get("sw_autotune", True) and opts.get("sw_kernel", "auto") == "auto" and not opts.get("sw_tile_y")
+            and not opts.get("sw_x_chunk") and solve.two_step_fits()
+            and n_steps * solve.n >= AUTOTUNE_MIN_VOXEL_STEPS):
+        solve.autotune()
     if initial_u is not None or initial_v is not None:
         solve.set_initial(initial_u, initial_v)
 

@@ -129,13 +129,13 @@ def aos_to_given(a, perm):
 class _Solve:
     """Device buffers of one solve in a batch, stored in the axis order `perm` (given axis perm[i] first)."""
 
-    def __init__(self, spec: SolveSpec, two_step=True, axes="auto"):
+    def __init__(self, spec: SolveSpec, two_step=True, axes="auto", by=STEP_BY):
         dev = E.device()
         self.spec = spec
         given = tuple(int(s) for s in spec.shape)
         if len(given) != 3 or min(given) < 1:
             raise ValueError(f"solve shape {given} must be three positive sizes")
-        self.perm = axis_order(given) if axes == "auto" else (0, 1, 2)
+        self.perm = axis_order(given, by) if axes == "auto" else (0, 1, 2)
         self.inv = tuple(int(i) for i in np.argsort(self.perm))
         shape = tuple(given[i] for i in self.perm)
         if shape[1] * shape[2] >= MAX_PLANE:
@@ -290,7 +290,7 @@ def n_step_items(solves, by=STEP_BY):
                for s in solves)
 
 
-def fill_xchunks(solves, n_blocks, per_block=2, floor=48):
+def fill_xchunks(solves, n_blocks, per_block=2, floor=48, by=STEP_BY):
     """Shorten x-chunks until the work items give each of `n_blocks` persistent blocks `per_block` of them.
 
     Long chunks pay fewer prologues (on the 48-solve workload whole columns
@@ -298,7 +298,7 @@ def fill_xchunks(solves, n_blocks, per_block=2, floor=48):
     tail of a batch) needs shorter items to occupy every CU: the longest
     chunk is halved, down to `floor` planes, until there are enough.
     """
-    while n_step_items(solves) < per_block * n_blocks:
+    while n_step_items(solves, by) < per_block * n_blocks:
         s = max(solves, key=lambda s: min(s.xchunk, s.shape[0]), default=None)
         if s is None or min(s.xchunk, s.shape[0]) <= floor:
             return
@@ -320,13 +320,14 @@ def xchunk_for(s, tuner=None):
 class Batch:
     """A set of solves advanced together on one GPU (float32 storage, increment form)."""
 
-    def __init__(self, specs, *, tile_y=STEP_BY, tuner=None, graph_steps=16, two_step=True, axes="auto"):
+    def __init__(self, specs, *, tile_y=None, tuner=None, graph_steps=16, two_step=True, axes="auto"):
         self.specs = list(specs)
-        self.tile_y = int(tile_y)
+        # tile rows of the step launch: given, else the tuner's, else the default
+        self.tile_y = int(tile_y) if tile_y else (tuner.tile_y() if tuner else STEP_BY)
         self.graph_steps = int(graph_steps)  # passes per captured graph, replayed (0: direct launches only)
         self.solves = []
         for sp in self.specs:
-            s = _Solve(sp, two_step, axes)
+            s = _Solve(sp, two_step, axes, self.tile_y)
             s.xchunk = s.base_xchunk = xchunk_for(s, tuner)
             self.solves.append(s)
         # the global pass and the step kernel's work-claim counters (SW_BATCH_PASS_STATE int64)
@@ -535,7 +536,7 @@ class Batch:
             act = [self.solves[k] for k in active]
             for s in act:
                 s.xchunk = s.base_xchunk
-            fill_xchunks(act, n_cu)
+            fill_xchunks(act, n_cu, by=self.tile_y)
             items = step_items(act, self.tile_y)
             # map local solve index back to the full descriptor array
             items[:, 0] = np.asarray(active, dtype=np.int32)[items[:, 0]]

@@ -107,24 +107,41 @@ __device__ inline void claim_items_xcd(unsigned long long* ctr, int64_t n_items,
     }
 }
 
-// Tile rows of the batched step: 8 output rows; the block has 8 + 3 rows of
-// threads so that a two-step pass (march2_step.h) fits, and single steps leave
-// the three extra rows to the barriers.
-constexpr int kBatchBY = 8;
-constexpr int kBatchRows = kBatchBY + 3;
+// Tile rows of the batched step: BY output rows (4, 8 or 12, one launch of
+// one shape; the autotuner picks it with the x-chunk cap); the block has
+// BY + 3 rows of threads so that a two-step pass (march2_step.h) fits, and
+// single steps leave the three extra rows to the barriers. The 4-row tile
+// prefetches one plane ahead so that two 7-wave blocks share a CU; the 8-row
+// tile two planes (one 11-wave block per CU), the 12-row tile one.
+constexpr int kBatchBYDefault = 8;
+
+// Waves per SIMD the kernel must fit (launch bound): two 7-wave blocks per CU
+// for the 4-row tile, else whatever one block needs.
+template <int BY>
+constexpr int batch_min_waves_per_simd() {
+    return BY == 4 ? 4 : (BY + 3 + 3) / 4;
+}
+
+template <int BY, bool FORCE>
+constexpr int batch_step2_dist() {
+    return BY == 4 ? 1 : 0;  // 0: march2_tile's default
+}
 
 // LDS of one block: the two-step tile's planes, or the single-step tile's.
+template <int BY>
 union BatchLds {
-    Step2Lds<float, true, kBatchBY> two_visc;
-    Step2Lds<float, false, kBatchBY> two_inviscid;
-    float one[2][6][kBatchBY + 2][66];
+    Step2Lds<float, true, BY> two_visc;
+    Step2Lds<float, false, BY> two_inviscid;
+    float one[2][6][BY + 2][66];
 };
 
 // pass_state[0] is the global pass; items are claimed on pass_state[1..9].
-__global__ void __launch_bounds__(64 * kBatchRows) batch_step_kernel(const sw_batch_solve* __restrict__ solves,
-                                                                     const int4* __restrict__ items, int64_t n_items,
-                                                                     long long* __restrict__ pass_state) {
-    __shared__ BatchLds lds;
+template <int BY>
+__global__ void __launch_bounds__(64 * (BY + 3), batch_min_waves_per_simd<BY>()) batch_step_kernel(const sw_batch_solve* __restrict__ solves,
+                                                                   const int4* __restrict__ items, int64_t n_items,
+                                                                   long long* __restrict__ pass_state) {
+    constexpr int R = BY + 3;
+    __shared__ BatchLds<BY> lds;
     const long long gp = pass_state[0];
     claim_items_xcd((unsigned long long*)(pass_state + 1), n_items, [&](long long k) {
         const int4 it = items[k];
@@ -158,31 +175,34 @@ __global__ void __launch_bounds__(64 * kBatchRows) batch_step_kernel(const sw_ba
         a.fs2 = (forcing && two) ? (float)S.env[local + 1] : 0.f;
         const int x0 = it.w * S.xchunk;
         const int x1 = min(x0 + S.xchunk, S.nx);
-        const int y0 = it.z * kBatchBY, z0 = it.y * 64;
+        const int y0 = it.z * BY, z0 = it.y * 64;
         const bool visc = a.e != nullptr, force = a.fs != 0.f || a.fs2 != 0.f;
-        constexpr int B = kBatchBY, R = kBatchRows;
         if (two) {
             if (visc) {
                 if (force)
-                    march2_tile<float, true, true, B>(a, x0, x1, y0, z0, lds.two_visc);
+                    march2_tile<float, true, true, BY, false, batch_step2_dist<BY, true>()>(a, x0, x1, y0, z0,
+                                                                                   lds.two_visc);
                 else
-                    march2_tile<float, true, false, B>(a, x0, x1, y0, z0, lds.two_visc);
+                    march2_tile<float, true, false, BY, false, batch_step2_dist<BY, false>()>(a, x0, x1, y0, z0,
+                                                                                   lds.two_visc);
             } else {
                 if (force)
-                    march2_tile<float, false, true, B>(a, x0, x1, y0, z0, lds.two_inviscid);
+                    march2_tile<float, false, true, BY, false, batch_step2_dist<BY, true>()>(a, x0, x1, y0, z0,
+                                                                                   lds.two_inviscid);
                 else
-                    march2_tile<float, false, false, B>(a, x0, x1, y0, z0, lds.two_inviscid);
+                    march2_tile<float, false, false, BY, false, batch_step2_dist<BY, false>()>(a, x0, x1, y0, z0,
+                                                                                   lds.two_inviscid);
             }
         } else if (visc) {
             if (force)
-                march_tile<float, true, true, true, B, 64, 6, R>(a, x0, x1, y0, z0, lds.one);
+                march_tile<float, true, true, true, BY, 64, 6, R>(a, x0, x1, y0, z0, lds.one);
             else
-                march_tile<float, true, true, false, B, 64, 6, R>(a, x0, x1, y0, z0, lds.one);
+                march_tile<float, true, true, false, BY, 64, 6, R>(a, x0, x1, y0, z0, lds.one);
         } else {
             if (force)
-                march_tile<float, true, false, true, B, 64, 6, R>(a, x0, x1, y0, z0, lds.one);
+                march_tile<float, true, false, true, BY, 64, 6, R>(a, x0, x1, y0, z0, lds.one);
             else
-                march_tile<float, true, false, false, B, 64, 6, R>(a, x0, x1, y0, z0, lds.one);
+                march_tile<float, true, false, false, BY, 64, 6, R>(a, x0, x1, y0, z0, lds.one);
         }
     });
 }
@@ -364,22 +384,42 @@ __global__ void __launch_bounds__(64 * kCgBY) batch_cg_grad_sub_kernel(const sw_
     }
 }
 
+template <int BY>
 unsigned persistent_step_blocks(int requested) {
     if (requested > 0) return (unsigned)requested;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_step_kernel, 64 * kBatchRows, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_step_kernel<BY>, 64 * (BY + 3), 0) !=
+            hipSuccess ||
         per_cu < 1)
         per_cu = 1;
     return (unsigned)(per_cu * device_cu_count());  // a multiple of 8 on an 8-XCD device
 }
 
-int launch_batch_passes(const sw_batch_solve* solves, const int32_t* items, int64_t n_items, int64_t* gpass,
-                        int blocks, int64_t n_passes, hipStream_t s) {
-    const unsigned nb = persistent_step_blocks(blocks);
+template <int BY>
+void launch_batch_passes_rows(const sw_batch_solve* solves, const int32_t* items, int64_t n_items, int64_t* gpass,
+                              int blocks, int64_t n_passes, hipStream_t s) {
+    const unsigned nb = persistent_step_blocks<BY>(blocks);
     for (int64_t k = 0; k < n_passes; ++k) {
-        hipLaunchKernelGGL(batch_step_kernel, dim3(nb), dim3(64, kBatchRows), 0, s, solves, (const int4*)items, n_items,
-                           (long long*)gpass);
+        hipLaunchKernelGGL(batch_step_kernel<BY>, dim3(nb), dim3(64, BY + 3), 0, s, solves, (const int4*)items,
+                           n_items, (long long*)gpass);
         hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(1), 0, s, (long long*)gpass);
+    }
+}
+
+bool batch_rows_supported(int tile_y) { return tile_y == 4 || tile_y == 8 || tile_y == 12; }
+
+int launch_batch_passes(const sw_batch_solve* solves, const int32_t* items, int64_t n_items, int64_t* gpass,
+                        int tile_y, int blocks, int64_t n_passes, hipStream_t s) {
+    switch (tile_y) {
+        case 4:
+            launch_batch_passes_rows<4>(solves, items, n_items, gpass, blocks, n_passes, s);
+            break;
+        case 12:
+            launch_batch_passes_rows<12>(solves, items, n_items, gpass, blocks, n_passes, s);
+            break;
+        default:
+            launch_batch_passes_rows<kBatchBYDefault>(solves, items, n_items, gpass, blocks, n_passes, s);
+            break;
     }
     SW_LAUNCH_CHECK();
     return SW_OK;
@@ -400,16 +440,16 @@ extern "C" {
 int sw_batch_step(const sw_batch_solve* solves, const int32_t* items, int64_t n_items, int64_t* gpass, int32_t tile_y,
                   int32_t blocks, int64_t n_passes, void* stream) {
     SW_REQUIRE(solves && gpass && (items || n_items == 0), SW_ERR_ARG, "null pointer");
-    SW_REQUIRE(tile_y == kBatchBY, SW_ERR_ARG, "tile_y %d must be %d", tile_y, kBatchBY);
+    SW_REQUIRE(batch_rows_supported(tile_y), SW_ERR_ARG, "tile_y %d must be 4, 8 or 12", tile_y);
     SW_REQUIRE(n_items >= 0 && n_passes >= 0 && blocks >= 0, SW_ERR_ARG, "negative size");
-    return launch_batch_passes(solves, items, n_items, gpass, blocks, n_passes, (hipStream_t)stream);
+    return launch_batch_passes(solves, items, n_items, gpass, tile_y, blocks, n_passes, (hipStream_t)stream);
 }
 
 int sw_batch_graph_create(const sw_batch_solve* solves, const int32_t* items, int64_t n_items, int64_t* gpass,
                           int32_t tile_y, int32_t blocks, int64_t n_passes, sw_graph** graph) {
     SW_REQUIRE(graph, SW_ERR_ARG, "null graph handle");
     SW_REQUIRE(solves && gpass && (items || n_items == 0), SW_ERR_ARG, "null pointer");
-    SW_REQUIRE(tile_y == kBatchBY, SW_ERR_ARG, "tile_y %d must be %d", tile_y, kBatchBY);
+    SW_REQUIRE(batch_rows_supported(tile_y), SW_ERR_ARG, "tile_y %d must be 4, 8 or 12", tile_y);
     SW_REQUIRE(n_items >= 0 && n_passes >= 1 && blocks >= 0, SW_ERR_ARG, "bad size");
     *graph = nullptr;
     hipStream_t cap;
@@ -419,7 +459,7 @@ int sw_batch_graph_create(const sw_batch_solve* solves, const int32_t* items, in
         (void)hipStreamDestroy(cap);
         SW_TRY_HIP(e);
     }
-    const int rc = launch_batch_passes(solves, items, n_items, gpass, blocks, n_passes, cap);
+    const int rc = launch_batch_passes(solves, items, n_items, gpass, tile_y, blocks, n_passes, cap);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(cap, &g);
     (void)hipStreamDestroy(cap);

@@ -451,13 +451,27 @@ int run_step2_mid(const sw_fdtd_state* st, const double* env2, void* const* u_mi
     return launch_step2<T, true>(a, st->eta_over_rho != nullptr, force, st->tile_y, st->x_chunk, s);
 }
 
-}  // namespace
-}  // namespace sw
+// n_passes force-free two-step passes, each from (u, b) into (u_next,
+// b_next) with the state's tile_y and x_chunk; the buffers are not rotated.
+template <typename T>
+int run_trial_passes(const sw_fdtd_state* st, int64_t n_passes, hipStream_t s) {
+    StepArgs<T> a = step_args<T>(st);
+    a.fs = a.fs2 = T(0);
+    for (int c = 0; c < 3; ++c) {
+        a.A[c] = (const T*)st->u[c];
+        a.B[c] = (const T*)st->b[c];
+        a.Ao[c] = (T*)st->u_next[c];
+        a.Bo[c] = (T*)st->b_next[c];
+    }
+    const bool visc = st->eta_over_rho != nullptr;
+    for (int64_t k = 0; k < n_passes; ++k)
+        if (const int rc = launch_step2<T>(a, visc, false, st->tile_y, st->x_chunk, s)) return rc;
+    return SW_OK;
+}
 
-using namespace sw;
-
-extern "C" int sw_fdtd_step2_mid(const sw_fdtd_state* st, const double* env2, void* const* u_mid,
-                                 void* const* d_mid, void* stream) {
+// The checks shared by the entry points that run two-step passes on a state
+// without rotating it.
+int check_pass_state(const sw_fdtd_state* st) {
     SW_REQUIRE(st, SW_ERR_ARG, "null state");
     int rc = check_dtype(st->dtype);
     if (rc) return rc;
@@ -465,14 +479,33 @@ extern "C" int sw_fdtd_step2_mid(const sw_fdtd_state* st, const double* env2, vo
     SW_REQUIRE(st->form == SW_FORM_INCREMENT, SW_ERR_ARG, "the two-step pass needs the increment form");
     SW_REQUIRE(st->dt > 0 && st->dx > 0 && st->dy > 0 && st->dz > 0, SW_ERR_ARG, "dt and spacing must be > 0");
     SW_REQUIRE(st->mu_over_rho, SW_ERR_ARG, "mu_over_rho is null");
-    SW_REQUIRE(u_mid && d_mid, SW_ERR_ARG, "null intermediate state");
     for (int c = 0; c < 3; ++c)
-        SW_REQUIRE(st->u[c] && st->b[c] && st->u_next[c] && st->b_next[c] && u_mid[c] && d_mid[c], SW_ERR_ARG,
-                   "null state buffer");
+        SW_REQUIRE(st->u[c] && st->b[c] && st->u_next[c] && st->b_next[c], SW_ERR_ARG, "null state buffer");
     const Grid g{(int)st->nx, (int)st->ny, (int)st->nz};
     SW_REQUIRE(g.nz >= 32 && g.ny >= 8 && g.nx >= 8 && g.plane() < kMaxMarchPlane, SW_ERR_UNSUPPORTED,
                "the two-step pass needs nz >= 32, ny >= 8, nx >= 8 and a y-z plane of fewer than %lld voxels",
                (long long)kMaxMarchPlane);
+    return SW_OK;
+}
+
+}  // namespace
+}  // namespace sw
+
+using namespace sw;
+
+extern "C" int sw_fdtd_trial_passes(const sw_fdtd_state* st, int64_t n_passes, void* stream) {
+    if (const int rc = check_pass_state(st)) return rc;
+    SW_REQUIRE(n_passes >= 0, SW_ERR_ARG, "n_passes must be >= 0");
+    hipStream_t s = (hipStream_t)stream;
+    if (st->dtype == SW_F32) return run_trial_passes<float>(st, n_passes, s);
+    return run_trial_passes<double>(st, n_passes, s);
+}
+
+extern "C" int sw_fdtd_step2_mid(const sw_fdtd_state* st, const double* env2, void* const* u_mid,
+                                 void* const* d_mid, void* stream) {
+    if (const int rc = check_pass_state(st)) return rc;
+    SW_REQUIRE(u_mid && d_mid, SW_ERR_ARG, "null intermediate state");
+    for (int c = 0; c < 3; ++c) SW_REQUIRE(u_mid[c] && d_mid[c], SW_ERR_ARG, "null intermediate state buffer");
     hipStream_t s = (hipStream_t)stream;
     if (st->dtype == SW_F32) return run_step2_mid<float>(st, env2, u_mid, d_mid, s);
     return run_step2_mid<double>(st, env2, u_mid, d_mid, s);

@@ -97,7 +97,8 @@ __device__ __forceinline__ void stage(Plane& L, int r, int q, const Col<T>& c) {
 
 // With MID the pass also stores step n+1 of its own planes (u_{n+1} = u_n +
 // d_{n+1} and d_{n+1}, the values two single steps store) into a.Am, a.Bm.
-template <typename T, bool VISC, bool FORCE, int BY, bool MID = false>
+// DIST overrides the prefetch distance (0: step2_dist).
+template <typename T, bool VISC, bool FORCE, int BY, bool MID = false, int DIST = 0>
 __device__ __forceinline__ void march2_tile(const StepArgs<T>& a, int x0, int x1, int y0, int z0,
                                             Step2Lds<T, VISC, BY>& lds) {
     static_assert(2 * BY + 4 <= 64 && BY + 3 <= 16, "tile rows");
@@ -154,7 +155,7 @@ __device__ __forceinline__ void march2_tile(const StepArgs<T>& a, int x0, int x1
     };
 
     // rings, slot (plane - xb) & (QN-1) (q) or & 3 (the others), xb = x0 - 1 the first iteration
-    constexpr int D = step2_dist<T, BY, FORCE>();
+    constexpr int D = DIST > 0 ? DIST : step2_dist<T, BY, FORCE>();
     static_assert(D >= 1 && D <= 2, "two-step prefetch distance");
     constexpr int QN = D == 1 ? 4 : 8, QM = QN - 1;  // the loop is unrolled by QN
     Col<T> q[QN];  // step n of this column: planes x-1 .. x+1+D

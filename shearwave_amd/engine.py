@@ -282,6 +282,25 @@ class Solve:
         s.tile_z = int(tile_z)
         self.state = s
 
+    def two_step_fits(self):
+        """Whether this solve's loop runs two-step passes whose launch shape the autotuner can pick."""
+        nx, ny, nz = self.shape
+        return (self.form == "increment" and self.state.kernel in (_KERNELS["auto"], _KERNELS["march2"])
+                and nz >= 32 and ny >= 8 and nx >= 8 and ny * nz < (1 << 28))
+
+    def autotune(self, tuner=None):
+        """Tile rows and x chunk of the two-step pass from the autotuner, which times the candidates on this
+        solve's own buffers once per grid class (reading the state and writing only the next-step buffers)
+        and caches the fastest. The state is unchanged, and every candidate computes each voxel with the
+        same arithmetic, so results do not depend on the choice. Returns (tile rows, x chunk)."""
+        from .autotune import default_tuner
+
+        if not self.two_step_fits():
+            raise ValueError("the autotuner picks the launch shape of the two-step pass, which this solve does not run")
+        ty, xc = (tuner or default_tuner()).pass_shape(self)
+        self.state.tile_y, self.state.x_chunk = int(ty), int(xc)
+        return int(ty), int(xc)
+
     # -------------------------------------------------------------- inputs
     def set_force(self, fx, fy, fz, tol, max_iters):
         """Project the force pattern (solver.py:189-192) and keep dT^2 f_S / rho for the loop."""

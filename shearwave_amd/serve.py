@@ -35,8 +35,8 @@ import torch
 
 from . import engine as E
 from ._abi import check, lib
-from .batch import PASS_STATE, Batch, SolveSpec, _Solve, fill_xchunks, n_compute_units, solve_bytes, step_items, \
-    xchunk_for
+from .batch import PASS_STATE, STEP_BY, Batch, SolveSpec, _Solve, fill_xchunks, n_compute_units, solve_bytes, \
+    step_items, xchunk_for
 
 
 @dataclass
@@ -80,12 +80,12 @@ class _TickGraph:
 class Server:
     """Continuous-batching solve server (float32 storage, increment form)."""
 
-    def __init__(self, tick_passes=10, max_active=128, budget_bytes=None, tuner=None, tile_y=8, two_step=True,
+    def __init__(self, tick_passes=10, max_active=128, budget_bytes=None, tuner=None, tile_y=None, two_step=True,
                  ticks_in_flight=3, graph=True, admit_voxels=96 << 20, idle_admit_voxels=None):
         self.tick_passes = int(tick_passes)
         self.two_step = bool(two_step)
         self.max_active = int(max_active)
-        self.tile_y = int(tile_y)
+        self.tile_y = int(tile_y) if tile_y else (tuner.tile_y() if tuner else STEP_BY)
         self.tuner = tuner
         self.ticks_in_flight = max(1, int(ticks_in_flight))
         self.admit_voxels = int(admit_voxels)  # at most this many voxels per batched projection of admissions
@@ -102,7 +102,7 @@ class Server:
         self.admit_stream = torch.cuda.Stream(self.dev)  # default priority
         self.gpass_dev = torch.zeros(PASS_STATE, dtype=torch.int64, device=self.dev)
         self.gpass = 0
-        self._helper = Batch([], tuner=tuner)  # descriptor / projection helpers
+        self._helper = Batch([], tuner=tuner, tile_y=self.tile_y)  # descriptor / projection helpers
         self.admission_s = []  # wall time of each admission (upload + batched projection)
 
     # ------------------------------------------------------------ admission
@@ -137,7 +137,7 @@ class Server:
                     t = time.perf_counter()
                     solves = []
                     for r in reqs:
-                        s = _Solve(r.spec, self.two_step)
+                        s = _Solve(r.spec, self.two_step, "auto", self.tile_y)
                         s.xchunk = s.base_xchunk = xchunk_for(s, self.tuner)
                         solves.append(s)
                     info = self._helper.project(solves)  # waits on the admission stream only
@@ -220,7 +220,7 @@ class Server:
                     sol = [s for _, s in active]
                     for s in sol:
                         s.xchunk = s.base_xchunk
-                    fill_xchunks(sol, self.n_cu)
+                    fill_xchunks(sol, self.n_cu, by=self.tile_y)
                     desc = self._helper._desc_array(sol)
                     items = step_items(sol, self.tile_y)
                     items_t = E.upload_async(items.reshape(-1))

@@ -7,7 +7,8 @@ import textwrap
 
 import pytest
 
-from shearwave_amd.autotune import CAPS, Tuner, balanced_chunk, tune_mix_shapes
+from shearwave_amd.autotune import BATCH_ROWS, CAPS, PASS_ROWS, PASS_XCHUNKS, Tuner, balanced_chunk, batch_candidates, \
+    pass_candidates, tune_mix_shapes
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -32,24 +33,71 @@ def test_tune_mix_is_fixed():
     assert len(set(shapes)) > 20  # uneven grids
 
 
-def test_tuner_picks_fastest_cap_and_caches(tmp_path):
+def test_tuner_picks_fastest_batch_shape_and_caches(tmp_path):
     calls = []
 
-    def fake(caps):
-        calls.append(tuple(caps))
-        return {c: abs(c - 128) + 1.0 for c in caps}  # fastest at 128
+    def fake(cands):
+        calls.append(tuple(cands))
+        return {(r, c): abs(c - 128) + abs(r - 8) * 100 + 1.0 for r, c in cands}  # fastest at (8, 128)
 
     path = str(tmp_path / "t.json")
     t = Tuner(path=path, timer=fake, arch="gfx950:256")
-    assert t.cap() == 128 and calls == [CAPS]
+    assert t.batch_shape() == (8, 128) and calls == [tuple(batch_candidates())]
+    assert t.cap() == 128 and t.tile_y() == 8
+    assert {r for r, _ in calls[0]} == set(BATCH_ROWS) and {c for _, c in calls[0]} == set(CAPS)
     assert t.xchunk((200, 64, 64)) == 100 and t.xchunk((100, 64, 64)) == 100
     assert len(calls) == 1  # measured once
     t2 = Tuner(path=path, timer=fake, arch="gfx950:256")
-    assert t2.cap() == 128 and len(calls) == 1  # reloaded from disk
+    assert t2.batch_shape() == (8, 128) and len(calls) == 1  # reloaded from disk
     table = json.load(open(path))
-    assert table["gfx950:256|batch_cap"]["cap"] == 128
+    e = table["gfx950:256|batch_shape"]
+    assert e["cap"] == 128 and e["tile_y"] == 8 and len(e["ps_per_voxel_step"]) == len(BATCH_ROWS) * len(CAPS)
     t3 = Tuner(path=path, timer=fake, arch="gfx950:304")  # another device is tuned anew
-    assert t3.cap() == 128 and len(calls) == 2
+    assert t3.batch_shape() == (8, 128) and len(calls) == 2
+
+
+def test_tuner_can_pick_other_tile_rows(tmp_path):
+    t = Tuner(path=str(tmp_path / "t.json"), timer=lambda cands: {c: (1.0 if c == (12, 64) else 2.0) for c in cands},
+              arch="x")
+    assert t.batch_shape() == (12, 64)
+
+
+class _FakeSolve:
+    def __init__(self, shape, dtype="float32", viscous=True):
+        self.shape, self.dtype, self.e = shape, dtype, (object() if viscous else None)
+
+
+def test_pass_candidates():
+    c = pass_candidates("float32", (256, 256, 256))
+    assert {r for r, _ in c} == set(PASS_ROWS["float32"]) and (8, 0) in c and (12, 256) in c
+    assert len(c) == len(set(c)) == len(PASS_ROWS["float32"]) * len(PASS_XCHUNKS)
+    c = pass_candidates("float32", (40, 64, 64))  # chunk depths at or beyond nx collapse to whole columns
+    assert sorted({xc for _, xc in c}) == [0, 32, 40] and len(c) == len(set(c))
+    assert {r for r, _ in pass_candidates("float64", (64, 64, 64))} == set(PASS_ROWS["float64"])
+
+
+def test_pass_shape_per_grid_class_and_cached(tmp_path):
+    calls = []
+
+    def fake(solve, cands):
+        calls.append((solve.shape, tuple(cands)))
+        want = (12, 64) if solve.shape[0] == 256 else (8, 0)
+        return {c: (1.0 if c == want else 2.0 + c[1]) for c in cands}
+
+    path = str(tmp_path / "t.json")
+    t = Tuner(path=path, pass_timer=fake, arch="gfx950:256")
+    assert t.pass_shape(_FakeSolve((256, 256, 256))) == (12, 64)
+    assert t.pass_shape(_FakeSolve((128, 96, 160))) == (8, 0)
+    assert t.pass_shape(_FakeSolve((256, 256, 256))) == (12, 64) and len(calls) == 2
+    assert calls[0][1] == tuple(pass_candidates("float32", (256, 256, 256)))
+    # storage type and viscosity are separate classes
+    t.pass_shape(_FakeSolve((256, 256, 256), viscous=False))
+    t.pass_shape(_FakeSolve((256, 256, 256), dtype="float64"))
+    assert len(calls) == 4 and calls[3][1] == tuple(pass_candidates("float64", (256, 256, 256)))
+    t2 = Tuner(path=path, pass_timer=fake, arch="gfx950:256")
+    assert t2.pass_shape(_FakeSolve((128, 96, 160))) == (8, 0) and len(calls) == 4  # from disk
+    e = t2.entry(t2.pass_key("float32", True, (256, 256, 256)))
+    assert e["tile_y"] == 12 and e["x_chunk"] == 64 and "12x64" in e["ps_per_voxel_step"]
 
 
 def test_c_struct_layouts_match_ctypes(tmp_path):

@@ -250,12 +250,32 @@ def test_projection_methods_agree(sw, specs):
 
 
 def test_tuner_times_the_mix(sw, tmp_path):
-    """The autotuner's measurement on the device: every cap timed on the tuning mix, the fastest cached."""
-    from shearwave_amd.autotune import CAPS, Tuner
+    """The autotuner's measurement on the device: every (tile rows, cap) pair timed on the tuning mix, the
+    fastest cached."""
+    from shearwave_amd.autotune import Tuner, batch_candidates
 
     t = Tuner(path=str(tmp_path / "tune.json"))
-    cap = t.cap()
-    entry = t._table[t.key()]
-    assert cap in CAPS and set(entry["ps_per_voxel_step"]) == {str(c) for c in CAPS}
-    times = {int(k): v for k, v in entry["ps_per_voxel_step"].items()}
-    assert all(v > 0 for v in times.values()) and times[cap] == min(times.values())
+    rows, cap = t.batch_shape()
+    entry = t.entry(t.key())
+    assert (rows, cap) in batch_candidates()
+    assert set(entry["ps_per_voxel_step"]) == {f"{r}x{c}" for r, c in batch_candidates()}
+    times = entry["ps_per_voxel_step"]
+    assert all(v > 0 for v in times.values()) and times[f"{rows}x{cap}"] == min(times.values())
+
+
+@pytest.mark.parametrize("rows", [4, 12])
+def test_tile_rows_change_nothing(sw, specs, rows):
+    """The batched step's tile rows (4, 8, 12; the autotuner picks one per batch) change the work items and
+    the launch, never a voxel's arithmetic: results equal the 8-row tiles' bit for bit (axis order fixed,
+    since the lane fill that picks it depends on the rows), graph replays included."""
+    from shearwave_amd.batch import Batch
+
+    got = {}
+    for by in (8, rows):
+        b = Batch(specs, tile_y=by, axes="given", graph_steps=5)
+        b.project()
+        b.run()
+        got[by] = [b.result(k) for k in range(len(specs))]
+    for (u8, v8), (u, v) in zip(got[8], got[rows]):
+        np.testing.assert_array_equal(u, u8)
+        np.testing.assert_array_equal(v, v8)

@@ -226,3 +226,76 @@ def test_pass_with_intermediate_state(sw, dtype, shape, viscous, force):
     z = torch.zeros_like(v0)
     assert not small.step2_into(v0, v0, z, z, z, z)
     assert float(z.abs().max()) == 0
+
+
+# ------------------------------------------------------- launch-shape tuning
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_trial_passes_leave_the_state_and_write_one_pass(sw, dtype):
+    """sw_fdtd_trial_passes (the autotuner's timing pass) reads the state and writes only the next-step
+    buffers: the state is unchanged, and the next-step buffers hold what one two-step pass of the loop
+    writes, whatever the number of trial passes."""
+    import ctypes as C
+
+    from shearwave_amd import _abi
+    from shearwave_amd import engine as E
+
+    shape = (36, 21, 70)
+    s = _solve(sw, shape, 31, "march2", True, dtype=dtype)
+    before = _state(s)
+    _abi.check(_abi.lib().sw_fdtd_trial_passes(C.byref(s.state), 3, E.stream_ptr()))
+    for a, b in zip(before, _state(s)):
+        torch.testing.assert_close(a, b, rtol=0, atol=0)
+    nxt = [t.clone() for t in s._vec(s.state.u_next)] + [t.clone() for t in s._vec(s.state.b_next)]
+    s.run(2)
+    for a, b in zip(nxt, _state(s)):
+        torch.testing.assert_close(a, b, rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_autotuned_launch_shape_changes_nothing(sw, dtype, tmp_path):
+    """The autotuner times every candidate tile shape on the solve's own buffers, caches the fastest, and
+    the solve then runs bit for bit as with the default launch shape (and as with every other candidate)."""
+    from shearwave_amd.autotune import PASS_ROWS, Tuner, pass_candidates
+
+    shape = (60, 40, 96)
+    t = Tuner(path=str(tmp_path / "tune.json"))
+    ref = _solve(sw, shape, 32, "auto", True, dtype=dtype)
+    ref.run(30)
+    want = _state(ref)
+    s = _solve(sw, shape, 32, "auto", True, dtype=dtype)
+    before = _state(s)
+    ty, xc = s.autotune(t)
+    assert (ty, xc) in pass_candidates(dtype, shape) and ty in PASS_ROWS[dtype]
+    e = t.entry(t.pass_key(dtype, True, shape))
+    assert len(e["ps_per_voxel_step"]) == len(pass_candidates(dtype, shape))
+    assert min(e["ps_per_voxel_step"].values()) == e["ps_per_voxel_step"][f"{ty}x{xc}"] > 0
+    for a, b in zip(before, _state(s)):  # tuning left the state alone
+        torch.testing.assert_close(a, b, rtol=0, atol=0)
+    s.run(30)
+    for a, b in zip(want, _state(s)):
+        torch.testing.assert_close(a, b, rtol=0, atol=0)
+    for cty, cxc in pass_candidates(dtype, shape)[::3]:  # and a sample of the other candidates
+        o = _solve(sw, shape, 32, "march2", True, tile_y=cty, x_chunk=cxc, dtype=dtype)
+        o.run(30)
+        for a, b in zip(want, _state(o)):
+            torch.testing.assert_close(a, b, rtol=0, atol=0)
+
+
+def test_dropin_autotune_option(sw, tmp_path, monkeypatch):
+    """shear_fdtd tunes the launch shape of a long enough loop (opts sw_autotune, default on) and returns
+    the same arrays as with tuning off."""
+    import shearwave_amd.autotune as A
+    from shearwave_amd import api
+
+    monkeypatch.setattr(A, "_DEFAULT", A.Tuner(path=str(tmp_path / "tune.json")))
+    monkeypatch.setattr(api, "AUTOTUNE_MIN_VOXEL_STEPS", 1)
+    shape = (40, 36, 70)
+    p = _problem(shape, 5, n_steps=24)
+    u1, v1 = _run_gpu(sw, p, shape, {"sw_autotune": False})
+    assert not A._DEFAULT._table
+    u2, v2 = _run_gpu(sw, p, shape)
+    assert any("|pass|float32|" in k for k in A._DEFAULT._table)
+    np.testing.assert_array_equal(u1, u2)
+    np.testing.assert_array_equal(v1, v2)
