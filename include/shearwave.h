@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define SW_ABI_VERSION 4
+#define SW_ABI_VERSION 5
 
 /* status codes */
 #define SW_OK 0
@@ -191,6 +191,12 @@ typedef struct sw_fdtd_outputs {
      * negative, it becomes the global index of that pass's last step. */
     int64_t finite_stride;
     int64_t* nonfinite_step;
+    /* how the in-loop invariants (inv_stride) are computed: 0 = by the next
+     * pass when that is a force-free two-step pass (it reads the state
+     * anyway and stores the invariants of its input state), else, and after
+     * the last pass of the call, by their own kernel right away; 1 = always
+     * by their own kernel right after the step. The values are the same. */
+    int64_t inv_mode;
 } sw_fdtd_outputs;
 
 /* The time loop of shear_fdtd (solver.py:194-239) for n_steps steps with

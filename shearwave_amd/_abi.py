@@ -88,6 +88,7 @@ class FdtdOutputs(C.Structure):
         ("trace_ld", _i64),
         ("finite_stride", _i64),
         ("nonfinite_step", _p),
+        ("inv_mode", _i64),
     ]
 
 
@@ -203,6 +204,10 @@ _lock = threading.Lock()
 _lib = None
 
 
+# SW_ABI_VERSION of include/shearwave.h that these ctypes declarations follow.
+ABI_VERSION = 5
+
+
 def lib():
     """Load the HIP library once; raise NativeLibraryMissing if it is absent."""
     global _lib
@@ -223,6 +228,9 @@ def lib():
             fn = getattr(handle, name)
             fn.restype = res
             fn.argtypes = args
+        if handle.sw_abi_version() != ABI_VERSION:  # a stale build would misread the structs
+            raise NativeLibraryMissing(f"{LIB_PATH} has ABI version {handle.sw_abi_version()}, this package "
+                                       f"needs {ABI_VERSION}; rebuild with `python -m shearwave_amd.build`")
         _lib = handle
     return _lib
 

@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(BZ * BY) step_march_kernel(StepArgs<T> a, int 
     march_tile<T, INCR, VISC, FORCE, BY, BZ>(a, x0, min(x0 + xchunk, a.g.nx), tyi * BY, tzi * BZ, lds);
 }
 
-template <typename T, bool VISC, bool FORCE, int BY, bool MID = false>
+template <typename T, bool VISC, bool FORCE, int BY, bool MID = false, bool STRAIN = false>
 __global__ void __launch_bounds__(64 * (BY + 3)) step2_march_kernel(StepArgs<T> a, int xchunk, int nty, int ntz) {
     __shared__ Step2Lds<T, VISC, BY> lds;
     const unsigned nb = gridDim.x;
@@ -101,7 +101,7 @@ __global__ void __launch_bounds__(64 * (BY + 3)) step2_march_kernel(StepArgs<T> 
     const int tyi = (int)(rest % (unsigned)nty);
     const int txi = (int)(rest / (unsigned)nty);
     const int x0 = txi * xchunk;
-    march2_tile<T, VISC, FORCE, BY, MID>(a, x0, min(x0 + xchunk, a.g.nx), tyi * BY, tzi * 64, lds);
+    march2_tile<T, VISC, FORCE, BY, MID, 0, STRAIN>(a, x0, min(x0 + xchunk, a.g.nx), tyi * BY, tzi * 64, lds);
 }
 
 // SW_STEP3_BY: rows of the three-step tile (block of 64 x (BY + 5) threads).
@@ -194,9 +194,9 @@ int launch_step(const StepArgs<T>& a, int kernel, int tile_y, int tile_z, int x_
     return SW_OK;
 }
 
-template <typename T, bool VISC, bool FORCE, int BY, bool MID = false>
+template <typename T, bool VISC, bool FORCE, int BY, bool MID = false, bool STRAIN = false>
 void launch_march2(const StepArgs<T>& a, int x_chunk, hipStream_t s) {
-    auto k = step2_march_kernel<T, VISC, FORCE, BY, MID>;
+    auto k = step2_march_kernel<T, VISC, FORCE, BY, MID, STRAIN>;
     static const int res = resident_blocks(k, 64 * (BY + 3));
     const MarchShape ms = march_shape(a.g, BY, 64, x_chunk, res);
     hipLaunchKernelGGL(k, dim3(ms.blocks()), dim3(64, BY + 3), 0, s, a, ms.xchunk, ms.nty, ms.ntz);
@@ -206,14 +206,16 @@ void launch_march2(const StepArgs<T>& a, int x_chunk, hipStream_t s) {
 // force, when either step applies it, is scaled by a.fs and a.fs2. Tile rows:
 // float32 8 (default) or 12 (force-free passes; forced ones use 8); float64 4
 // (a 7-wave block leaves each wave the registers the float64 queues need).
-template <typename T, bool VISC, bool FORCE, bool MID = false>
+// STRAIN (force-free passes only) also stores the strain invariants of u_n.
+template <typename T, bool VISC, bool FORCE, bool MID = false, bool STRAIN = false>
 int launch_step2_rows(const StepArgs<T>& a, int tile_y, int x_chunk, hipStream_t s) {
+    static_assert(!STRAIN || (!FORCE && !MID), "the strain pass is force-free");
     if constexpr (DType<T>::code == SW_F32) {
         const int by = tile_y > 0 ? tile_y : 8;
         if (by == 8 || (by == 12 && (FORCE || MID))) {
-            launch_march2<T, VISC, FORCE, 8, MID>(a, x_chunk, s);
+            launch_march2<T, VISC, FORCE, 8, MID, STRAIN>(a, x_chunk, s);
         } else if (by == 12) {
-            if constexpr (!FORCE && !MID) launch_march2<T, VISC, false, 12>(a, x_chunk, s);
+            if constexpr (!FORCE && !MID) launch_march2<T, VISC, false, 12, false, STRAIN>(a, x_chunk, s);
         } else {
             sw::set_error("two-step pass: float32 tile_y %d not supported (8 or 12)", by);
             return SW_ERR_ARG;
@@ -224,7 +226,7 @@ int launch_step2_rows(const StepArgs<T>& a, int tile_y, int x_chunk, hipStream_t
             sw::set_error("two-step pass: float64 tile_y %d not supported (4)", by);
             return SW_ERR_ARG;
         }
-        launch_march2<T, VISC, FORCE, 4, MID>(a, x_chunk, s);
+        launch_march2<T, VISC, FORCE, 4, MID, STRAIN>(a, x_chunk, s);
     }
     SW_LAUNCH_CHECK();
     return SW_OK;
@@ -255,7 +257,13 @@ int launch_step3(const StepArgs<T>& a, bool visc, int x_chunk, hipStream_t s) {
 }
 
 template <typename T, bool MID = false>
-int launch_step2(const StepArgs<T>& a, bool visc, bool force, int tile_y, int x_chunk, hipStream_t s) {
+int launch_step2(const StepArgs<T>& a, bool visc, bool force, int tile_y, int x_chunk, hipStream_t s,
+                 bool strain = false) {
+    if constexpr (!MID) {
+        if (strain && !force)
+            return visc ? launch_step2_rows<T, true, false, false, true>(a, tile_y, x_chunk, s)
+                        : launch_step2_rows<T, false, false, false, true>(a, tile_y, x_chunk, s);
+    }
     if (visc) return force ? launch_step2_rows<T, true, true, MID>(a, tile_y, x_chunk, s)
                            : launch_step2_rows<T, true, false, MID>(a, tile_y, x_chunk, s);
     return force ? launch_step2_rows<T, false, true, MID>(a, tile_y, x_chunk, s)
@@ -358,6 +366,21 @@ int run_steps(sw_fdtd_state* st, int64_t n_steps, const double* env, const sw_fd
     const bool traces = out && out->n_traces > 0;
 
     StepArgs<T> a = step_args<T>(st);
+    a.ihx = (T)(0.5 / st->dx);
+    a.ihy = (T)(0.5 / st->dy);
+    a.ihz = (T)(0.5 / st->dz);
+    // In-loop strain invariants of the state after an output step are computed by the next pass when
+    // that is a force-free two-step pass (it reads the state anyway); otherwise, and after the last
+    // pass of this call, by their own kernel.
+    const bool fuse_strain = pairs && march_fits && out && out->inv_mode == 0;
+    T* inv_pending = nullptr;
+    auto flush_inv = [&]() -> int {
+        if (!inv_pending) return SW_OK;
+        T* base = inv_pending;
+        inv_pending = nullptr;
+        return strain_invariants_in_loop(DType<T>::code, st->u[0], st->u[1], st->u[2], base, g.nx, g.ny, g.nz,
+                                         st->dx, st->dy, st->dz, s);
+    };
 
     for (int64_t it = 0; it < n_steps;) {
         const double fs = env ? env[it] : 0.0;
@@ -378,10 +401,16 @@ int run_steps(sw_fdtd_state* st, int64_t n_steps, const double* env, const sw_fd
             a.Ao[c] = (T*)st->u_next[c];
             a.Bo[c] = (T*)st->b_next[c];
         }
-        int rc = three ? launch_step3<T>(a, visc, st->x_chunk, s)
-                 : two ? launch_step2<T>(a, visc, force, st->tile_y, st->x_chunk, s)
-                       : launch_step_any<T>(a, incr, visc, force, one_kernel, one_tile_y, st->tile_z, st->x_chunk, s);
+        const bool strain = inv_pending && two && !force;
+        int rc = strain ? SW_OK : flush_inv();
         if (rc) return rc;
+        if (strain)
+            for (int k = 0; k < 4; ++k) a.inv[k] = inv_pending + k * n;
+        rc = three ? launch_step3<T>(a, visc, st->x_chunk, s)
+             : two ? launch_step2<T>(a, visc, force, st->tile_y, st->x_chunk, s, strain)
+                   : launch_step_any<T>(a, incr, visc, force, one_kernel, one_tile_y, st->tile_z, st->x_chunk, s);
+        if (rc) return rc;
+        if (strain) inv_pending = nullptr;
         for (int c = 0; c < 3; ++c) {
             if (incr) {
                 std::swap(st->u[c], st->u_next[c]);
@@ -409,10 +438,8 @@ int run_steps(sw_fdtd_state* st, int64_t n_steps, const double* env, const sw_fd
         }
         if (out->inv_stride > 0 && (step + 1) % out->inv_stride == 0) {
             const int64_t k = (step + 1) / out->inv_stride - 1;
-            T* base = (T*)out->inv_out + k * 4 * n;
-            rc = strain_invariants_in_loop(DType<T>::code, st->u[0], st->u[1], st->u[2], base, g.nx, g.ny, g.nz,
-                                           st->dx, st->dy, st->dz, s);
-            if (rc) return rc;
+            inv_pending = (T*)out->inv_out + k * 4 * n;
+            if (!fuse_strain && (rc = flush_inv())) return rc;
         }
         if (traces && two) {
             hipLaunchKernelGGL(gather2_kernel<T>, dim3((unsigned)((out->n_traces + 255) / 256)), dim3(256), 0, s,
@@ -425,7 +452,7 @@ int run_steps(sw_fdtd_state* st, int64_t n_steps, const double* env, const sw_fd
             if (rc) return rc;
         }
     }
-    return SW_OK;
+    return flush_inv();
 }
 
 // One two-step pass from (u, b) into (u_next, b_next) that also stores its

@@ -31,6 +31,7 @@
 #include <type_traits>
 
 #include "march_step.h"
+#include "strain_point.h"
 
 namespace sw {
 
@@ -97,8 +98,12 @@ __device__ __forceinline__ void stage(Plane& L, int r, int q, const Col<T>& c) {
 
 // With MID the pass also stores step n+1 of its own planes (u_{n+1} = u_n +
 // d_{n+1} and d_{n+1}, the values two single steps store) into a.Am, a.Bm.
+// With STRAIN it also stores the in-loop strain invariants of its input
+// state u_n into a.inv (strain_point.h): u_n's x neighbours are in the
+// column's queue and its y/z neighbours in the step-n LDS plane, so the
+// invariants cost their 16 B of stores per voxel and no read.
 // DIST overrides the prefetch distance (0: step2_dist).
-template <typename T, bool VISC, bool FORCE, int BY, bool MID = false, int DIST = 0>
+template <typename T, bool VISC, bool FORCE, int BY, bool MID = false, int DIST = 0, bool STRAIN = false>
 __device__ __forceinline__ void march2_tile(const StepArgs<T>& a, int x0, int x1, int y0, int z0,
                                             Step2Lds<T, VISC, BY>& lds) {
     static_assert(2 * BY + 4 <= 64 && BY + 3 <= 16, "tile rows");
@@ -184,6 +189,22 @@ __device__ __forceinline__ void march2_tile(const StepArgs<T>& a, int x0, int x1
         k[(K + D) & 3] = coef(x + D);
         o[(K + D) & 3] = outer_col(x + D);
         __syncthreads();
+        if constexpr (STRAIN) {  // the invariants of step n at plane x
+            if (out_thread && p_in && x >= x0 && x < x1) {
+                const Col<T>& m = q[(K + QM) & QM];
+                const Col<T>& p = q[(K + 1) & QM];
+                auto dy = [&](int f) { return S0[f][r0 + 1][q0] - S0[f][r0 - 1][q0]; };
+                auto dz = [&](int f) { return S0[f][r0][q0 + 1] - S0[f][r0][q0 - 1]; };
+                const UDiff<T> du{p.u0 - m.u0, p.u1 - m.u1, p.u2 - m.u2, dy(0), dy(1), dy(2), dz(0), dz(1), dz(2)};
+                T vol, dev, ms, vm;
+                inloop_invariants(p_int && x > 0 && x < g.nx - 1, du, a.ihx, a.ihy, a.ihz, vol, dev, ms, vm);
+                const int64_t px = (int64_t)x * P;
+                store_out(&at_byte(a.inv[0] + px, pyz), vol);
+                store_out(&at_byte(a.inv[1] + px, pyz), dev);
+                store_out(&at_byte(a.inv[2] + px, pyz), ms);
+                store_out(&at_byte(a.inv[3] + px, pyz), vm);
+            }
+        }
         // step n+1 at plane x on the tile and its ring
         {
             T i0, i1, i2;

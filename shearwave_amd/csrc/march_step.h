@@ -26,6 +26,8 @@ struct StepArgs {
     T fs2;         // env[it+1] (the second step of a two-step pass)
     T* Am[3];      // u_{n+1} of a two-step pass (its intermediate state, when asked for)
     T* Bm[3];      // d_{n+1} of a two-step pass
+    T* inv[4];     // the strain invariants of u_n (volumetric, deviatoric, max shear, von Mises), when asked for
+    T ihx, ihy, ihz;  // 1 / (2h) per axis, for them
 };
 
 // One (y, z) column's state at one x plane: u and d = u - u_prev.

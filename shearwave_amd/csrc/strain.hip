@@ -4,6 +4,7 @@
 // invariants out, no tensor in memory: 12 + 16 B per voxel in float32).
 #include <type_traits>
 
+#include "strain_point.h"
 #include "sw_common.h"
 
 // bit-for-bit float32 parity with NumPy: no multiply-add contraction in these kernels
@@ -35,20 +36,6 @@ __device__ inline void voigt_at(const T* __restrict__ ux, const T* __restrict__ 
     e[5] = ux_y + uy_x;
 }
 
-// Invariants in the reference's order of operations (strain.py:110-128), in T;
-// von Mises is also returned in double as sqrt(2/3) * (double)dev.
-template <typename T>
-__device__ inline void invariants(const T* e, T& vol, T& dev, T& ms, double& vm) {
-    vol = ((e[0] + e[1]) + e[2]) / T(3);
-    const T dxx = e[0] - vol, dyy = e[1] - vol, dzz = e[2] - vol;
-    const T nrm = (dxx * dxx + dyy * dyy) + dzz * dzz;
-    const T shr = (e[3] * e[3] + e[4] * e[4]) + e[5] * e[5];
-    dev = sqrt(nrm + T(0.5) * shr);
-    const T j2 = T(0.5) * (nrm + T(0.5) * shr);
-    ms = sqrt(j2 > T(0) ? j2 : T(0));
-    vm = 0.816496580927726 * (double)dev;  // sqrt(2/3)
-}
-
 template <typename T>
 __global__ void __launch_bounds__(256) strain_tensor_kernel(const T* __restrict__ ux, const T* __restrict__ uy,
                                                             const T* __restrict__ uz, T* __restrict__ eps, Grid g,
@@ -70,7 +57,7 @@ __global__ void __launch_bounds__(256) strain_invariants_kernel(const T* __restr
         for (int k = 0; k < 6; ++k) e[k] = eps[6 * i + k];
         T a, b, c;
         double d;
-        invariants(e, a, b, c, d);
+        strain_invariants_of(e, a, b, c, d);
         vol[i] = a;
         dev[i] = b;
         ms[i] = c;
@@ -90,7 +77,7 @@ __global__ void __launch_bounds__(256) strain_invariants_from_u_kernel(const T* 
         voigt_at(ux, uy, uz, g, x, y, z, i, h2x, h2y, h2z, e);
         T a, b, c;
         double d;
-        invariants(e, a, b, c, d);
+        strain_invariants_of(e, a, b, c, d);
         vol[i] = a;
         dev[i] = b;
         ms[i] = c;
@@ -177,26 +164,17 @@ __global__ void __launch_bounds__(64 * BY) strain_inv_march_kernel(const T* __re
         vn = col_halo(x + 1);
         __syncthreads();
         if (!inyz) return;
-        T e[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
-        if (yz_interior && x > 0 && x < g.nx - 1) {
-            const int r = ty + 1, q = tz + 1;
-            auto dy = [&](int f) { return (L[f][r + 1][q] - L[f][r - 1][q]) * ihy; };
-            auto dz = [&](int f) { return (L[f][r][q + 1] - L[f][r][q - 1]) * ihz; };
-            e[0] = (p.a - m.a) * ihx;
-            e[1] = dy(1);
-            e[2] = dz(2);
-            e[3] = dz(1) + dy(2);
-            e[4] = dz(0) + (p.c - m.c) * ihx;
-            e[5] = dy(0) + (p.b - m.b) * ihx;
-        }
-        T a, b, sh;
-        double d;
-        invariants(e, a, b, sh, d);
+        const int r = ty + 1, q = tz + 1;
+        auto dy = [&](int f) { return L[f][r + 1][q] - L[f][r - 1][q]; };
+        auto dz = [&](int f) { return L[f][r][q + 1] - L[f][r][q - 1]; };
+        const UDiff<T> d{p.a - m.a, p.b - m.b, p.c - m.c, dy(0), dy(1), dy(2), dz(0), dz(1), dz(2)};
+        T a, b, sh, v;
+        inloop_invariants(yz_interior && x > 0 && x < g.nx - 1, d, ihx, ihy, ihz, a, b, sh, v);
         const int64_t px = (int64_t)x * P;
         store_out(&at_byte(vol + px, yz), a);
         store_out(&at_byte(dev + px, yz), b);
         store_out(&at_byte(ms + px, yz), sh);
-        store_out(&at_byte(vm + px, yz), (T)d);
+        store_out(&at_byte(vm + px, yz), v);
     };
     using B0 = std::integral_constant<int, 0>;
     using B1 = std::integral_constant<int, 1>;

@@ -332,12 +332,16 @@ class Solve:
 
     # ----------------------------------------------------------------- loop
     def run(self, n_steps, env=None, *, snap_stride=0, snap_out=None, inv_stride=0, inv_out=None,
-            trace_idx=None, trace_out=None, finite_stride=0):
+            trace_idx=None, trace_out=None, finite_stride=0, inv_separate=False):
         """Advance n_steps steps; env (host, length n_steps) scales the force per step.
 
         With ``finite_stride`` > 0, u is checked for inf/NaN every that many
         global steps (after the pass that reaches the step), without
-        synchronising; :meth:`first_nonfinite_step` reads the result.
+        synchronising; :meth:`first_nonfinite_step` reads the result. The
+        in-loop strain invariants (``inv_stride``) are computed by the pass
+        after their step where that is a force-free two-step pass, else by
+        their own kernel; ``inv_separate`` always uses their own kernel (the
+        same values).
         """
         n_steps = int(n_steps)
         if n_steps <= 0:
@@ -362,6 +366,7 @@ class Solve:
         if finite_stride:
             out.finite_stride = int(finite_stride)
             out.nonfinite_step = self._nonfinite.data_ptr()
+        out.inv_mode = 1 if inv_separate else 0
         check(lib().sw_fdtd_run(C.byref(self.state), n_steps, env_arr, C.byref(out), stream_ptr()))
         self.step_count += n_steps
 

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
     missing = [s for s in declared_symbols() if not hasattr(lib, s)]
     assert not missing, missing
     lib.sw_abi_version.restype = ctypes.c_int
-    assert lib.sw_abi_version() == 4
+    assert lib.sw_abi_version() == 5
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="HIP library not built")
@@ -76,7 +76,7 @@ def test_struct_layouts_match_header():
     from shearwave_amd import _abi
 
     assert ctypes.sizeof(_abi.CgInfo) == 40  # int64, 2 double, 2 int32, double
-    assert ctypes.sizeof(_abi.FdtdOutputs) == 11 * 8
+    assert ctypes.sizeof(_abi.FdtdOutputs) == 12 * 8
 
 
 def test_product_does_not_import_the_oracle():

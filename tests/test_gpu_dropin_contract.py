@@ -381,3 +381,32 @@ def test_preconditioner_only_where_the_plain_cg_converges(sw):
     assert runs[(1e-5, 200)]["iters"] > 30  # plain CG
     assert runs[(1e-8, 20000)]["iters"] <= 12 and runs[(1e-8, 20000)]["true_relres"] < 1e-8
     assert runs[(1e-8, 40)]["reason"] == "max_iters" and runs[(1e-8, 40)]["iters"] == 40  # plain, stopped early
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+@pytest.mark.parametrize("stride", [4, 5])
+def test_in_loop_strain_fused_into_the_next_pass(sw, dtype, stride):
+    """The in-loop invariants that the next force-free two-step pass computes from its input state equal
+    those of their own kernel bit for bit. Stride 5 puts outputs before single steps, stride 4 before
+    two-step passes; the force of the first steps puts some before forced passes (own kernel), and the
+    last step's invariants come after the last pass (own kernel)."""
+    import torch
+
+    from shearwave_amd import engine as E
+
+    p = _problem(shape=(40, 36, 70), n_steps=60, seed=29)
+    shape = (40, 36, 70)
+    got = []
+    for separate in (True, False):
+        s = E.Solve(shape, p["h"], p["dt"], p["rho"], p["mu"], p["opts"]["eta"], dtype=dtype,
+                    vel_damp=p["opts"]["vel_damp"])
+        s.set_force(None, None, p["bz"], 1e-10, 5000)
+        inv = torch.full((p["n_steps"] // stride, 4) + shape, float("nan"), dtype=E.torch_dtype(dtype),
+                         device=E.device())
+        s.run(p["n_steps"], p["opts"]["env_t"], inv_stride=stride, inv_out=inv, inv_separate=separate)
+        got.append((inv.cpu().numpy(), [t.cpu().numpy() for t in s.u_components()]))
+    (inv_a, u_a), (inv_b, u_b) = got
+    assert np.isfinite(inv_a).all() and float(np.abs(inv_a).max()) > 0
+    np.testing.assert_array_equal(inv_b, inv_a)
+    for a, b in zip(u_a, u_b):
+        np.testing.assert_array_equal(b, a)
