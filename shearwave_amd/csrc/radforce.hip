@@ -22,8 +22,13 @@ constexpr int kEnergyThreads = 256;
 
 // A wave walks whole voxel rows (grid-stride over voxels); lane l keeps the
 // sums of samples tc + l + 64 u, u < 4, of a 256-sample chunk, so one voxel's
-// chunk is four coalesced 256 B loads. The block's four waves are then summed
-// in wave order into the block's partial.
+// chunk is four coalesced 256 B loads, and kEnergyV voxels are loaded per
+// iteration (16 loads in flight per lane). The loads are unconditional:
+// lanes past nt read the chunk's first sample and discard it (a load under a
+// per-lane condition is a branch that waits for it). The block's four waves
+// are then summed in wave order into the block's partial.
+constexpr int kEnergyV = 4;
+
 template <typename T>
 __global__ void __launch_bounds__(kEnergyThreads) energy_partial_kernel(const T* __restrict__ p, int64_t n_vox,
                                                                         int64_t nt, double* __restrict__ part) {
@@ -34,27 +39,35 @@ __global__ void __launch_bounds__(kEnergyThreads) energy_partial_kernel(const T*
     for (int64_t tc = 0; tc < nt; tc += 4 * kWave) {
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
         bool tin[4];
+        int64_t off[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) tin[u] = tc + lane + kWave * u < nt;
+        for (int u = 0; u < 4; ++u) {
+            tin[u] = tc + lane + kWave * u < nt;
+            off[u] = tin[u] ? tc + lane + kWave * u : tc;
+        }
         int64_t v = w0;
-        for (; v + n_waves < n_vox; v += 2 * n_waves) {  // two voxels: eight loads in flight
-            const T* r0 = p + v * nt + tc + lane;
-            const T* r1 = r0 + n_waves * nt;
-            double x[8];
+        for (; v + (kEnergyV - 1) * n_waves < n_vox; v += kEnergyV * n_waves) {
+            T x[kEnergyV][4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                x[u] = tin[u] ? (double)r0[kWave * u] : 0.0;
-                x[4 + u] = tin[u] ? (double)r1[kWave * u] : 0.0;
-            }
+            for (int j = 0; j < kEnergyV; ++j)
 #pragma unroll
-            for (int u = 0; u < 4; ++u) acc[u] += x[u] * x[u] + x[4 + u] * x[4 + u];
+                for (int u = 0; u < 4; ++u) x[j][u] = p[(v + j * n_waves) * nt + off[u]];
+#pragma unroll
+            for (int j = 0; j < kEnergyV; ++j)
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const double d = tin[u] ? (double)x[j][u] : 0.0;
+                    acc[u] += d * d;
+                }
         }
         for (; v < n_vox; v += n_waves) {
-            const T* r0 = p + v * nt + tc + lane;
+            T x[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) x[u] = p[v * nt + off[u]];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const double x = tin[u] ? (double)r0[kWave * u] : 0.0;
-                acc[u] += x * x;
+                const double d = tin[u] ? (double)x[u] : 0.0;
+                acc[u] += d * d;
             }
         }
 #pragma unroll
