@@ -150,7 +150,7 @@ __device__ __forceinline__ void march2_tile(const StepArgs<T>& a, int x0, int x1
 
     auto po = [&](int xx) { return (int64_t)min(max(xx, 0), g.nx - 1) * P; };
     auto col = [&](int xx, uint32_t o) { return load_col<T, true>(a, po(xx), o); };
-    auto coef = [&](int xx) { return load_coef<T, VISC, FORCE>(a, po(xx), pyz); };
+    auto coef = [&](int xx) { return load_coef_cached<T, VISC, FORCE>(a, po(xx), pyz); };
     StepArgs<T> a2 = a;  // step n+2: the second step's force scale
     a2.fs = a.fs2;
     auto outer_col = [&](int xx) {
