@@ -26,10 +26,13 @@ A single drop-in solve (``Tuner.pass_shape``): the two-step pass of one
 grid, timed on the solve's own buffers before its loop (``sw_fdtd_trial_passes``
 reads the state and writes the next-step buffers without advancing the
 state) for every tile-row count the storage type has (`PASS_ROWS`) and every
-chunk depth in `PASS_XCHUNKS` (0: the depth that makes the grid one wave of
-resident blocks), cached per storage type, viscosity and grid shape. Every
-candidate computes each voxel with the same arithmetic, so the choice
-changes the time and never the result.
+depth that cuts the marching axis into `PASS_SPLITS` equal chunks, and 0
+(the depth that makes the grid one wave of resident blocks), cached per
+storage type, viscosity and grid shape. Equal chunks only: a depth that
+leaves a short last chunk leaves the last wave of blocks part idle (at 256^3,
+96-plane chunks ran 35% slower than 128-plane ones). Every candidate computes
+each voxel with the same arithmetic, so the choice changes the time and never
+the result.
 """
 
 from __future__ import annotations
@@ -45,7 +48,8 @@ DEFAULT_CAP = 96  # the measured best on the 48-solve mix; used without a tuner
 DEFAULT_ROWS = 8
 
 PASS_ROWS = {"float32": (8, 12), "float64": (4,)}
-PASS_XCHUNKS = (0, 32, 48, 64, 96, 128, 192, 256)
+PASS_SPLITS = (1, 2, 3, 4, 6, 8, 12, 16)  # chunks along the marching axis
+PASS_MIN_XCHUNK = 8
 
 # The tuning mix: 24 grids with stored extents drawn from {64, 72, .., 256} (seed 11).
 TUNE_MIX_SEED = 11
@@ -73,13 +77,13 @@ def batch_candidates():
 
 
 def pass_candidates(dtype, shape):
-    """(tile rows, x chunk) pairs the drop-in tuner times for a grid: chunk depths at or beyond nx are one
-    candidate (whole columns)."""
+    """(tile rows, x chunk) pairs the drop-in tuner times for a grid: 0 (one wave of resident blocks) and
+    the depths of equal splits of the marching axis, at least PASS_MIN_XCHUNK planes, each once."""
     nx = int(shape[0])
-    xcs = []
-    for xc in PASS_XCHUNKS:
-        v = min(xc, nx) if xc else 0
-        if v not in xcs:
+    xcs = [0]
+    for k in PASS_SPLITS:
+        v = math.ceil(nx / k)
+        if v >= min(nx, PASS_MIN_XCHUNK) and v not in xcs:
             xcs.append(v)
     return [(r, xc) for r in PASS_ROWS[dtype] for xc in xcs]
 
@@ -187,7 +191,7 @@ class Tuner:
     def _save(self):
         try:
             os.makedirs(os.path.dirname(self.path), exist_ok=True)
-            tmp = self.path + ".tmp"
+            tmp = f"{self.path}.{os.getpid()}.tmp"  # processes sharing the file each write their own
             with open(tmp, "w") as f:
                 json.dump(self._table, f, indent=1, sort_keys=True)
             os.replace(tmp, self.path)
@@ -237,13 +241,15 @@ def _time_mix(cands, n_passes=6, rounds=2):
     return best
 
 
-def _time_pass(solve, cands, rounds=2, target_s=2e-3):
+def _time_pass(solve, cands, rounds=1, target_s=1e-3):
     """Seconds per voxel-step of the two-step pass on `solve`'s buffers, per (tile rows, x chunk).
 
-    Each candidate runs one untimed pass, then enough timed passes for about
-    `target_s` (at least two), `rounds` times interleaved; the best time is
-    kept. The passes read the solve's state and write its next-step buffers
-    only (``sw_fdtd_trial_passes``), so the solve is unchanged.
+    After a few untimed passes (clocks up), each candidate runs one untimed
+    pass, then enough timed passes for about `target_s` (at least two),
+    `rounds` times interleaved; the best time is kept. The passes read the
+    solve's state and write its next-step buffers only
+    (``sw_fdtd_trial_passes``), so the solve is unchanged. About 25 ms at
+    256^3 for the 18 float32 candidates.
     """
     import ctypes as C
 
@@ -256,6 +262,8 @@ def _time_pass(solve, cands, rounds=2, target_s=2e-3):
     vox = int(solve.n)
     n_passes = max(2, min(50, int(target_s * 150e9 / (2 * vox))))  # at about 150 Gvoxel-steps/s
     st = E.stream_ptr()
+    s0 = _abi.FdtdState.from_buffer_copy(solve.state)
+    _abi.check(lib.sw_fdtd_trial_passes(C.byref(s0), 3, st))
     best = {}
     for _ in range(rounds):
         for rows, xc in cands:

@@ -107,7 +107,7 @@ __device__ __forceinline__ void march3_tile(const StepArgs<T>& a, int x0, int x1
 
     auto po = [&](int xx) { return (int64_t)min(max(xx, 0), g.nx - 1) * P; };
     auto col = [&](int xx, uint32_t o) { return load_col<T, true>(a, po(xx), o); };
-    auto coef = [&](int xx) { return load_coef<T, VISC, FORCE>(a, po(xx), pyz); };
+    auto coef = [&](int xx) { return load_coef_cached<T, VISC, FORCE>(a, po(xx), pyz); };
     auto outer_col = [&](int xx) {
         Col<T> c{};
         if (outer) c = col(xx, oyz);

@@ -7,7 +7,7 @@ import textwrap
 
 import pytest
 
-from shearwave_amd.autotune import BATCH_ROWS, CAPS, PASS_ROWS, PASS_XCHUNKS, Tuner, balanced_chunk, batch_candidates, \
+from shearwave_amd.autotune import BATCH_ROWS, CAPS, PASS_ROWS, PASS_SPLITS, Tuner, balanced_chunk, batch_candidates, \
     pass_candidates, tune_mix_shapes
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -69,10 +69,13 @@ class _FakeSolve:
 
 def test_pass_candidates():
     c = pass_candidates("float32", (256, 256, 256))
-    assert {r for r, _ in c} == set(PASS_ROWS["float32"]) and (8, 0) in c and (12, 256) in c
-    assert len(c) == len(set(c)) == len(PASS_ROWS["float32"]) * len(PASS_XCHUNKS)
-    c = pass_candidates("float32", (40, 64, 64))  # chunk depths at or beyond nx collapse to whole columns
-    assert sorted({xc for _, xc in c}) == [0, 32, 40] and len(c) == len(set(c))
+    assert {r for r, _ in c} == set(PASS_ROWS["float32"]) and len(c) == len(set(c))
+    assert sorted({xc for _, xc in c}) == [0, 16, 22, 32, 43, 64, 86, 128, 256]  # one wave, and equal splits
+    for _, xc in c:  # every depth but 0 cuts the axis into equal chunks
+        assert xc == 0 or any(-(-256 // k) == xc for k in PASS_SPLITS)
+    c = pass_candidates("float32", (40, 64, 64))  # splits below PASS_MIN_XCHUNK planes are left out
+    assert sorted({xc for _, xc in c}) == [0, 10, 14, 20, 40]
+    assert sorted({xc for _, xc in pass_candidates("float32", (6, 64, 64))}) == [0, 6]
     assert {r for r, _ in pass_candidates("float64", (64, 64, 64))} == set(PASS_ROWS["float64"])
 
 
@@ -81,7 +84,7 @@ def test_pass_shape_per_grid_class_and_cached(tmp_path):
 
     def fake(solve, cands):
         calls.append((solve.shape, tuple(cands)))
-        want = (12, 64) if solve.shape[0] == 256 else (8, 0)
+        want = (12, 64) if solve.shape[0] == 256 else (8, 0)  # both among the candidates of these shapes
         return {c: (1.0 if c == want else 2.0 + c[1]) for c in cands}
 
     path = str(tmp_path / "t.json")
