@@ -200,6 +200,95 @@ __global__ void __launch_bounds__(256) strain_peak_kernel(const T* __restrict__ 
     }
 }
 
+// peak / rms / mean over a contiguous gate read 16 bytes per lane
+// (vec16_gate): lane sl of a voxel's S-lane segment reads chunks sl, sl + S,
+// ... of Vec16<T>::n samples, two chunks and kPeakVU voxel groups per
+// iteration (four 16-byte loads in flight per lane), unconditional loads
+// (chunk clamped, values past the gate discarded). One maximum and two float64
+// partials per vector element, combined in a fixed order; the maximum is the
+// same in any order, NaN included.
+constexpr int kPeakVU = 2;
+
+template <typename T>
+__global__ void __launch_bounds__(256) strain_peak_vec_kernel(const T* __restrict__ p, int64_t n_vox, int64_t nt,
+                                                              int64_t g0, int64_t n_gate, int S,
+                                                              const T* __restrict__ c, const T* __restrict__ rho,
+                                                              T* __restrict__ peak, T* __restrict__ rms,
+                                                              T* __restrict__ mean) {
+    using V = typename Vec16<T>::type;
+    constexpr int N = Vec16<T>::n;
+    const Seg q = seg_of(S);
+    const int64_t gsz = 64 / S;
+    const int64_t nch = n_gate / N, ntv = nt / N, g0v = g0 / N;
+    const V* pv = reinterpret_cast<const V*>(p);
+    for (int64_t wv = q.first; wv * gsz < n_vox; wv += kPeakVU * q.stride) {  // uniform in the wave
+        int64_t v[kPeakVU], base[kPeakVU];
+        bool ok[kPeakVU];
+#pragma unroll
+        for (int g = 0; g < kPeakVU; ++g) {
+            v[g] = (wv + g * q.stride) * gsz + q.seg;
+            ok[g] = v[g] < n_vox;
+            base[g] = (ok[g] ? v[g] : 0) * ntv + g0v;
+        }
+        T mx[kPeakVU][N];
+        double s2[kPeakVU][N] = {}, s1[kPeakVU][N] = {};
+#pragma unroll
+        for (int g = 0; g < kPeakVU; ++g)
+#pragma unroll
+            for (int j = 0; j < N; ++j) mx[g][j] = T(0);
+        for (int64_t c0 = 0; c0 < nch; c0 += 2 * S) {  // uniform in the wave
+            const int64_t ca = c0 + q.sl, cb = c0 + S + q.sl;
+            const bool ia = ca < nch, ib = cb < nch;
+            const int64_t la = ia ? ca : nch - 1, lb = ib ? cb : nch - 1;
+            V xa[kPeakVU], xb[kPeakVU];
+#pragma unroll
+            for (int g = 0; g < kPeakVU; ++g) {
+                xa[g] = pv[base[g] + la];
+                xb[g] = pv[base[g] + lb];
+            }
+#pragma unroll
+            for (int g = 0; g < kPeakVU; ++g)
+#pragma unroll
+                for (int j = 0; j < N; ++j) {
+                    if (ia) {
+                        const T x = xa[g][j];
+                        mx[g][j] = nan_max_t(mx[g][j], x < T(0) ? -x : x);
+                        s2[g][j] += (double)x * (double)x;
+                        s1[g][j] += (double)x;
+                    }
+                    if (ib) {
+                        const T x = xb[g][j];
+                        mx[g][j] = nan_max_t(mx[g][j], x < T(0) ? -x : x);
+                        s2[g][j] += (double)x * (double)x;
+                        s1[g][j] += (double)x;
+                    }
+                }
+        }
+#pragma unroll
+        for (int g = 0; g < kPeakVU; ++g) {
+            T m = mx[g][0];
+            double a2 = s2[g][0], a1 = s1[g][0];
+#pragma unroll
+            for (int j = 1; j < N; ++j) {
+                m = nan_max_t(m, mx[g][j]);
+                a2 += s2[g][j];
+                a1 += s1[g][j];
+            }
+            for (int off = S >> 1; off > 0; off >>= 1) {
+                m = nan_max_t(m, __shfl_xor(m, off, kWave));
+                a2 += __shfl_xor(a2, off, kWave);
+                a1 += __shfl_xor(a1, off, kWave);
+            }
+            if (!ok[g] || q.sl != 0) continue;
+            const T K = bulk(rho, c, v[g]);
+            const double n = (double)n_gate;
+            peak[v[g]] = m / K;                        // acoustic_strain.py:143
+            rms[v[g]] = (T)std::sqrt((T)(a2 / n)) / K;  // acoustic_strain.py:146
+            mean[v[g]] = -(T)(a1 / n) / K;              // acoustic_strain.py:149
+        }
+    }
+}
+
 #ifndef SW_GRAD_U
 #define SW_GRAD_U 2
 #endif
@@ -465,11 +554,19 @@ int sw_acoustic_strain_peak(int dtype, const void* p, int64_t n_vox, int64_t nt,
     if ((rc = check_record(n_vox, nt, gate, gate_start, n_gate))) return rc;
     SW_REQUIRE(p && c && rho && peak && rms && mean, SW_ERR_ARG, "null pointer");
     hipStream_t s = (hipStream_t)stream;
-    const int S = gate_lanes(n_gate, 16);
-    SW_DISPATCH(dtype, T,
-                hipLaunchKernelGGL(strain_peak_kernel<T>, dim3(segment_blocks(n_vox, S)), dim3(256), 0, s,
-                                   (const T*)p, n_vox, nt, gate, gate_start, n_gate, S, (const T*)c, (const T*)rho,
-                                   (T*)peak, (T*)rms, (T*)mean));
+    SW_DISPATCH(dtype, T, {
+        if (!gate && vec16_gate<T>(p, nt, gate_start, n_gate)) {
+            const int S = gate_lanes(n_gate / Vec16<T>::n, 16);
+            hipLaunchKernelGGL(strain_peak_vec_kernel<T>, dim3(segment_blocks(n_vox, S)), dim3(256), 0, s,
+                               (const T*)p, n_vox, nt, gate_start, n_gate, S, (const T*)c, (const T*)rho, (T*)peak,
+                               (T*)rms, (T*)mean);
+        } else {
+            const int S = gate_lanes(n_gate, 16);
+            hipLaunchKernelGGL(strain_peak_kernel<T>, dim3(segment_blocks(n_vox, S)), dim3(256), 0, s,
+                               (const T*)p, n_vox, nt, gate, gate_start, n_gate, S, (const T*)c, (const T*)rho,
+                               (T*)peak, (T*)rms, (T*)mean);
+        }
+    });
     SW_LAUNCH_CHECK();
     return SW_OK;
 }

@@ -196,6 +196,76 @@ __global__ void __launch_bounds__(256) radforce_kernel(const T* __restrict__ p, 
     }
 }
 
+// The force pass over a contiguous gate whose samples can be read 16 bytes
+// per lane (vec16_gate): lane sl of a voxel's S-lane segment reads chunks
+// sl, sl + S, ... of Vec16<T>::n samples each, kForceVU voxel groups and two
+// chunks per iteration, so four 16-byte loads are in flight per lane (a
+// voxel's 512 B of gated float32 samples are two loads of 256 B per segment).
+// The loads are unconditional (chunk index clamped, values past the gate
+// discarded). One float64 partial per vector element, summed in a fixed
+// order, then the butterfly over the segment.
+constexpr int kForceVU = 2;
+
+template <typename T>
+__global__ void __launch_bounds__(256) radforce_vec_kernel(const T* __restrict__ p, int64_t n_vox, int64_t nt,
+                                                           int64_t g0, int64_t n_gate, int S,
+                                                           const T* __restrict__ c, const T* __restrict__ rho,
+                                                           const T* __restrict__ alpha, double f0_mhz,
+                                                           T* __restrict__ o0) {
+    using V = typename Vec16<T>::type;
+    constexpr int N = Vec16<T>::n;
+    const int lane = threadIdx.x & 63;
+    const int seg = lane / S, sl = lane % S;
+    const int per_wave = 64 / S;
+    const int64_t n_waves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int64_t nch = n_gate / N, ntv = nt / N, g0v = g0 / N;
+    const V* pv = reinterpret_cast<const V*>(p);
+    for (int64_t wv = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; wv * per_wave < n_vox;
+         wv += kForceVU * n_waves) {  // uniform in the wave
+        int64_t v[kForceVU], base[kForceVU];
+        bool ok[kForceVU];
+#pragma unroll
+        for (int g = 0; g < kForceVU; ++g) {
+            v[g] = (wv + g * n_waves) * per_wave + seg;
+            ok[g] = v[g] < n_vox;
+            base[g] = (ok[g] ? v[g] : 0) * ntv + g0v;
+        }
+        double s[kForceVU][N] = {};
+        for (int64_t c0 = 0; c0 < nch; c0 += 2 * S) {  // uniform in the wave
+            const int64_t ca = c0 + sl, cb = c0 + S + sl;
+            const bool ia = ca < nch, ib = cb < nch;
+            const int64_t la = ia ? ca : nch - 1, lb = ib ? cb : nch - 1;
+            V xa[kForceVU], xb[kForceVU];
+#pragma unroll
+            for (int g = 0; g < kForceVU; ++g) {
+                xa[g] = pv[base[g] + la];
+                xb[g] = pv[base[g] + lb];
+            }
+#pragma unroll
+            for (int g = 0; g < kForceVU; ++g)
+#pragma unroll
+                for (int j = 0; j < N; ++j) {
+                    const double a = ia ? (double)xa[g][j] : 0.0, b = ib ? (double)xb[g][j] : 0.0;
+                    s[g][j] += a * a;
+                    s[g][j] += b * b;
+                }
+        }
+#pragma unroll
+        for (int g = 0; g < kForceVU; ++g) {
+            double a0 = s[g][0];
+#pragma unroll
+            for (int j = 1; j < N; ++j) a0 += s[g][j];
+            for (int off = S >> 1; off > 0; off >>= 1) a0 += __shfl_xor(a0, off, kWave);  // within the segment
+            if (!ok[g] || sl != 0) continue;
+            const int64_t vg = v[g];
+            const double cc = (double)c[vg];
+            const double anp = ((double)alpha[vg] * f0_mhz) / kDbPerNeper * 100.0;  // radiation_force.py:78-79
+            const double isppa = (a0 / (double)n_gate) / ((double)rho[vg] * cc);  // radiation_force.py:74-75
+            o0[vg] = (T)(2.0 * anp * isppa / cc);                                  // radiation_force.py:82
+        }
+    }
+}
+
 int check_sizes(int64_t n_vox, int64_t nt) {
     SW_REQUIRE(n_vox >= 1 && nt >= 1, SW_ERR_ARG, "n_vox (%lld) and nt (%lld) must be positive", (long long)n_vox,
                (long long)nt);
@@ -215,6 +285,18 @@ template <typename T, bool VEL>
 void launch_force(const T* p, const T* vx, const T* vy, const T* vz, int64_t n_vox, int64_t nt, const int32_t* gate,
                   int64_t g0, int64_t n_gate, const T* c, const T* rho, const T* alpha, double f0, T* o0, T* o1,
                   T* o2, hipStream_t s) {
+    if constexpr (!VEL) {
+        if (!gate && vec16_gate<T>(p, nt, g0, n_gate)) {
+            const int S = segment_lanes(n_gate / Vec16<T>::n);
+            const int64_t waves = (n_vox + (64 / S) - 1) / (64 / S);
+            int64_t blocks = (waves + 4 * kForceVU - 1) / (4 * kForceVU);
+            const int64_t cap = (int64_t)device_cu_count() * 32;
+            if (blocks > cap) blocks = cap;
+            hipLaunchKernelGGL(radforce_vec_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, s, p, n_vox, nt, g0,
+                               n_gate, S, c, rho, alpha, f0 / 1e6, o0);
+            return;
+        }
+    }
     const int S = segment_lanes(n_gate);
     const int64_t waves = (n_vox + (64 / S) - 1) / (64 / S);
     int64_t blocks = (waves + 3) / 4;

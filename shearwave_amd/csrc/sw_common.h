@@ -236,6 +236,29 @@ struct DType<double> {
     static constexpr int code = SW_F64;
 };
 
+// 16 bytes of consecutive samples, loaded by one lane in one instruction:
+// four float32 or two float64 values.
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    typedef float type __attribute__((ext_vector_type(4)));
+    static constexpr int n = 4;
+};
+template <>
+struct Vec16<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+    static constexpr int n = 2;
+};
+
+// Whether a record of nt samples per voxel at p can be read in 16-byte runs
+// along t over the contiguous gate [g0, g0 + n_gate).
+template <typename T>
+inline bool vec16_gate(const void* p, int64_t nt, int64_t g0, int64_t n_gate) {
+    constexpr int N = Vec16<T>::n;
+    return ((uintptr_t)p % 16 == 0) && nt % N == 0 && g0 % N == 0 && n_gate % N == 0 && n_gate > 0;
+}
+
 }  // namespace sw
 
 #define SW_DISPATCH(dtype, T, ...)       \

@@ -59,8 +59,13 @@ def test_gradient_statistics_bit_exact(sw, dtype, gate):
         np.testing.assert_array_equal(got[k], want[k])
 
 
+# contiguous gates on 16-byte boundaries (start and length multiples of four samples) take the kernel that
+# reads 16 bytes per lane: 8, 10 and 6 chunks of four float32 samples per voxel
+ALIGNED_GATES = [slice(8, 40), slice(0, 40), slice(4, 28)]
+
+
 @pytest.mark.parametrize("dtype,tol", [(np.float32, 1e-6), (np.float64, 1e-12)])
-@pytest.mark.parametrize("gate", GATES)
+@pytest.mark.parametrize("gate", GATES + ALIGNED_GATES)
 def test_peak_rms_mean(sw, dtype, tol, gate):
     p, c, rho = record((12, 9, 10), 40, 3, dtype)
     got = sw.compute_acoustic_strain_peak(p, c, rho, gate)

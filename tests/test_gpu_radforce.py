@@ -65,6 +65,22 @@ def test_explicit_gates(sw, gate):
     assert rel(b, ref) < 1e-12
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("nt,gate", [(256, slice(64, 192)), (64, slice(8, 40)), (40, slice(0, 40)), (48, slice(4, 28)),
+                                     (24, slice(4, 8))])
+def test_contiguous_aligned_gates(sw, dtype, nt, gate):
+    """Contiguous gates on 16-byte boundaries take the kernel that reads 16 bytes per lane (32, 8, 10, 6 and
+    1 chunks of four float32 samples per voxel): the force of the oracle (float32 to the tolerance of its
+    float32 sums)."""
+    shape = (9, 8, 11)
+    p = field(shape, nt, 6, dtype)
+    c, rho, alpha = maps(shape, dtype=dtype)
+    b = sw.compute_radiation_force(p, c, rho, alpha, 1.0e6, gate)
+    ref = R.compute_radiation_force(p, c, rho, alpha, 1.0e6, gate)
+    assert b.dtype == ref.dtype and b.shape == shape
+    assert rel(b, ref) < (1e-12 if dtype == np.float64 else 1e-6), rel(b, ref)
+
+
 def test_empty_gate_is_nan_like_numpy(sw):
     shape, nt = (4, 4, 4), 10
     p = field(shape, nt, 5, np.float64)
