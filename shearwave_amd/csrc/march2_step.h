@@ -70,6 +70,13 @@ __device__ __forceinline__ void march_incl(int xb, int xe, F&& body) {
     }
 }
 
+// Slot of a ring of N entries for the plane `off` planes from the current
+// one, in iteration K of a loop unrolled by a multiple of N.
+template <int N>
+constexpr int ring(int k) {
+    return ((k % N) + N) % N;
+}
+
 // Planes of step n loaded ahead of the one being staged (1 or 2; the
 // coefficient ring holds x-1 .. x+2). A block holds one CU on its own, so its
 // waves must keep several planes of loads in flight; two planes fit the
@@ -159,17 +166,23 @@ __device__ __forceinline__ void march2_tile(const StepArgs<T>& a, int x0, int x1
         return c;
     };
 
-    // rings, slot (plane - xb) & (QN-1) (q) or & 3 (the others), xb = x0 - 1 the first iteration
+    // rings: a plane p sits in slot (p - xb) mod (ring size), xb = x0 - 1 the first
+    // iteration; the loop is unrolled by QN, a multiple of every ring size, so
+    // each slot is a compile-time index
+    // (a third plane of prefetch, with rings of 6, measured no faster at 256^3)
     constexpr int D = DIST > 0 ? DIST : step2_dist<T, BY, FORCE>();
     static_assert(D >= 1 && D <= 2, "two-step prefetch distance");
-    constexpr int QN = D == 1 ? 4 : 8, QM = QN - 1;  // the loop is unrolled by QN
-    Col<T> q[QN];  // step n of this column: planes x-1 .. x+1+D
-    Coef<T> k[4];  // coefficients: planes x-1 .. x+D
-    Col<T> s[4];   // step n+1 of this column: planes x-2 .. x
-    Col<T> o[4];   // step n at the outer point: planes x .. x+D
+    constexpr int QN = D == 1 ? 4 : 8;  // unroll; ring of q: planes x-1 .. x+1+D
+    constexpr int KN = 4;               // ring of k (planes x-1 .. x+D) and o (x .. x+D)
+    constexpr int SN = 4;               // ring of s: planes x-2 .. x
+    static_assert(QN >= D + 3 && KN >= D + 2 && SN >= 3 && QN % KN == 0 && QN % SN == 0 && QN % 2 == 0, "rings");
+    Col<T> q[QN];   // step n of this column: planes x-1 .. x+1+D
+    Coef<T> k[KN];  // coefficients: planes x-1 .. x+D
+    Col<T> s[SN];   // step n+1 of this column: planes x-2 .. x
+    Col<T> o[KN];   // step n at the outer point: planes x .. x+D
     const int xb = x0 - 1;
 #pragma unroll
-    for (int j = -1; j <= D; ++j) q[(j + QN) & QM] = col(x0 - 1 + j, pyz);  // planes x0-2 .. x0-1+D
+    for (int j = -1; j <= D; ++j) q[ring<QN>(j)] = col(x0 - 1 + j, pyz);  // planes x0-2 .. x0-1+D
 #pragma unroll
     for (int j = 0; j < D; ++j) {
         k[j] = coef(x0 - 1 + j);
@@ -183,16 +196,16 @@ __device__ __forceinline__ void march2_tile(const StepArgs<T>& a, int x0, int x1
         auto& S1 = lds.s1[B];
         // stage step n of plane x
         if (prim) stage<T, VISC>(S0, r0, q0, q[K]);
-        if (outer) stage<T, VISC>(S0, ro, qo, o[K & 3]);
+        if (outer) stage<T, VISC>(S0, ro, qo, o[ring<KN>(K)]);
         // prefetch: step n at x+1+D, coefficients and outer point at x+D
-        q[(K + 1 + D) & QM] = col(x + 1 + D, pyz);
-        k[(K + D) & 3] = coef(x + D);
-        o[(K + D) & 3] = outer_col(x + D);
+        q[ring<QN>(K + 1 + D)] = col(x + 1 + D, pyz);
+        k[ring<KN>(K + D)] = coef(x + D);
+        o[ring<KN>(K + D)] = outer_col(x + D);
         __syncthreads();
         if constexpr (STRAIN) {  // the invariants of step n at plane x
             if (out_thread && p_in && x >= x0 && x < x1) {
-                const Col<T>& m = q[(K + QM) & QM];
-                const Col<T>& p = q[(K + 1) & QM];
+                const Col<T>& m = q[ring<QN>(K - 1)];
+                const Col<T>& p = q[ring<QN>(K + 1)];
                 auto dy = [&](int f) { return S0[f][r0 + 1][q0] - S0[f][r0 - 1][q0]; };
                 auto dz = [&](int f) { return S0[f][r0][q0 + 1] - S0[f][r0][q0 - 1]; };
                 const UDiff<T> du{p.u0 - m.u0, p.u1 - m.u1, p.u2 - m.u2, dy(0), dy(1), dy(2), dz(0), dz(1), dz(2)};
@@ -208,7 +221,8 @@ __device__ __forceinline__ void march2_tile(const StepArgs<T>& a, int x0, int x1
         // step n+1 at plane x on the tile and its ring
         {
             T i0, i1, i2;
-            point_inc<T, VISC, FORCE>(a, q[(K + QM) & QM], q[K], q[(K + 1) & QM], S0, r0, q0, k[K & 3], i0, i1, i2);
+            point_inc<T, VISC, FORCE>(a, q[ring<QN>(K - 1)], q[K], q[ring<QN>(K + 1)], S0, r0, q0, k[ring<KN>(K)], i0,
+                                      i1, i2);
             const bool in = p_int && x > 0 && x < g.nx - 1;
             Col<T> n;
             n.u0 = in ? q[K].u0 + i0 : T(0);
@@ -217,7 +231,7 @@ __device__ __forceinline__ void march2_tile(const StepArgs<T>& a, int x0, int x1
             n.d0 = in ? i0 : T(0);
             n.d1 = in ? i1 : T(0);
             n.d2 = in ? i2 : T(0);
-            s[K & 3] = n;
+            s[ring<SN>(K)] = n;
             if (prim) stage<T, VISC>(S1, r1, q1, n);
             if constexpr (MID) {
                 if (out_thread && p_in && x >= x0 && x < x1) {
@@ -234,13 +248,14 @@ __device__ __forceinline__ void march2_tile(const StepArgs<T>& a, int x0, int x1
         // step n+2 at plane x-1 (staged in the other step-n+1 buffer one plane ago)
         if (out_thread && x - 1 >= x0 && p_in) {
             auto& S1p = lds.s1[B ^ 1];
-            const Col<T>& c = s[(K + 3) & 3];
+            const Col<T>& c = s[ring<SN>(K - 1)];
             T i0, i1, i2;
-            point_inc<T, VISC, FORCE>(a2, s[(K + 2) & 3], c, s[K & 3], S1p, r1, q1, k[(K + 3) & 3], i0, i1, i2);
+            point_inc<T, VISC, FORCE>(a2, s[ring<SN>(K - 2)], c, s[ring<SN>(K)], S1p, r1, q1, k[ring<KN>(K - 1)], i0,
+                                      i1, i2);
             const bool in = p_int && x - 1 > 0 && x - 1 < g.nx - 1;
             if (!in) i0 = i1 = i2 = T(0);
             const int64_t px = (int64_t)(x - 1) * P;
-            const Col<T>& un = q[(K + QM) & QM];  // step n of plane x-1
+            const Col<T>& un = q[ring<QN>(K - 1)];  // step n of plane x-1
             store_out(&at_byte(a.Ao[0] + px, pyz), in ? un.u0 + (c.d0 + i0) : T(0));
             store_out(&at_byte(a.Ao[1] + px, pyz), in ? un.u1 + (c.d1 + i1) : T(0));
             store_out(&at_byte(a.Ao[2] + px, pyz), in ? un.u2 + (c.d2 + i2) : T(0));
