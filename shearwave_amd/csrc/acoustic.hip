@@ -145,6 +145,52 @@ __global__ void __launch_bounds__(256) volumetric_kernel(const T* __restrict__ p
     }
 }
 
+// The volumetric strain over a contiguous gate read and written 16 bytes per
+// lane (vec16_gate, 16-byte aligned output): chunks of Vec16<T>::n samples,
+// kVolU voxel groups per iteration, the loop over chunks uniform in the wave;
+// the same arithmetic per sample as volumetric_kernel.
+template <typename T>
+__global__ void __launch_bounds__(256) volumetric_vec_kernel(const T* __restrict__ p, int64_t n_vox, int64_t nt,
+                                                             int64_t g0, int64_t n_gate, int S,
+                                                             const T* __restrict__ c, const T* __restrict__ rho,
+                                                             T* __restrict__ out) {
+    using V = typename Vec16<T>::type;
+    constexpr int N = Vec16<T>::n;
+    const int64_t nch = n_gate / N, ntv = nt / N, g0v = g0 / N;
+    const V* pv = reinterpret_cast<const V*>(p);
+    V* ov = reinterpret_cast<V*>(out);
+    const Seg q = seg_of(S);
+    const int64_t gsz = 64 / S;
+    for (int64_t wv = q.first; wv * gsz < n_vox; wv += kVolU * q.stride) {  // uniform in the wave
+        int64_t v[kVolU];
+        bool ok[kVolU];
+        T K[kVolU];
+#pragma unroll
+        for (int u = 0; u < kVolU; ++u) {
+            v[u] = (wv + u * q.stride) * gsz + q.seg;
+            ok[u] = v[u] < n_vox;
+            if (!ok[u]) v[u] = 0;  // read voxel 0, store nothing
+            K[u] = bulk(rho, c, v[u]);
+        }
+        for (int64_t c0 = 0; c0 < nch; c0 += S) {  // uniform in the wave
+            const int64_t ch = c0 + q.sl;
+            const bool live = ch < nch;
+            const int64_t cl = live ? ch : nch - 1;
+            V x[kVolU];
+#pragma unroll
+            for (int u = 0; u < kVolU; ++u) x[u] = pv[v[u] * ntv + g0v + cl];
+#pragma unroll
+            for (int u = 0; u < kVolU; ++u) {
+                if (!ok[u] || !live) continue;
+                V r;
+#pragma unroll
+                for (int j = 0; j < N; ++j) r[j] = -x[u][j] / K[u];  // acoustic_strain.py:104
+                __builtin_nontemporal_store(r, ov + v[u] * nch + ch);
+            }
+        }
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ T nan_max_t(T m, T x) {
     return (x > m || x != x) ? x : m;  // NaN propagates, as np.max / np.maximum
@@ -349,6 +395,82 @@ __global__ void __launch_bounds__(256) strain_gradient_kernel(const T* __restric
     }
 }
 
+// The gradient over a contiguous gate read and written 16 bytes per lane
+// (vec16_gate, and 16-byte aligned outputs): lane sl of a voxel's S-lane
+// segment takes chunks sl, sl + S, ... of Vec16<T>::n samples, reading them
+// from the six neighbour records and writing the three components as one
+// 16-byte store each. The loop over chunks is uniform in the wave (chunk
+// index clamped, stores only for chunks in the gate); the arithmetic per
+// sample is the scalar kernel's, so the values are the same bit for bit.
+template <typename T>
+__global__ void __launch_bounds__(256) strain_gradient_vec_kernel(const T* __restrict__ p, Grid g, int64_t nt,
+                                                                  int64_t g0, int64_t n_gate, int S,
+                                                                  const T* __restrict__ c, const T* __restrict__ rho,
+                                                                  T h2x, T h2y, T h2z, T* __restrict__ gx,
+                                                                  T* __restrict__ gy, T* __restrict__ gz) {
+    using V = typename Vec16<T>::type;
+    constexpr int N = Vec16<T>::n;
+    const int64_t P = g.plane(), R = g.nz;
+    const int64_t nch = n_gate / N, ntv = nt / N, g0v = g0 / N;
+    const V* pv = reinterpret_cast<const V*>(p);
+    V* ox = reinterpret_cast<V*>(gx);
+    V* oy = reinterpret_cast<V*>(gy);
+    V* oz = reinterpret_cast<V*>(gz);
+    const Seg q = seg_of(S);
+    const Order w = order_walk(g, S);
+    for (int64_t wv = w.first; wv < w.end; wv += kGradU * w.stride) {  // uniform in the wave
+        int64_t v[kGradU], nb[kGradU][6];
+        bool ok[kGradU], in[kGradU];
+        T K[kGradU][6];
+#pragma unroll
+        for (int u = 0; u < kGradU; ++u) {
+            int x = 0, y = 0, z = 0;
+            ok[u] = wv + u * w.stride < w.end && order_voxel((wv + u * w.stride) * (64 / S) + q.seg, g, w.ntz, x, y, z);
+            v[u] = ok[u] ? g.at(x, y, z) : 0;
+            in[u] = ok[u] && g.interior(x, y, z);
+            // the six neighbours (x+, x-, y+, y-, z+, z-); off the interior the voxel itself (unused)
+            const int64_t off[6] = {in[u] ? P : 0, in[u] ? -P : 0, in[u] ? R : 0,
+                                    in[u] ? -R : 0, in[u] ? 1 : 0, in[u] ? -1 : 0};
+#pragma unroll
+            for (int n = 0; n < 6; ++n) {
+                K[u][n] = bulk(rho, c, v[u] + off[n]);
+                nb[u][n] = (v[u] + off[n]) * ntv + g0v;
+            }
+        }
+        for (int64_t c0 = 0; c0 < nch; c0 += S) {  // uniform in the wave
+            const int64_t ch = c0 + q.sl;
+            const bool live = ch < nch;
+            const int64_t cl = live ? ch : nch - 1;
+            V x[kGradU][6];
+#pragma unroll
+            for (int u = 0; u < kGradU; ++u)
+#pragma unroll
+                for (int n = 0; n < 6; ++n) x[u][n] = pv[nb[u][n] + cl];
+#pragma unroll
+            for (int u = 0; u < kGradU; ++u) {
+                if (!ok[u] || !live) continue;
+                const int64_t o = v[u] * nch + ch;
+                V rx, ry, rz;
+#pragma unroll
+                for (int j = 0; j < N; ++j) {
+                    if (!in[u]) {
+                        rx[j] = ry[j] = rz[j] = T(0);
+                        continue;
+                    }
+                    // strain -p / K at the neighbours (acoustic_strain.py:196), gradient_center (solver.py:726-728)
+                    rx[j] = (-x[u][0][j] / K[u][0] - -x[u][1][j] / K[u][1]) / h2x;
+                    ry[j] = (-x[u][2][j] / K[u][2] - -x[u][3][j] / K[u][3]) / h2y;
+                    rz[j] = (-x[u][4][j] / K[u][4] - -x[u][5][j] / K[u][5]) / h2z;
+                }
+                // the outputs are not read again here: non-temporal, so the neighbour records keep the L2
+                __builtin_nontemporal_store(rx, ox + o);
+                __builtin_nontemporal_store(ry, oy + o);
+                __builtin_nontemporal_store(rz, oz + o);
+            }
+        }
+    }
+}
+
 __device__ __forceinline__ double nan_max(double m, double x) { return (x > m || x != x) ? x : m; }
 
 // Gradient statistics as an x march. A block owns an 8 x 32 (y, z) tile and an
@@ -538,10 +660,17 @@ int sw_acoustic_strain_volumetric(int dtype, const void* p, int64_t n_vox, int64
     SW_REQUIRE(p && c && rho && (out || n_gate == 0), SW_ERR_ARG, "null pointer");
     if (n_gate == 0) return SW_OK;
     hipStream_t s = (hipStream_t)stream;
-    const int S = gate_lanes(n_gate, 64);
-    SW_DISPATCH(dtype, T,
-                hipLaunchKernelGGL(volumetric_kernel<T>, dim3(segment_blocks(n_vox, S)), dim3(256), 0, s, (const T*)p,
-                                   n_vox, nt, gate, gate_start, n_gate, S, (const T*)c, (const T*)rho, (T*)out));
+    SW_DISPATCH(dtype, T, {
+        if (!gate && (uintptr_t)out % 16 == 0 && vec16_gate<T>(p, nt, gate_start, n_gate)) {
+            const int S = gate_lanes(n_gate / Vec16<T>::n, 64);
+            hipLaunchKernelGGL(volumetric_vec_kernel<T>, dim3(segment_blocks(n_vox, S)), dim3(256), 0, s,
+                               (const T*)p, n_vox, nt, gate_start, n_gate, S, (const T*)c, (const T*)rho, (T*)out);
+        } else {
+            const int S = gate_lanes(n_gate, 64);
+            hipLaunchKernelGGL(volumetric_kernel<T>, dim3(segment_blocks(n_vox, S)), dim3(256), 0, s, (const T*)p,
+                               n_vox, nt, gate, gate_start, n_gate, S, (const T*)c, (const T*)rho, (T*)out);
+        }
+    });
     SW_LAUNCH_CHECK();
     return SW_OK;
 }
@@ -584,11 +713,20 @@ int sw_acoustic_strain_gradient(int dtype, const void* p, int64_t nx, int64_t ny
     hipStream_t s = (hipStream_t)stream;
     const Grid g{(int)nx, (int)ny, (int)nz};
     // 2.0 * h in float64, then the storage type: NumPy 2 casts the Python scalar to the array's dtype
-    const int S = gate_lanes(n_gate, 64);
-    SW_DISPATCH(dtype, T,
-                hipLaunchKernelGGL(strain_gradient_kernel<T>, dim3(segment_blocks(order_positions(g), S)), dim3(256), 0, s,
-                                   (const T*)p, g, nt, gate, gate_start, n_gate, S, (const T*)c, (const T*)rho,
-                                   (T)(2.0 * dx), (T)(2.0 * dy), (T)(2.0 * dz), (T*)gx, (T*)gy, (T*)gz));
+    SW_DISPATCH(dtype, T, {
+        const bool out16 = (uintptr_t)gx % 16 == 0 && (uintptr_t)gy % 16 == 0 && (uintptr_t)gz % 16 == 0;
+        if (!gate && out16 && vec16_gate<T>(p, nt, gate_start, n_gate)) {
+            const int S = gate_lanes(n_gate / Vec16<T>::n, 64);
+            hipLaunchKernelGGL(strain_gradient_vec_kernel<T>, dim3(segment_blocks(order_positions(g), S)), dim3(256),
+                               0, s, (const T*)p, g, nt, gate_start, n_gate, S, (const T*)c, (const T*)rho,
+                               (T)(2.0 * dx), (T)(2.0 * dy), (T)(2.0 * dz), (T*)gx, (T*)gy, (T*)gz);
+        } else {
+            const int S = gate_lanes(n_gate, 64);
+            hipLaunchKernelGGL(strain_gradient_kernel<T>, dim3(segment_blocks(order_positions(g), S)), dim3(256), 0,
+                               s, (const T*)p, g, nt, gate, gate_start, n_gate, S, (const T*)c, (const T*)rho,
+                               (T)(2.0 * dx), (T)(2.0 * dy), (T)(2.0 * dz), (T*)gx, (T*)gy, (T*)gz);
+        }
+    });
     SW_LAUNCH_CHECK();
     return SW_OK;
 }
