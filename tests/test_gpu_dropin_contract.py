@@ -383,13 +383,13 @@ def test_preconditioner_only_where_the_plain_cg_converges(sw):
     assert runs[(1e-8, 40)]["reason"] == "max_iters" and runs[(1e-8, 40)]["iters"] == 40  # plain, stopped early
 
 
-@pytest.mark.parametrize("dtype", ["float32", "float64"])
+@pytest.mark.parametrize("dtype,tile_y", [("float32", 0), ("float64", 0), ("float32", 12)])
 @pytest.mark.parametrize("stride", [4, 5])
-def test_in_loop_strain_fused_into_the_next_pass(sw, dtype, stride):
+def test_in_loop_strain_fused_into_the_next_pass(sw, dtype, tile_y, stride):
     """The in-loop invariants that the next force-free two-step pass computes from its input state equal
     those of their own kernel bit for bit. Stride 5 puts outputs before single steps, stride 4 before
     two-step passes; the force of the first steps puts some before forced passes (own kernel), and the
-    last step's invariants come after the last pass (own kernel)."""
+    last step's invariants come after the last pass (own kernel). Float32 also with the 12-row tile."""
     import torch
 
     from shearwave_amd import engine as E
@@ -399,7 +399,7 @@ def test_in_loop_strain_fused_into_the_next_pass(sw, dtype, stride):
     got = []
     for separate in (True, False):
         s = E.Solve(shape, p["h"], p["dt"], p["rho"], p["mu"], p["opts"]["eta"], dtype=dtype,
-                    vel_damp=p["opts"]["vel_damp"])
+                    vel_damp=p["opts"]["vel_damp"], kernel="march2", tile_y=tile_y)
         s.set_force(None, None, p["bz"], 1e-10, 5000)
         inv = torch.full((p["n_steps"] // stride, 4) + shape, float("nan"), dtype=E.torch_dtype(dtype),
                          device=E.device())
