@@ -121,8 +121,11 @@ int sw_project_vector_field(int dtype, int vec_dtype, void* fx, void* fy, void* 
 int sw_project_preconditioned(int vec_dtype, int64_t nx, int64_t ny, int64_t nz, double tol, int64_t max_iters);
 
 /* No reference counterpart: for callers that cannot allocate device memory
- * (code under HIP graph capture, tight serving loops, harnesses that forbid
- * hipMalloc). sw_project_vector_field with the CG workspace taken from a
+ * (tight serving loops, harnesses that forbid hipMalloc). Like
+ * sw_project_vector_field it synchronizes `stream` to poll the CG's stop flag,
+ * so it cannot itself be recorded into a graph; it allocates nothing, so a
+ * caller's captured work around it needs no allocation either.
+ * sw_project_vector_field with the CG workspace taken from a
  * static device array of the library instead of a caller's buffer, for grids
  * of at most sw_project_noalloc_max_voxels() (2^20) voxels; larger grids are
  * rejected with SW_ERR_ARG. It allocates nothing. Each GPU holds one such
