@@ -17,11 +17,12 @@ with the float64 oracle beside it). Further records: the projection by the
 preconditioned and the plain CG, the float64 form of the solve, the solve
 through the drop-in API from host arrays, the 64^3 CPU-runnable case, the
 48-solve batch, serving at six arrival rates, the gradient workload, the
-radiation force and acoustic strain, and the static solve.
+radiation force and acoustic strain, the static solve, and non-separable
+(per-step projected) forcing.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--nt 2000] [--no-cpu] [--no-fp64]
-                    [--no-batch] [--no-serve] [--no-adjoint] [--no-radforce] [--no-static]
-                    [--only radforce|static|batch|serve|adjoint] [--config1]
+                    [--no-batch] [--no-serve] [--no-adjoint] [--no-radforce] [--no-static] [--no-force4d]
+                    [--only radforce|static|batch|serve|adjoint|force4d] [--config1]
 """
 
 from __future__ import annotations
@@ -76,8 +77,9 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation run")
     ap.add_argument("--no-radforce", action="store_true", help="skip the radiation-force run")
     ap.add_argument("--no-static", action="store_true", help="skip the static elasticity solve run")
+    ap.add_argument("--no-force4d", action="store_true", help="skip the non-separable forcing run")
     ap.add_argument("--config1", action="store_true", help="only run the 64^3 CPU-vs-GPU comparison and exit")
-    ap.add_argument("--only", choices=["radforce", "static", "batch", "serve", "adjoint"],
+    ap.add_argument("--only", choices=["radforce", "static", "batch", "serve", "adjoint", "force4d"],
                     help="only run that section (radiation force + acoustic strain, the static solve, the batching "
                          "comparison or the differentiation run), print its record and exit")
     return ap.parse_args()
@@ -791,6 +793,105 @@ def dropin_host_to_host(w, dtype="float32"):
                     "no in-loop strain"}
 
 
+def bench_force4d(cpu_steps=1, window=16, proj_steps=(0, 33, 66)):
+    """Row f1: non-separable forcing (solver.py:118-122, 199), a 128^3 x nT=100 float32 bz projected every step
+    at the reference's defaults (tol 1e-5, 200 iterations), through the drop-in API from host arrays.
+
+    Three calls, each timed whole: the default (the force uploaded once and
+    sliced on the device, the forces of each window of `window` steps
+    projected in one batched CG before the window's steps run), the same
+    with each step's force projected on its own (``sw_force_window`` 1), and
+    each step's slice cut and uploaded from the host and projected on its own
+    (``sw_resident_force`` False as well). The projections alone, with HIP
+    events: a window's batched CG, and single steps' CG; GB/s at 34 B per
+    voxel-iteration, the float32-vector CG's traffic. CPU baseline: the
+    float32 oracle (the reference's arithmetic) over `cpu_steps` steps of the
+    same slices, one core.
+    """
+    import torch
+
+    import shearwave_amd as sw
+    from oracle import solver as O  # the checker
+    from shearwave_amd import engine as E
+    from shearwave_amd import workloads as W
+
+    w = W.force4d_128()
+    n_steps, h, bz = w["n_steps"], w["h"], w["bz"]
+    vox = int(np.prod(w["shape"]))
+    opts = {"eta": w["eta"], "poisson_tol": w["tol"], "poisson_max_iters": w["max_iters"], "sw_force_window": window}
+    mats = (w["rho"], w["mu"], h, h, h, w["dt"])
+    sw.shear_fdtd(None, None, np.ascontiguousarray(bz[..., :2]), *mats, 2, opts)  # warm-up
+    runs = {}
+    for name, extra in (("batched_windows", {}), ("per_step", {"sw_force_window": 1}),
+                        ("per_step_host_sliced", {"sw_force_window": 1, "sw_resident_force": False})):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sw.shear_fdtd(None, None, bz, *mats, n_steps, {**opts, **extra})
+        t = time.perf_counter() - t0
+        runs[name] = {"seconds": round(t, 4), "ms_per_step": round(t / n_steps * 1e3, 3)}
+        progress(f"4-D forcing: {name} call done ({t:.2f} s)")
+
+    # the projections alone: one window's batched CG, and single steps' CG, on a solve of the workload
+    s = E.Solve(w["shape"], (h, h, h), w["dt"], w["rho"], w["mu"], w["eta"])
+    bz_d = torch.from_numpy(bz).to(E.device())
+
+    def events():
+        return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    forces = [(None, None, bz_d[..., t]) for t in range(window)]
+    s.project_forces(forces, w["tol"], w["max_iters"])  # warm
+    a, b = events()
+    a.record()
+    _, infos = s.project_forces(forces, w["tol"], w["max_iters"])
+    b.record()
+    b.synchronize()
+    ms_w = a.elapsed_time(b)
+    it_w = sum(max(1, i["iters"]) for i in infos)
+    batched = {"steps": window, "ms": round(ms_w, 3), "ms_per_step": round(ms_w / window, 3),
+               "iters": sorted({i["iters"] for i in infos}),
+               "us_per_step_iter": round(ms_w * 1e3 / it_w, 2),
+               "GBs": round(34 * vox * it_w / (ms_w / 1e3) / 1e9, 1)}
+    singles = []
+    for k in proj_steps:
+        a, b = events()
+        a.record()
+        info, _ = s.set_force(None, None, bz_d[..., k], w["tol"], w["max_iters"])
+        b.record()
+        b.synchronize()
+        ms = a.elapsed_time(b)
+        it = max(1, info["iters"])
+        singles.append({"step": k, "iters": info["iters"], "ms": round(ms, 3), "us_per_iter": round(ms * 1e3 / it, 2),
+                        "GBs": round(34 * vox * it / (ms / 1e3) / 1e9, 1)})
+    del s, bz_d
+    torch.cuda.empty_cache()
+    cpu = None
+    if cpu_steps > 0:  # the float32 oracle on the first cpu_steps slices
+        info_cpu = []
+        t0 = time.perf_counter()
+        O.shear_fdtd(None, None, np.ascontiguousarray(bz[..., :cpu_steps]), *mats, cpu_steps, opts=opts,
+                     dtype=np.float32, proj_info=info_cpu)
+        t_cpu = (time.perf_counter() - t0) / cpu_steps
+        cpu = {"ms_per_step": round(t_cpu * 1e3, 1), "cores": 1, "kind": "port",
+               "cg_iters": [int(i.get("iters", -1)) for i in info_cpu],
+               "sample": f"float32 oracle, the first {cpu_steps} steps (each projected), one core",
+               "gpu_speedup": round(t_cpu / (runs["batched_windows"]["seconds"] / n_steps), 1)}
+    return {
+        "workload": "128^3, h=0.5 mm, rho 1000, mu 3 kPa with 3 inclusions (seed 6), eta 1, nT=100, bz a "
+                    "(128,128,128,100) float32 Gaussian push (sigma 2 mm) moving 16 voxels along z, ramped "
+                    "300 -> 0, projected every step (tol 1e-5, 200 iterations); host arrays in and out",
+        "force_bytes": int(bz.nbytes),
+        "calls": runs,
+        "speedup_over_per_step_host_sliced": round(runs["per_step_host_sliced"]["seconds"] /
+                                                   runs["batched_windows"]["seconds"], 3),
+        "projection": {"batched_window": batched, "single_steps": singles,
+                       "roofline": {"bound": "hbm", "kernel": "batched float32-vector CG iteration (dir + step)",
+                                    "alg_bytes_per_voxel_iter": 34, "achieved": batched["GBs"],
+                                    "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                    "frac": round(batched["GBs"] / HBM_PEAK_GBS, 4)}},
+        "cpu_baseline": cpu,
+    }
+
+
 def config1_cpu_vs_gpu():
     """Config 1 (the reference's CPU case, 64^3, nT=500) on CPU and GPU: the CPU-baseline leg for it.
 
@@ -852,6 +953,7 @@ def main():
         return
     if args.only:
         sections = {"radforce": bench_radforce, "static": bench_static, "adjoint": bench_adjoint,
+                    "force4d": bench_force4d,
                     "batch": lambda: bench_batch(args.batch_solves),
                     "serve": lambda: bench_serve(tuple(float(r) for r in args.serve_rates.split(",")),
                                                  args.serve_requests)}
@@ -1014,6 +1116,9 @@ def main():
     if not args.no_static:
         rec["static_solve"] = bench_static()
         progress("static solve done")
+    if not args.no_force4d:
+        rec["force4d"] = bench_force4d(cpu_steps=0 if args.no_cpu else 1)
+        progress("non-separable forcing done")
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(w, args.cpu_steps, warm=2, n_steps64=3, proj_iters=3,
                                            full_iters=int(rec["phases"]["plain_cg_iters"]))

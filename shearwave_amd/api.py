@@ -29,6 +29,14 @@ Build-only options (namespaced ``sw_``):
     them (solver.py:1020-1041);
   * ``sw_strain_stride``: with ``shear_fdtd(..., return_strain=True)``, the
     four strain invariants every that many steps, computed in-loop;
+  * ``sw_resident_force``: with 4-D (per-step) body forces, upload each
+    component once and take every step's slice on the device (default True;
+    used when they take at most half of the free device memory), instead of
+    slicing and uploading each step's force from the host;
+  * ``sw_force_window``: with per-step forces and float32 storage, the
+    forces of up to this many steps (default 16) are projected together in
+    one batched CG before those steps run, where the plain CG serves them
+    (1 = each step's on its own, as the float64 form always does);
   * ``sw_check_finite``: check u for inf/NaN every that many steps (default
     ``max(1, n_steps // 20)``, 0 = never) and warn (``RuntimeWarning``) with
     the first step found; the returned arrays are unchanged, as the
@@ -84,6 +92,48 @@ def _check_body(comp, nx, ny, nz, n_steps):
     if len(shape) == 4 and shape == (nx, ny, nz, n_steps):
         return comp
     raise ValueError("Body-force component has incompatible shape.")
+
+
+def _resident_body(comps, dtype):
+    """The body-force components of a per-step solve on the device, each uploaded once in its
+    source type (float32 or float64; others as `dtype`) and sliced per step there: the host's
+    slice of step `it`, ``b[..., it]``, reads one value per nT (a cache line per value). Returns
+    None when they would take more than half of the free device memory."""
+    arrs = []
+    need = 0
+    for c in comps:
+        if c is None or _is_dev(c):
+            arrs.append(c)
+            continue
+        a = np.asarray(c)
+        if a.dtype not in (np.float32, np.float64):
+            a = a.astype(np.float32 if dtype == "float32" else np.float64)
+        arrs.append(a)
+        need += a.nbytes
+    free, _total = torch.cuda.mem_get_info(E.device())
+    if 2 * need > free:
+        return None
+    # a pageable copy (about 30 GB/s): a pinned staging block this large would first cost its allocation
+    # (about 3.4 GB/s on first use)
+    dev = E.device()
+    return [a if a is None or _is_dev(a) else torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in arrs]
+
+
+def _force_window(solve, opts, n_steps, tol, max_iters):
+    """Steps whose per-step forces are projected together in one batched CG (1: each on its own).
+
+    ``opts['sw_force_window']`` (default 16) caps it; the window is used where
+    the batched CG serves the solve (float32 storage and CG vectors, the
+    plain CG: ``Solve.batched_projection_fits``) and its buffers (about 44 B
+    per voxel and step: the force, its projection and the CG vectors) take
+    at most a quarter of the free device memory. Each projection is
+    poisson_cg's as in the single-solve path, in another summation order.
+    """
+    w = min(int(opts.get("sw_force_window", 16) or 1), n_steps)
+    if w <= 1 or not solve.batched_projection_fits(tol, max_iters):
+        return 1
+    free, _total = torch.cuda.mem_get_info(E.device())
+    return max(1, min(w, free // (4 * 44 * solve.n)))
 
 
 def _trace_flat(trace_indices, shape):
@@ -224,6 +274,8 @@ def shear_fdtd(  # noqa: PLR0912, PLR0915
 
     if per_step:
         env = np.ones(n_steps)
+        if opts.get("sw_resident_force", True):
+            comps = _resident_body(comps, dtype) or comps
     elif env_t is not None:
         env = env_t.astype(np.float64)
     else:
@@ -234,15 +286,29 @@ def shear_fdtd(  # noqa: PLR0912, PLR0915
             print("Pre-computing Helmholtz projection (separable force)...")
         solve.set_force(*comps, poisson_tol, poisson_max_iters)
 
+    window = 0
+    if per_step:
+        window = _force_window(solve, opts, n_steps, poisson_tol, poisson_max_iters)
+
+    def step_force(t):
+        return [None if c is None else (c[..., t] if len(c.shape) == 4 else c) for c in comps]
+
     if callback is None and not reproject_every and not per_step and not verbose:
         solve.run(n_steps, env, **outs)
     else:
         it = 0
+        w_outs = None
         while it < n_steps:
-            if per_step:
+            if per_step and window > 1:
                 k = 1
-                sl = [None if c is None else (c[..., it] if len(c.shape) == 4 else c) for c in comps]
-                solve.set_force(*sl, poisson_tol, poisson_max_iters)
+                if it % window == 0:  # the next window's forces, projected in one batched CG
+                    w_outs = None
+                    w_outs, _ = solve.project_forces([step_force(t) for t in range(it, min(it + window, n_steps))],
+                                                     poisson_tol, poisson_max_iters)
+                solve.use_force(w_outs[it % window])
+            elif per_step:
+                k = 1
+                solve.set_force(*step_force(it), poisson_tol, poisson_max_iters)
             elif callback is not None or verbose:
                 k = 1
             else:  # reprojection only: run up to the next reprojection

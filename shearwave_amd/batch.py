@@ -317,6 +317,80 @@ def xchunk_for(s, tuner=None):
     return tuner.xchunk(s.shape) if tuner else default_xchunk(s.shape)
 
 
+@dataclass
+class ProjectionJob:
+    """One force field of a batched projection: f (3, nx, ny, nz) float32 on the device, projected in place;
+    out (3, nx, ny, nz) float32 receives scale * f_S * inv_rho."""
+
+    shape: tuple
+    h: tuple
+    f: object
+    out: object
+    inv_rho: object
+    scale: float
+    tol: float
+    max_iters: int
+
+
+def batched_projection(jobs, xchunk=None):
+    """One batched plain CG over several force fields (sw_batch_project): r and p in float32, phi, the dot
+    products and the residual norms in float64, each field's CG stopping on its own (poisson_cg's rules,
+    solver.py:1010-1050). `xchunk` (a multiple of CG_SUB) overrides :func:`cg_xchunk`; it changes the speed
+    only. Returns one CG info dict per job; the stream is synchronized."""
+    dev = E.device()
+    cgs = (_abi.BatchCg * len(jobs))()
+    items = []
+    keep = []
+    if xchunk is None:
+        xchunk = cg_xchunk([j.shape for j in jobs], torch.cuda.get_device_properties(dev).multi_processor_count)
+    elif xchunk < 1 or xchunk % CG_SUB:
+        raise ValueError(f"xchunk must be a positive multiple of {CG_SUB}, got {xchunk}")
+    cost = []
+    for k, job in enumerate(jobs):
+        nx, ny, nz = job.shape
+        xc = min(nx, xchunk)
+        n_items0 = len(items)
+        for tx in range(math.ceil(nx / xc)):
+            for ty in range(math.ceil(ny / CG_BY)):
+                for tz in range(math.ceil(nz / 64)):
+                    items.append((k, tz, ty, tx))
+                    cost.append(item_cost(min(xc, nx - tx * xc), min(64, nz - 64 * tz)))
+        phi = torch.empty(job.shape, dtype=torch.float64, device=dev)
+        vec = torch.empty((3,) + tuple(job.shape), dtype=torch.float32, device=dev)  # r, p0, p1
+        n_slots = math.ceil(nx / CG_SUB) * math.ceil(ny / CG_BY) * math.ceil(nz / 64)
+        part = torch.empty(2 * n_slots, dtype=torch.float64, device=dev)
+        keep += [phi, vec, part]
+        c = cgs[k]
+        c.nx, c.ny, c.nz = job.shape
+        c.xchunk = xc
+        c.dx, c.dy, c.dz = job.h
+        c.tol = float(job.tol)
+        c.max_iters = int(job.max_iters)
+        for j in range(3):
+            c.f[j] = job.f[j].data_ptr()
+            c.out[j] = job.out[j].data_ptr()
+        c.inv_rho = job.inv_rho.data_ptr()
+        c.scale = float(job.scale)
+        c.phi = phi.data_ptr()
+        c.r, c.p0, c.p1 = (vec[j].data_ptr() for j in range(3))
+        c.part = part.data_ptr()
+        c.item0 = n_items0
+        c.n_items = len(items) - n_items0
+    # longest first: the dir and step kernels' blocks claim items in list order
+    order = np.argsort(-np.asarray(cost, dtype=np.int64), kind="stable")
+    items_t = torch.from_numpy(np.asarray(items, dtype=np.int32).reshape(-1, 4)[order].reshape(-1)).to(dev)
+    desc = torch.frombuffer(bytearray(bytes(cgs)), dtype=torch.uint8).to(dev)
+    # a CG state per job, then the kernels' work-claim counters
+    states = torch.empty((len(jobs) + 1) * int(lib().sw_cg_state_bytes()), dtype=torch.uint8, device=dev)
+    info = (_abi.CgInfo * len(jobs))()
+    check(lib().sw_batch_project(desc.data_ptr(), len(jobs), items_t.data_ptr(), len(items), states.data_ptr(),
+                                 info, E.stream_ptr()))
+    torch.cuda.current_stream().synchronize()
+    del keep
+    return [{"iters": int(i.iters), "relres": float(i.relres), "max_div": float(i.max_div),
+             "reason": _abi.CG_REASONS.get(int(i.reason), str(i.reason))} for i in info]
+
+
 class Batch:
     """A set of solves advanced together on one GPU (float32 storage, increment form)."""
 
@@ -386,59 +460,10 @@ class Batch:
             return self._project_each(solves)
         if method != "batched":
             raise ValueError(f"unknown projection method {method!r}")
-        dev = E.device()
-        cgs = (_abi.BatchCg * len(solves))()
-        items = []
-        keep = []
-        if xchunk is None:
-            xchunk = cg_xchunk([s.shape for s in solves], torch.cuda.get_device_properties(dev).multi_processor_count)
-        elif xchunk < 1 or xchunk % CG_SUB:
-            raise ValueError(f"xchunk must be a positive multiple of {CG_SUB}, got {xchunk}")
-        cost = []
-        for k, s in enumerate(solves):
-            sp = s.spec
-            nx, ny, nz = s.shape
-            xc = min(nx, xchunk)
-            n_items0 = len(items)
-            for tx in range(math.ceil(nx / xc)):
-                for ty in range(math.ceil(ny / CG_BY)):
-                    for tz in range(math.ceil(nz / 64)):
-                        items.append((k, tz, ty, tx))
-                        cost.append(item_cost(min(xc, nx - tx * xc), min(64, nz - 64 * tz)))
-            phi = torch.empty(s.shape, dtype=torch.float64, device=dev)
-            vec = torch.empty((3,) + s.shape, dtype=torch.float32, device=dev)  # r, p0, p1
-            n_slots = math.ceil(nx / CG_SUB) * math.ceil(ny / CG_BY) * math.ceil(nz / 64)
-            part = torch.empty(2 * n_slots, dtype=torch.float64, device=dev)
-            keep += [phi, vec, part]
-            c = cgs[k]
-            c.nx, c.ny, c.nz = s.shape
-            c.xchunk = xc
-            c.dx, c.dy, c.dz = s.h
-            c.tol = float(sp.tol)
-            c.max_iters = int(sp.max_iters)
-            for j in range(3):
-                c.f[j] = s.f[j].data_ptr()
-                c.out[j] = s.force[j].data_ptr()
-            c.inv_rho = s.inv_rho.data_ptr()
-            c.scale = float(sp.dt) ** 2
-            c.phi = phi.data_ptr()
-            c.r, c.p0, c.p1 = (vec[j].data_ptr() for j in range(3))
-            c.part = part.data_ptr()
-            c.item0 = n_items0
-            c.n_items = len(items) - n_items0
-        # longest first: the dir and step kernels' blocks claim items in list order
-        order = np.argsort(-np.asarray(cost, dtype=np.int64), kind="stable")
-        items_t = torch.from_numpy(np.asarray(items, dtype=np.int32).reshape(-1, 4)[order].reshape(-1)).to(dev)
-        desc = torch.frombuffer(bytearray(bytes(cgs)), dtype=torch.uint8).to(dev)
-        # a CG state per solve, then the kernels' work-claim counters
-        states = torch.empty((len(solves) + 1) * int(lib().sw_cg_state_bytes()), dtype=torch.uint8, device=dev)
-        info = (_abi.CgInfo * len(solves))()
-        check(lib().sw_batch_project(desc.data_ptr(), len(solves), items_t.data_ptr(), len(items), states.data_ptr(),
-                                     info, E.stream_ptr()))
-        torch.cuda.current_stream().synchronize()
-        del keep
-        self.cg_info = [{"iters": int(i.iters), "relres": float(i.relres), "max_div": float(i.max_div),
-                         "reason": _abi.CG_REASONS.get(int(i.reason), str(i.reason))} for i in info]
+        jobs = [ProjectionJob(shape=s.shape, h=s.h, f=s.f, out=s.force, inv_rho=s.inv_rho,
+                              scale=float(s.spec.dt) ** 2, tol=float(s.spec.tol), max_iters=int(s.spec.max_iters))
+                for s in solves]
+        self.cg_info = batched_projection(jobs, xchunk)
         return self.cg_info
 
     def _project_each(self, solves):

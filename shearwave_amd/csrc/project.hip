@@ -264,6 +264,13 @@ __global__ void __launch_bounds__(64 * kCgBY) cg_flush_march_kernel(double* __re
 #define SW_CG_BLOCKS_PER_CU 1
 #endif
 
+// Shortest x chunk of the march, in planes: a chunk re-reads the planes of
+// its prologue, so short chunks cost traffic on large grids.
+#ifndef SW_CG_MIN_XC
+#define SW_CG_MIN_XC 12
+#endif
+constexpr int kCgMinXc = SW_CG_MIN_XC;
+
 struct CgShape {
     int xchunk, nty, ntz, ntx;
     unsigned blocks() const { return (unsigned)nty * ntz * ntx; }
@@ -274,20 +281,20 @@ CgShape cg_shape(const Grid& g) {
     c.nty = (g.ny + kCgBY - 1) / kCgBY;
     c.ntz = (g.nz + 63) / 64;
     const int64_t cols = (int64_t)c.nty * c.ntz;
-    // about SW_CG_BLOCKS_PER_CU 512-thread blocks per CU in one wave, chunks of at least 12 planes
+    // about SW_CG_BLOCKS_PER_CU blocks per CU in one wave, chunks of at least kCgMinXc planes
     const int64_t target = (int64_t)device_cu_count() * SW_CG_BLOCKS_PER_CU;
     int64_t ntx = (target + cols - 1) / cols;
     if (ntx < 1) ntx = 1;
     int64_t xc = (g.nx + ntx - 1) / ntx;
-    if (xc < 12) xc = 12;
+    if (xc < kCgMinXc) xc = kCgMinXc;
     c.xchunk = (int)xc;
     c.ntx = (g.nx + c.xchunk - 1) / c.xchunk;
     return c;
 }
 
-// Upper bound on the march grid's block count (chunks never shorter than 12 planes).
+// Upper bound on the march grid's block count (chunks never shorter than kCgMinXc planes).
 int64_t cg_max_blocks(int64_t nx, int64_t ny, int64_t nz) {
-    return ((ny + kCgBY - 1) / kCgBY) * ((nz + 63) / 64) * ((nx + 11) / 12);
+    return ((ny + kCgBY - 1) / kCgBY) * ((nz + 63) / 64) * ((nx + kCgMinXc - 1) / kCgMinXc);
 }
 
 // f_S = mask (mask f - grad_c phi); optionally out = scale * f_S * inv_rho.

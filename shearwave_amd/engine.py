@@ -310,13 +310,48 @@ class Solve:
                 f[c].zero_()
             else:
                 f[c].copy_(to_device(comp, self.dtype, self.shape))
-        if self.force is None:
-            self.force = self._force_slot
-            for c in range(3):
-                self.state.force[c] = self.force[c].data_ptr()
+        self.use_force(self._force_slot)
         info = project_inplace(f, self.spacing, tol, max_iters, scale_out=self.force, inv_rho=self.inv_rho,
                                scale=self.dt * self.dt, vectors=self.cg_vectors)
         return info, f
+
+    def batched_projection_fits(self, tol, max_iters):
+        """Whether :meth:`project_forces` serves this solve's projections: float32 storage and CG vectors
+        (the batched CG's), where the single-solve projection would run the plain CG too (not the
+        spectral-preconditioned one, which converges in a few iterations)."""
+        return (self.dtype == "float32" and self.cg_vectors == "float32"
+                and not lib().sw_project_preconditioned(_abi.SW_F32, *self.shape, float(tol), int(max_iters)))
+
+    def project_forces(self, forces, tol, max_iters):
+        """Project several force patterns in one batched CG (``batch.batched_projection``), each as
+        :meth:`set_force` would (solver.py:189-192: mask, divergence, poisson_cg, gradient subtraction; the
+        batched CG's float32 vectors and float64 reductions). ``forces``: a list of (fx, fy, fz) as set_force
+        takes them. Returns (outs, infos): outs[k] (3, nx, ny, nz) is pattern k's dT^2 f_S / rho, for
+        :meth:`use_force`."""
+        from .batch import ProjectionJob, batched_projection
+
+        if self.dtype != "float32":
+            raise ValueError("the batched projection runs on float32 storage")
+        k = len(forces)
+        f = torch.empty((k, 3) + self.shape, dtype=torch.float32, device=device())
+        outs = torch.empty((k, 3) + self.shape, dtype=torch.float32, device=device())
+        for j, comps in enumerate(forces):
+            for c, comp in enumerate(comps):
+                if comp is None:
+                    f[j, c].zero_()
+                else:
+                    f[j, c].copy_(to_device(comp, self.dtype, self.shape))
+        jobs = [ProjectionJob(shape=self.shape, h=self.spacing, f=f[j], out=outs[j], inv_rho=self.inv_rho,
+                              scale=self.dt * self.dt, tol=float(tol), max_iters=int(max_iters)) for j in range(k)]
+        infos = batched_projection(jobs)
+        return outs, infos
+
+    def use_force(self, f):
+        """The loop's force: f (3, nx, ny, nz) of this solve's storage type, dT^2 f_S / rho (kept referenced)."""
+        if self.force is not f:
+            self.force = f
+            for c in range(3):
+                self.state.force[c] = f[c].data_ptr()
 
     def set_initial(self, u0=None, v0=None):
         """Initial state from AoS (nx, ny, nz, 3) arrays (solver.py:157-171)."""

@@ -81,6 +81,29 @@ def dropin_256(n=256, seed=2, n_steps=2000):
                 bz=bz, env=env, tol=1e-8, max_iters=20000, strain_stride=20)
 
 
+def force4d_128(n=128, n_steps=100, seed=6):
+    """Non-separable forcing (solver.py:118-122, 199): a (n, n, n, nT) float32 bz, a Gaussian push
+    (sigma 2 mm) whose centre moves 16 voxels along z over the run while its amplitude ramps from 300 to 0,
+    projected every step at the reference's defaults (tol 1e-5, 200 iterations); h = 0.5 mm, rho 1000,
+    mu 3 kPa with 3 seeded inclusions, eta 1."""
+    rng = np.random.default_rng(seed)
+    shape = (n, n, n)
+    h = 0.5e-3
+    rho, eta = 1000.0, 1.0
+    mu = inclusion_map(shape, h, rng, 3, 3000.0)
+    dt, dt_cfl = stable_dt(h, rho, float(mu.max()), eta)
+    sigma = 2e-3
+    ax = (np.arange(n) - (n - 1) / 2) * h
+    gxy = np.exp(-(ax[:, None] ** 2 + ax[None, :] ** 2) / (2 * sigma**2)).astype(np.float32)
+    c = (np.linspace(-8, 8, n_steps) * h)[None, :]
+    amp = np.linspace(300.0, 0.0, n_steps)[None, :]
+    gz = (amp * np.exp(-((ax[:, None] - c) ** 2) / (2 * sigma**2))).astype(np.float32)  # (nz, nT)
+    bz = np.empty(shape + (n_steps,), np.float32)
+    np.multiply(gxy[:, :, None, None], gz[None, None, :, :], out=bz)
+    return dict(shape=shape, h=h, rho=rho, mu=mu, eta=eta, dt=dt, dt_cfl=dt_cfl, n_steps=n_steps, bz=bz,
+                tol=1e-5, max_iters=200)
+
+
 # algorithmic HBM bytes per voxel of each kernel (read + write, once each)
 def alg_step_bytes(itemsize, viscous=True, forced=False):
     """Algorithmic bytes of one time step per voxel, in the reference's (u, u_prev) form.

@@ -190,6 +190,52 @@ def test_4d_forcing_float32(sw):
     u64, v64 = O.shear_fdtd(bx, None, bz, rho, mu, *h, 2e-6, n_steps, opts=opts, dtype=np.float64)
     eu, ev = rel(u32, u64), rel(v32, v64)
     assert eu < 1e-5 and ev < 1e-5, (eu, ev)
+    # the forces resident on the device (default) and sliced on the host each step give the same bits
+    u_h, v_h = sw.shear_fdtd(bx, None, bz, rho, mu, *h, 2e-6, n_steps, {**opts, "sw_resident_force": False})
+    assert np.array_equal(u_h, u32) and np.array_equal(v_h, v32)
+
+
+@pytest.mark.parametrize("max_iters", [20, 60])
+def test_unconverged_projection_float32_vectors(sw, max_iters):
+    """The reference's default projection stops at max_iters before tol (poisson_tol 1e-5, 200 iterations,
+    solver.py:95-96, on large grids): the float32-vector CG's iterate at that count within 1e-5 of the float64
+    oracle's, on the projected force."""
+    rng = np.random.default_rng(12)
+    shape = (40, 36, 44)
+    f = [(rng.standard_normal(shape) * taper(shape)).astype(np.float32) for _ in range(2)]
+    f.append((300.0 * gauss(shape, 4.0) + rng.standard_normal(shape) * taper(shape)).astype(np.float32))
+    h = (0.3e-3,) * 3
+    info = {}
+    got = sw.project_vector_field(*f, *h, 1e-5, max_iters, info=info)
+    assert info["reason"] == "max_iters" and info["iters"] == max_iters, info
+    want = O.project_vector_field(*(a.astype(np.float64) for a in f), *h, 1e-5, max_iters, dtype=np.float64)
+    errs = [rel(a, b) for a, b in zip(got, want)]
+    assert max(errs) < 1e-5, errs
+
+
+def test_4d_forcing_unconverged_projections_float32(sw):
+    """Per-step projections that stop at max_iters (20) before tol: float32 storage within 1e-5 of the float64
+    oracle on u and v, with the steps' forces projected in batched windows (the default: 16 and 8 steps here),
+    each on its own (window 1), and windows of 5 over forces sliced on the host (the same bits as resident)."""
+    shape, n_steps = (26, 24, 68), 24
+    rng = np.random.default_rng(33)
+    h = (0.3e-3, 0.3e-3, 0.3e-3)
+    rho = 1000.0 * (1 + rng.uniform(-0.01, 0.01, shape))
+    mu = 3000.0 + 3000.0 * rng.uniform(0, 1, shape)
+    g = gauss(shape, 3.0)[..., None]
+    bz = 300.0 * g * np.linspace(1.0, 0.2, n_steps) + rng.standard_normal(shape + (1,)) * taper(shape)[..., None]
+    opts = {"eta": 1.0, "poisson_tol": 1e-5, "poisson_max_iters": 20}
+    info = []
+    u64, v64 = O.shear_fdtd(None, None, bz, rho, mu, *h, 2e-6, n_steps, opts=opts, dtype=np.float64, proj_info=info)
+    assert all(i["iters"] == 20 for i in info), [i["iters"] for i in info]
+    runs = {}
+    for name, extra in (("default", {}), ("each", {"sw_force_window": 1}), ("w5", {"sw_force_window": 5}),
+                        ("w5_host", {"sw_force_window": 5, "sw_resident_force": False})):
+        u32, v32 = sw.shear_fdtd(None, None, bz, rho, mu, *h, 2e-6, n_steps, {**opts, **extra})
+        eu, ev = rel(u32, u64), rel(v32, v64)
+        assert eu < 1e-5 and ev < 1e-5, (name, eu, ev)
+        runs[name] = (u32, v32)
+    assert np.array_equal(runs["w5"][0], runs["w5_host"][0]) and np.array_equal(runs["w5"][1], runs["w5_host"][1])
 
 
 # --------------------------------------------------------- full-size workloads
