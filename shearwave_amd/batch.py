@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import ctypes as C
 import functools
+import heapq
 import itertools
 import math
 import time
@@ -37,25 +38,44 @@ MAX_PLANE = 1 << 28  # the kernels address a y-z plane with 32-bit offsets
 CG_BY = 16  # tile rows of the batched CG (fixed by the kernels)
 CG_SUB = 48  # x planes per partial sum of the batched CG (fixed by the kernels)
 CG_XCHUNKS = (192, 96, 48)  # x planes per batched CG work item, longest first
-CG_ITEMS_PER_CU = 4  # a longer item is taken only while the call keeps this many items per CU
+CG_PROLOGUE = 6  # x planes a CG work item loads before its first output plane
 
 
 def cg_xchunk(shapes, n_cu):
     """x planes per CG work item for one batched projection of grids `shapes`.
 
-    Longer items pay fewer march prologues, but a call with few solves needs
-    short items to occupy every CU: the longest of CG_XCHUNKS that still gives
-    CG_ITEMS_PER_CU items per CU, else the shortest. The kernels sum every
-    dot product per CG_SUB-plane sub-chunk, so the choice does not change any
+    Longer items pay fewer march prologues (about CG_PROLOGUE planes each),
+    shorter ones balance the blocks: each candidate of CG_XCHUNKS is priced
+    by the time its items take when the blocks (one per CU) claim them
+    longest first, each item costing its planes plus a prologue, and the
+    cheapest is taken (the longer on a tie). The kernels sum every dot
+    product per CG_SUB-plane sub-chunk, so the choice does not change any
     result.
     """
-    def n_items(xc):
-        return sum(math.ceil(nx / xc) * math.ceil(ny / CG_BY) * math.ceil(nz / 64) for nx, ny, nz in shapes)
-
+    best = None
     for xc in CG_XCHUNKS:
-        if n_items(xc) >= CG_ITEMS_PER_CU * n_cu:
-            return xc
-    return CG_XCHUNKS[-1]
+        costs = []
+        for nx, ny, nz in shapes:
+            xe = min(nx, xc)
+            tiles = math.ceil(ny / CG_BY) * math.ceil(nz / 64)
+            for c in range(math.ceil(nx / xe)):
+                costs += [min(xe, nx - c * xe) + CG_PROLOGUE] * tiles
+        span = _claimed_makespan(costs, n_cu)
+        if best is None or span < best[0]:
+            best = (span, xc)
+    return best[1]
+
+
+def _claimed_makespan(costs, n_blocks):
+    """Finish time of items claimed longest first by n_blocks blocks, each taking the next item when free."""
+    costs = sorted(costs, reverse=True)
+    if len(costs) <= n_blocks:
+        return costs[0] if costs else 0
+    free = [0] * n_blocks
+    heapq.heapify(free)
+    for c in costs:
+        heapq.heappush(free, heapq.heappop(free) + c)
+    return max(free)
 
 
 @dataclass

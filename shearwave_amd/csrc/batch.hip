@@ -566,10 +566,13 @@ int sw_batch_project(const sw_batch_cg* solves, int32_t n_solves, const int32_t*
     int rc = poll();
     int64_t chunk = 8;
     while (rc == SW_OK) {
-        bool running = false;
-        for (int k = 0; k < n_solves; ++k) running |= (h[k].stop == 0);
-        if (!running) break;
-        for (int64_t j = 0; j < chunk; ++j) {
+        // launches up to the next poll: no more than any running solve can still take
+        int64_t left = 0;
+        for (int k = 0; k < n_solves; ++k)
+            if (h[k].stop == 0 && h[k].max_iters - h[k].iters > left) left = h[k].max_iters - h[k].iters;
+        if (left <= 0) break;
+        const int64_t n_launch = chunk < left ? chunk : left;
+        for (int64_t j = 0; j < n_launch; ++j) {
             hipLaunchKernelGGL(batch_cg_dir_kernel, dim3(nb), blk, 0, s, solves, st, n_solves, it, n_items);
             hipLaunchKernelGGL(batch_cg_step_kernel, dim3(nb), blk, 0, s, solves, st, n_solves, it, n_items);
         }

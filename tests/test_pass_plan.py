@@ -29,14 +29,17 @@ def test_plan_pairs_steps():
 
 
 def test_cg_xchunk_rule():
-    """The batched CG item length: the longest of 192, 96, 48 planes that keeps four items
-    per CU in the call, else 48 (host logic only)."""
-    from shearwave_amd.batch import CG_SUB, CG_XCHUNKS, cg_xchunk
+    """The batched CG item length: of 192, 96 and 48 planes, the one whose items (planes plus a 6-plane
+    prologue each) finish first when one block per CU claims them longest first (host logic only)."""
+    from shearwave_amd.batch import CG_SUB, CG_XCHUNKS, _claimed_makespan, cg_xchunk
 
     assert all(xc % CG_SUB == 0 for xc in CG_XCHUNKS)
+    assert _claimed_makespan([5, 3, 3, 2, 2], 2) == 8  # (5, 3) and (3, 2, 2): longest first to the free block
+    assert _claimed_makespan([4, 4], 8) == 4
     one = [(128, 128, 160)]  # 8 y tiles x 3 z tiles: 24 items at 192 planes, 72 at 48
-    assert cg_xchunk(one, 256) == 48
-    assert cg_xchunk(one, 6) == 192  # 24 items >= 4 x 6 CUs
-    assert cg_xchunk(one, 9) == 96  # 24 < 36 <= 48 items at 96 planes
-    assert cg_xchunk(one * 48, 256) == 192  # 1152 items >= 1024
-    assert cg_xchunk([(8, 8, 8)], 256) == 48
+    assert cg_xchunk(one, 256) == 48  # every item at once: the shortest items finish first
+    assert cg_xchunk(one, 6) == 192  # 4 whole-axis items per block: the fewest prologues
+    assert cg_xchunk(one, 9) == 96  # 24 whole-axis items leave 3 of 9 blocks idle in the last round
+    assert cg_xchunk(one * 48, 256) == 96  # 1152 whole-axis items: 4.5 rounds of 256
+    assert cg_xchunk([(128, 128, 128)] * 16, 256) == 192  # 256 whole-axis items, one per CU
+    assert cg_xchunk([(8, 8, 8)], 256) == 192  # one item whatever the length: the longest on the tie
