@@ -478,10 +478,12 @@ __device__ __forceinline__ double nan_max(double m, double x) { return (x > m ||
 // in windows of kGW, in order; for each window the block marches x over its chunk,
 // staging every plane's tile plus a one-voxel halo in LDS (a ring of three
 // planes), with the strain -p / K of each staged voxel and sample computed
-// once, in float64 as the reference. A thread then reads its voxel's six
-// neighbour strains from LDS. The maxima and the in-order sum of |grad|^2 of
-// the thread's kGXC voxels stay in registers across the windows, so the sum keeps
-// the reference's sample order. Each record is read about 1.6 times (halo and
+// once, in float64 as the reference. A staging thread takes whole voxels: K
+// once per voxel, and the window's samples of a contiguous, aligned gate in
+// 16-byte loads. A thread then reads its voxel's six neighbour strains from
+// LDS. The maxima and the in-order sum of |grad|^2 of the thread's kGXC
+// voxels stay in registers across the windows, so the sum keeps the
+// reference's sample order. Each record is read about 1.6 times (halo and
 // the two planes before each chunk) instead of once per neighbour.
 #ifndef SW_GP_WINDOW
 #define SW_GP_WINDOW 8
@@ -492,17 +494,22 @@ __device__ __forceinline__ double nan_max(double m, double x) { return (x > m ||
 constexpr int kGTY = 8, kGTZ = 32, kGW = SW_GP_WINDOW, kGXC = SW_GP_XCHUNK;
 constexpr int kGHY = kGTY + 2, kGHZ = kGTZ + 2, kGHV = kGHY * kGHZ;
 constexpr int kGThreads = kGTY * kGTZ;
-constexpr int kGFetch = (kGHV * kGW + kGThreads - 1) / kGThreads;  // staged elements per thread
+constexpr int kGStage = (kGHV + kGThreads - 1) / kGThreads;  // staged voxels per thread
+constexpr int kGPitch = kGHV + 4;                            // doubles per sample row of a staged plane
 static_assert(kGThreads == 256, "gradient-statistics block");
+static_assert(kGW % 4 == 0, "gradient-statistics window: whole 16-byte runs");
 
 struct GpLds {
-    double s[3][kGHV][kGW + 1];  // strain of (plane slot, halo'd tile voxel, sample); +1 spreads the banks
+    double s[3][kGW][kGPitch];  // strain of (plane slot, sample, halo'd tile voxel)
 };
 
+// vec: the gate is the contiguous range [g0, g0 + n_gate) of a record whose
+// windows start on 16-byte boundaries (p aligned, nt and g0 multiples of the
+// values per 16 bytes); checked on the host.
 template <typename T>
 __global__ void __launch_bounds__(kGThreads) strain_gradient_peak_kernel(
     const T* __restrict__ p, Grid g, int64_t nt, const int32_t* __restrict__ gate, int64_t g0, int64_t n_gate,
-    const T* __restrict__ c, const T* __restrict__ rho, double h2x, double h2y, double h2z, int nty, int ntz,
+    const T* __restrict__ c, const T* __restrict__ rho, double h2x, double h2y, double h2z, int nty, int ntz, int vec,
     double* __restrict__ peak_mag, double* __restrict__ peak_gx, double* __restrict__ peak_gy,
     double* __restrict__ peak_gz, double* __restrict__ rms_mag) {
     __shared__ GpLds L;
@@ -521,51 +528,65 @@ __global__ void __launch_bounds__(kGThreads) strain_gradient_peak_kernel(
     const int64_t P = g.plane();
     const int hv = (tid / kGTZ + 1) * kGHZ + tid % kGTZ + 1;  // this column in the halo'd tile
     auto slot = [&](int x) { return (x - x0 + 3) % 3; };
-    // Staging of plane xs (tile + halo, samples k0 .. k0 + kGW - 1) in two
-    // halves: fetch issues every load of the thread's elements, commit divides
-    // and writes LDS, so the loads of plane x + 2 are in flight while plane x
-    // is computed. Element i of this thread (e = tid + i * kGThreads) is voxel
-    // e / kGW of the halo'd tile and sample tid % kGW of the window (the same
-    // sample for all i, as kGThreads is a multiple of kGW); eoff is its voxel's
-    // offset in a plane, -1 outside the grid (such an element reads voxel 0 and
-    // is not committed).
-    int32_t eoff[kGFetch];
+    // The voxels this thread stages: halo'd tile voxels tid and tid + 256;
+    // soff is the voxel's offset in a plane, -1 outside the grid or the tile
+    // (such a voxel reads voxel 0 of the plane and is not committed).
+    int32_t soff[kGStage];
 #pragma unroll
-    for (int i = 0; i < kGFetch; ++i) {
-        const int e = tid + i * kGThreads, h = e / kGW;
+    for (int i = 0; i < kGStage; ++i) {
+        const int h = tid + i * kGThreads;
         const int yy = y0 - 1 + h / kGHZ, zz = z0 - 1 + h % kGHZ;
-        const bool ok = e < kGHV * kGW && yy >= 0 && yy < g.ny && zz >= 0 && zz < g.nz;
-        eoff[i] = ok ? yy * g.nz + zz : -1;  // planes hold fewer than 2^31 voxels (checked on the host)
+        const bool ok = h < kGHV && yy >= 0 && yy < g.ny && zz >= 0 && zz < g.nz;
+        soff[i] = ok ? yy * g.nz + zz : -1;  // planes hold fewer than 2^31 voxels (checked on the host)
     }
-    const int u_el = tid % kGW;
+    using V = typename Vec16<T>::type;
+    constexpr int VN = Vec16<T>::n;
     struct Fetch {
-        T p[kGFetch], r[kGFetch], c[kGFetch];
+        T p[kGStage][kGW], r[kGStage], c[kGStage];
     };
-    auto fetch = [&](int xs, int64_t t_el, Fetch& f) {
+    // Staging of plane xs (samples k0 .. k0 + kGW - 1) in two halves: fetch
+    // issues every load, commit divides and writes LDS, so the loads of plane
+    // x + 2 are in flight while plane x is computed. Past the gate's end
+    // (the last, partial window) a voxel reads its last gated sample again.
+    auto fetch = [&](int xs, int64_t k0, bool full, Fetch& f) {
         if (xs < 0 || xs >= g.nx) return;  // block-uniform
         int64_t base = (int64_t)xs * P;
         // Opaque to the optimiser (an empty asm, no instruction): otherwise it
-        // hoists each unrolled plane's element addresses out of the window loop,
-        // the kernel needs 256 VGPRs and one wave per SIMD instead of 252 and two,
-        // and runs at 5.0 ms instead of 3.0 ms on a 160^3 x 256 record.
+        // hoists each unrolled plane's addresses out of the window loop and
+        // runs out of registers.
         asm volatile("" : "+s"(base));
 #pragma unroll
-        for (int i = 0; i < kGFetch; ++i) {
-            const bool ok = eoff[i] >= 0 && t_el >= 0;
-            const int64_t v = ok ? base + eoff[i] : 0;
-            f.p[i] = p[v * nt + (ok ? t_el : 0)];
+        for (int i = 0; i < kGStage; ++i) {
+            const int64_t v = base + (soff[i] >= 0 ? soff[i] : 0);
             f.r[i] = rho[v];
             f.c[i] = c[v];
+            const T* pv = p + v * nt;
+            if (vec && full) {  // block-uniform
+                const V* q = reinterpret_cast<const V*>(pv + g0 + k0);
+#pragma unroll
+                for (int w = 0; w < kGW / VN; ++w) {
+                    const V t = q[w];
+#pragma unroll
+                    for (int e = 0; e < VN; ++e) f.p[i][w * VN + e] = t[e];
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < kGW; ++u)
+                    f.p[i][u] = pv[gate_sample(gate, g0, min<int64_t>(k0 + u, n_gate - 1))];
+            }
         }
     };
-    auto commit = [&](int xs, int64_t t_el, const Fetch& f) {
+    auto commit = [&](int xs, const Fetch& f) {
         if (xs < 0 || xs >= g.nx) return;  // block-uniform
-        double(*dst)[kGW + 1] = L.s[slot(xs)];
+        double(*dst)[kGPitch] = L.s[slot(xs)];
 #pragma unroll
-        for (int i = 0; i < kGFetch; ++i) {
-            if (eoff[i] < 0 || t_el < 0) continue;
+        for (int i = 0; i < kGStage; ++i) {
+            if (soff[i] < 0) continue;
+            const int h = tid + i * kGThreads;
             // -p / K in float64, K = rho c^2 promoted as float64 / float32 in NumPy (acoustic_strain.py:258-262)
-            dst[(tid + i * kGThreads) / kGW][u_el] = -(double)f.p[i] / (double)(f.r[i] * (f.c[i] * f.c[i]));
+            const double K = (double)(f.r[i] * (f.c[i] * f.c[i]));
+#pragma unroll
+            for (int u = 0; u < kGW; ++u) dst[u][h] = -(double)f.p[i][u] / K;
         }
     };
     double pm[kGXC], px[kGXC], py[kGXC], pz[kGXC], sum[kGXC];
@@ -574,28 +595,28 @@ __global__ void __launch_bounds__(kGThreads) strain_gradient_peak_kernel(
     Fetch f;
     for (int64_t k0 = 0; k0 < n_gate; k0 += kGW) {
         const int nw = (int)min<int64_t>(kGW, n_gate - k0);
-        const int64_t t_el = k0 + u_el < n_gate ? gate_sample(gate, g0, k0 + u_el) : -1;  // this thread's sample
-        fetch(x0 - 1, t_el, f);
-        commit(x0 - 1, t_el, f);
-        fetch(x0, t_el, f);
-        commit(x0, t_el, f);
-        fetch(x0 + 1, t_el, f);
+        const bool full = nw == kGW;
+        fetch(x0 - 1, k0, full, f);
+        commit(x0 - 1, f);
+        fetch(x0, k0, full, f);
+        commit(x0, f);
+        fetch(x0 + 1, k0, full, f);
 #pragma unroll
         for (int j = 0; j < kGXC; ++j) {
             const int x = x0 + j;
             if (x >= x1) break;  // block-uniform
-            commit(x + 1, t_el, f);
-            if (j + 1 < kGXC && x + 1 < x1) fetch(x + 2, t_el, f);
+            commit(x + 1, f);
+            if (j + 1 < kGXC && x + 1 < x1) fetch(x + 2, k0, full, f);
             __syncthreads();
             if (col && g.interior(x, y, z)) {
-                const double(*sp)[kGW + 1] = L.s[slot(x + 1)];
-                const double(*sc)[kGW + 1] = L.s[slot(x)];
-                const double(*sm)[kGW + 1] = L.s[slot(x - 1)];
+                const double(*sp)[kGPitch] = L.s[slot(x + 1)];
+                const double(*sc)[kGPitch] = L.s[slot(x)];
+                const double(*sm)[kGPitch] = L.s[slot(x - 1)];
                 for (int u = 0; u < nw; ++u) {
                     // gradient_center of the strain (solver.py:726-728), acoustic_strain.py:262-270
-                    const double ax = (sp[hv][u] - sm[hv][u]) / h2x;
-                    const double ay = (sc[hv + kGHZ][u] - sc[hv - kGHZ][u]) / h2y;
-                    const double az = (sc[hv + 1][u] - sc[hv - 1][u]) / h2z;
+                    const double ax = (sp[u][hv] - sm[u][hv]) / h2x;
+                    const double ay = (sc[u][hv + kGHZ] - sc[u][hv - kGHZ]) / h2y;
+                    const double az = (sc[u][hv + 1] - sc[u][hv - 1]) / h2z;
                     const double m = std::sqrt(ax * ax + ay * ay + az * az);
                     pm[j] = nan_max(pm[j], m);
                     px[j] = nan_max(px[j], std::fabs(ax));
@@ -745,11 +766,13 @@ int sw_acoustic_strain_gradient_peak(int dtype, const void* p, int64_t nx, int64
     hipStream_t s = (hipStream_t)stream;
     const Grid g{(int)nx, (int)ny, (int)nz};
     const int nty = (g.ny + kGTY - 1) / kGTY, ntz = (g.nz + kGTZ - 1) / kGTZ, ntx = (g.nx + kGXC - 1) / kGXC;
-    SW_DISPATCH(dtype, T,
-                hipLaunchKernelGGL(strain_gradient_peak_kernel<T>, dim3((unsigned)(nty * ntz * ntx)), dim3(kGThreads),
-                                   0, s, (const T*)p, g, nt, gate, gate_start, n_gate, (const T*)c, (const T*)rho,
-                                   2.0 * dx, 2.0 * dy, 2.0 * dz, nty, ntz, peak_mag, peak_gx, peak_gy, peak_gz,
-                                   rms_mag));
+    SW_DISPATCH(dtype, T, {
+        constexpr int N = Vec16<T>::n;
+        const int vec = !gate && (uintptr_t)p % 16 == 0 && nt % N == 0 && gate_start % N == 0;
+        hipLaunchKernelGGL(strain_gradient_peak_kernel<T>, dim3((unsigned)(nty * ntz * ntx)), dim3(kGThreads), 0, s,
+                           (const T*)p, g, nt, gate, gate_start, n_gate, (const T*)c, (const T*)rho, 2.0 * dx,
+                           2.0 * dy, 2.0 * dz, nty, ntz, vec, peak_mag, peak_gx, peak_gy, peak_gz, rms_mag);
+    });
     SW_LAUNCH_CHECK();
     return SW_OK;
 }

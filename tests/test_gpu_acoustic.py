@@ -62,6 +62,37 @@ def test_gradient_statistics_bit_exact(sw, dtype, gate):
         np.testing.assert_array_equal(got[k], want[k])
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("gate", ALIGNED_GATES + [slice(8, 44)])
+def test_gradient_statistics_aligned_gates(sw, dtype, gate):
+    """Contiguous gates on 16-byte boundaries (the staging reads each voxel's window of samples in 16-byte
+    runs; slice(8, 44) ends in a partial window), on a grid with partial tiles in y and z."""
+    p, c, rho = record((10, 13, 37), 48, 3, dtype)
+    got = sw.compute_acoustic_strain_gradient_peak(p, c, rho, *H, gate)
+    want = A.compute_acoustic_strain_gradient_peak(p, c, rho, *H, gate)
+    for k in want:
+        np.testing.assert_array_equal(got[k], want[k])
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("gate", [slice(None), slice(8, 40)])
+def test_gradient_statistics_propagate_nan(sw, dtype, gate):
+    """np.maximum propagates NaN: a NaN sample, an infinite one (inf - inf in a gradient) and a zero bulk
+    modulus (0/0 and p/0 strains) give NaN and inf statistics exactly where NumPy does."""
+    p, c, rho = record((9, 10, 12), 48, 4, dtype)
+    p[3, 4, 5, 20] = np.nan
+    p[6, 2, 7, 30] = np.inf
+    p[6, 3, 7, 30] = np.inf
+    c[2, 7, 3] = 0.0
+    p[2, 7, 3, :10] = 0.0
+    with np.errstate(all="ignore"):
+        got = sw.compute_acoustic_strain_gradient_peak(p, c, rho, *H, gate)
+        want = A.compute_acoustic_strain_gradient_peak(p, c, rho, *H, gate)
+    for k in want:
+        assert np.isnan(want[k]).any()
+        np.testing.assert_array_equal(got[k], want[k])
+
+
 @pytest.mark.parametrize("dtype,tol", [(np.float32, 1e-6), (np.float64, 1e-12)])
 @pytest.mark.parametrize("gate", GATES + ALIGNED_GATES)
 def test_peak_rms_mean(sw, dtype, tol, gate):
