@@ -804,9 +804,11 @@ def bench_force4d(cpu_steps=1, window=16, proj_steps=(0, 33, 66)):
     each step's slice cut and uploaded from the host and projected on its own
     (``sw_resident_force`` False as well). The projections alone, with HIP
     events: a window's batched CG, and single steps' CG; GB/s at 34 B per
-    voxel-iteration, the float32-vector CG's traffic. CPU baseline: the
-    float32 oracle (the reference's arithmetic) over `cpu_steps` steps of the
-    same slices, one core.
+    voxel-iteration, the float32-vector CG's traffic. One reprojection
+    (``reproject_every``) of a state of the same grid at the same defaults.
+    CPU baseline: the float32 oracle (the reference's arithmetic) over
+    `cpu_steps` steps of the same slices, and one projection of the
+    reprojection's, one core.
     """
     import torch
 
@@ -862,9 +864,32 @@ def bench_force4d(cpu_steps=1, window=16, proj_steps=(0, 33, 66)):
         it = max(1, info["iters"])
         singles.append({"step": k, "iters": info["iters"], "ms": round(ms, 3), "us_per_iter": round(ms * 1e3 / it, 2),
                         "GBs": round(34 * vox * it / (ms / 1e3) / 1e9, 1)})
+    # reproject_every (solver.py:236-239): one reprojection of a state of this grid, u and u_prev each projected
+    # at the same defaults (float64 fields and CG vectors in the increment form)
+    rng = np.random.default_rng(6)
+    u0 = (1e-6 * rng.standard_normal(w["shape"] + (3,))).astype(np.float32)
+    v0 = (1e-3 * rng.standard_normal(w["shape"] + (3,))).astype(np.float32)
+    s.set_initial(u0, v0)
+    s.reproject(w["tol"], w["max_iters"])  # warm
+    s.set_initial(u0, v0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rinfo = s.reproject(w["tol"], w["max_iters"])
+    torch.cuda.synchronize()
+    t_rep = time.perf_counter() - t0
+    reproj = {"ms": round(t_rep * 1e3, 2), "cg_iters": [i["iters"] for i in rinfo],
+              "note": "u and u_prev = u - d projected in float64 (fields and CG vectors), d re-formed in float64"}
     del s, bz_d
     torch.cuda.empty_cache()
     cpu = None
+    if cpu_steps > 0:  # the oracle's projection, the unit of a reprojection (two per reprojection)
+        k = 20
+        t0 = time.perf_counter()
+        O.project_vector_field(u0[..., 0], u0[..., 1], u0[..., 2], h, h, h, w["tol"], k, dtype=np.float32)
+        t_k = time.perf_counter() - t0
+        reproj["cpu_ms"] = round(t_k / k * sum(reproj["cg_iters"]) * 1e3, 1)
+        reproj["cpu_note"] = (f"float32 oracle: one projection of u0 capped at {k} CG iterations ({t_k:.2f} s), "
+                              "extrapolated to the GPU run's iterations of both projections, one core")
     if cpu_steps > 0:  # the float32 oracle on the first cpu_steps slices
         info_cpu = []
         t0 = time.perf_counter()
@@ -888,6 +913,7 @@ def bench_force4d(cpu_steps=1, window=16, proj_steps=(0, 33, 66)):
                                     "alg_bytes_per_voxel_iter": 34, "achieved": batched["GBs"],
                                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                     "frac": round(batched["GBs"] / HBM_PEAK_GBS, 4)}},
+        "reprojection": reproj,
         "cpu_baseline": cpu,
     }
 
