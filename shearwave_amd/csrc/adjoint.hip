@@ -104,7 +104,13 @@ __global__ void __launch_bounds__(256) adjoint_step_kernel(AdjArgs<T> a) {
 // Same products and sums as the per-voxel kernel above; the gradient sums
 // accumulate in float64.
 
-constexpr int kAdjBY = 8;
+// Tile rows: 8 in float32 (181 VGPRs for the queues); 4 in float64, whose
+// queues hold twice the registers (the compiler keeps the excess in AGPRs,
+// one wave per SIMD, no scratch; 8 rows would spill to scratch).
+template <typename T>
+constexpr int adj_by() {
+    return sizeof(T) == 4 ? 8 : 4;
+}
 
 template <typename T, bool VISC>
 struct AdjRaw {
@@ -341,31 +347,31 @@ int sw_adjoint_step(int dtype, const void* const* u, const void* const* d, const
         // the x-marching tile where it fits (one 64-wide z tile at least, planes of
         // fewer than 2^28 voxels for its 32-bit offsets), else one voxel per thread
         const char* env_k = getenv("SW_ADJ_KERNEL");
-        // (float32 only: the float64 queues would spill)
-        bool march = dtype == SW_F32 && nz >= 32 && ny >= kAdjBY && g.plane() < kMaxMarchPlane;
+        constexpr int BY = adj_by<T>();
+        bool march = nz >= 32 && ny >= BY && g.plane() < kMaxMarchPlane;
         if (env_k && env_k[0] == 'd') march = false;
         if (march) {
             auto launch = [&](auto kern) {
                 static const int res = [&] {
                     int n = 0;
-                    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, 64 * kAdjBY, 0) != hipSuccess || n < 1)
+                    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, 64 * BY, 0) != hipSuccess || n < 1)
                         n = 1;
                     return n;
                 }();
-                const int nty = (int)((ny + kAdjBY - 1) / kAdjBY), ntz = (int)((nz + 63) / 64);
+                const int nty = (int)((ny + BY - 1) / BY), ntz = (int)((nz + 63) / 64);
                 const int64_t cols = (int64_t)nty * ntz, target = (int64_t)device_cu_count() * res;
                 int64_t ntx = (target + cols - 1) / cols;
                 if (ntx < 1) ntx = 1;
                 int64_t xc = (nx + ntx - 1) / ntx;
                 if (xc < 4) xc = 4;
                 ntx = (nx + xc - 1) / xc;
-                hipLaunchKernelGGL(kern, dim3((unsigned)(cols * ntx)), dim3(64, kAdjBY), 0, (hipStream_t)stream, a,
+                hipLaunchKernelGGL(kern, dim3((unsigned)(cols * ntx)), dim3(64, BY), 0, (hipStream_t)stream, a,
                                    (int)xc, nty, ntz);
             };
             if (eta_over_rho)
-                launch(adjoint_march_kernel<T, true, kAdjBY>);
+                launch(adjoint_march_kernel<T, true, BY>);
             else
-                launch(adjoint_march_kernel<T, false, kAdjBY>);
+                launch(adjoint_march_kernel<T, false, BY>);
         } else if (eta_over_rho) {
             hipLaunchKernelGGL((adjoint_step_kernel<T, true>), dim3(row_grid(g, 0)), row_block(), 0,
                                (hipStream_t)stream, a);
