@@ -247,13 +247,18 @@ __global__ void __launch_bounds__(256) strain_peak_kernel(const T* __restrict__ 
 }
 
 // peak / rms / mean over a contiguous gate read 16 bytes per lane
-// (vec16_gate): lane sl of a voxel's S-lane segment reads chunks sl, sl + S,
+// (vec16_gate): lane sl of a voxel's S-lane segment (kPeakLanes) reads chunks sl, sl + S,
 // ... of Vec16<T>::n samples, two chunks and kPeakVU voxel groups per
 // iteration (four 16-byte loads in flight per lane), unconditional loads
 // (chunk clamped, values past the gate discarded). One maximum and two float64
 // partials per vector element, combined in a fixed order; the maximum is the
 // same in any order, NaN included.
 constexpr int kPeakVU = 2;
+// Lanes per voxel of the 16-byte kernel: 4, a 64-byte run of the record per
+// load instruction and 16 voxels per wave. At 128 gated float32 samples, 4
+// lanes took 428 us on a 160^3 x 256 record against 466 (8), 492 (2) and
+// 586 (16): more voxels per wave keep more lines in flight.
+constexpr int kPeakLanes = 4;
 
 template <typename T>
 __global__ void __launch_bounds__(256) strain_peak_vec_kernel(const T* __restrict__ p, int64_t n_vox, int64_t nt,
@@ -706,7 +711,7 @@ int sw_acoustic_strain_peak(int dtype, const void* p, int64_t n_vox, int64_t nt,
     hipStream_t s = (hipStream_t)stream;
     SW_DISPATCH(dtype, T, {
         if (!gate && vec16_gate<T>(p, nt, gate_start, n_gate)) {
-            const int S = gate_lanes(n_gate / Vec16<T>::n, 16);
+            const int S = gate_lanes(n_gate / Vec16<T>::n, kPeakLanes);
             hipLaunchKernelGGL(strain_peak_vec_kernel<T>, dim3(segment_blocks(n_vox, S)), dim3(256), 0, s,
                                (const T*)p, n_vox, nt, gate_start, n_gate, S, (const T*)c, (const T*)rho, (T*)peak,
                                (T*)rms, (T*)mean);
@@ -737,7 +742,9 @@ int sw_acoustic_strain_gradient(int dtype, const void* p, int64_t nx, int64_t ny
     SW_DISPATCH(dtype, T, {
         const bool out16 = (uintptr_t)gx % 16 == 0 && (uintptr_t)gy % 16 == 0 && (uintptr_t)gz % 16 == 0;
         if (!gate && out16 && vec16_gate<T>(p, nt, gate_start, n_gate)) {
-            const int S = gate_lanes(n_gate / Vec16<T>::n, 64);
+            // 8 lanes per voxel: 2715 us on a 160^3 x 256 float32 record at 128 gated samples, against
+            // 2891 (16), 3062 (32) and 3985 (4)
+            const int S = gate_lanes(n_gate / Vec16<T>::n, 8);
             hipLaunchKernelGGL(strain_gradient_vec_kernel<T>, dim3(segment_blocks(order_positions(g), S)), dim3(256),
                                0, s, (const T*)p, g, nt, gate_start, n_gate, S, (const T*)c, (const T*)rho,
                                (T)(2.0 * dx), (T)(2.0 * dy), (T)(2.0 * dz), (T*)gx, (T*)gy, (T*)gz);
