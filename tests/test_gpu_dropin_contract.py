@@ -238,6 +238,33 @@ def test_4d_forcing_unconverged_projections_float32(sw):
     assert np.array_equal(runs["w5"][0], runs["w5_host"][0]) and np.array_equal(runs["w5"][1], runs["w5_host"][1])
 
 
+def test_4d_forcing_windows_mixed_components_zero_steps_callback_reproject(sw):
+    """Per-step forcing through every part of the loop at once: a 4-D bz with all-zero steps (their projections
+    exit early), a 3-D bx repeated every step (solver.py:1060-1069), a callback on every step, reprojection every
+    7 steps, and windows of 6 steps' projections (the last one short); float32 within 1e-5 and float64 within
+    1e-10 of the float64 oracle, and the callback sees every step."""
+    shape, n_steps = (24, 20, 66), 20
+    rng = np.random.default_rng(35)
+    h = (0.3e-3, 0.3e-3, 0.3e-3)
+    rho = 1000.0 * (1 + rng.uniform(-0.01, 0.01, shape))
+    mu = 3000.0 + 3000.0 * rng.uniform(0, 1, shape)
+    g = gauss(shape, 3.0)[..., None]
+    ramp = np.linspace(1.0, 0.3, n_steps)
+    ramp[[4, 5, 13]] = 0.0
+    bz = 300.0 * g * ramp
+    bx = 40.0 * gauss(shape, 4.0) + rng.standard_normal(shape) * taper(shape)
+    opts = {"eta": 1.0, "vel_damp": 0.01, "poisson_tol": 1e-6, "poisson_max_iters": 40, "reproject_every": 7,
+            "sw_force_window": 6}
+    u64, v64 = O.shear_fdtd(bx, None, bz, rho, mu, *h, 2e-6, n_steps, opts=opts, dtype=np.float64)
+    for dtype, tol in (("float32", 1e-5), ("float64", 1e-10)):
+        seen = []
+        u, v = sw.shear_fdtd(bx, None, bz, rho, mu, *h, 2e-6, n_steps, {**opts, "sw_dtype": dtype},
+                             callback=lambda it, uu: seen.append(it))
+        assert seen == list(range(n_steps)), (dtype, seen)
+        eu, ev = rel(u, u64), rel(v, v64)
+        assert eu < tol and ev < tol, (dtype, eu, ev)
+
+
 # --------------------------------------------------------- full-size workloads
 
 
