@@ -19,7 +19,7 @@ H = (0.5e-3, 0.5e-3, 0.5e-3)
 N_STEPS = 60
 
 
-def setup(dtype="float64", checkpoint_every=None, seed=0, shape=None):
+def setup(dtype="float64", checkpoint_every=None, seed=0, shape=None, viscous=True):
     from shearwave_amd.adjoint import Problem, plane_sensors
 
     SHAPE = shape or globals()["SHAPE"]
@@ -34,7 +34,7 @@ def setup(dtype="float64", checkpoint_every=None, seed=0, shape=None):
     env = np.zeros(N_STEPS)
     env[:12] = 1.0
     prob = Problem(SHAPE, H, dt, rho, (None, None, bz), env, plane_sensors(SHAPE, 5), dtype=dtype, vel_damp=0.01,
-                   tol=1e-12, checkpoint_every=checkpoint_every)
+                   tol=1e-12, checkpoint_every=checkpoint_every, viscous=viscous)
     dev = torch.device("cuda")
     mu_t = torch.tensor(mu, dtype=torch.float64, device=dev)
     eta_t = torch.tensor(eta, dtype=torch.float64, device=dev)
@@ -130,6 +130,17 @@ def test_march_adjoint_matches_per_voxel_kernel(sw, monkeypatch, dtype, tol):
     assert float(gmu_d.abs().max()) > 0 and float(geta_d.abs().max()) > 0
     for a, b in ((gmu, gmu_d), (geta, geta_d)):
         assert float(torch.linalg.norm(a - b) / torch.linalg.norm(b)) < tol
+
+
+@pytest.mark.parametrize("dtype,tol", [("float32", 1e-5), ("float64", 1e-12)])
+def test_march_adjoint_inviscid_matches_per_voxel_kernel(sw, monkeypatch, dtype, tol):
+    """The inviscid x-marching adjoint (no eta fields) against the per-voxel kernel, on the mu gradient."""
+    prob, mu, eta, target = setup(dtype, shape=MARCH_SHAPE, seed=4, viscous=False)
+    _, gmu, _ = loss_and_grad(prob, mu, eta, target)
+    monkeypatch.setenv("SW_ADJ_KERNEL", "direct")
+    _, gmu_d, _ = loss_and_grad(prob, mu, eta, target)
+    assert float(gmu_d.abs().max()) > 0
+    assert float(torch.linalg.norm(gmu - gmu_d) / torch.linalg.norm(gmu_d)) < tol
 
 
 def test_march_adjoint_close_to_fp64(sw):
