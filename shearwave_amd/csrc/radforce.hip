@@ -29,29 +29,49 @@ constexpr int kEnergyThreads = 256;
 // are then summed in wave order into the block's partial.
 constexpr int kEnergyV = 4;
 
-template <typename T>
+// VEC (records with nt % 4 == 0 and a 16-byte aligned start): lane l takes
+// samples tc + 4 l + u instead of tc + l + 64 u, four consecutive samples read
+// as 16-byte runs; each sample's sum still runs over the voxels in the same
+// order, so the partials are the same bit for bit.
+template <typename T, bool VEC>
 __global__ void __launch_bounds__(kEnergyThreads) energy_partial_kernel(const T* __restrict__ p, int64_t n_vox,
                                                                         int64_t nt, double* __restrict__ part) {
     __shared__ double red[kEnergyThreads / kWave][4 * kWave];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int64_t n_waves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     const int64_t w0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    using V = typename Vec16<T>::type;
+    constexpr int N = Vec16<T>::n;
+    // sample offset (in the chunk) of this lane's value u
+    auto slot = [&](int u) { return VEC ? 4 * lane + u : lane + kWave * u; };
     for (int64_t tc = 0; tc < nt; tc += 4 * kWave) {
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
         bool tin[4];
         int64_t off[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            tin[u] = tc + lane + kWave * u < nt;
-            off[u] = tin[u] ? tc + lane + kWave * u : tc;
+            tin[u] = tc + slot(u) < nt;
+            off[u] = tin[u] ? tc + slot(u) : tc;  // with VEC, all four in or all four out (nt % 4 == 0)
         }
+        auto load = [&](int64_t vv, T (&x)[4]) {
+            if (VEC) {
+                const V* q = reinterpret_cast<const V*>(p + vv * nt + off[0]);
+#pragma unroll
+                for (int w = 0; w < 4 / N; ++w) {
+                    const V t = q[w];
+#pragma unroll
+                    for (int e = 0; e < N; ++e) x[w * N + e] = t[e];
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) x[u] = p[vv * nt + off[u]];
+            }
+        };
         int64_t v = w0;
         for (; v + (kEnergyV - 1) * n_waves < n_vox; v += kEnergyV * n_waves) {
             T x[kEnergyV][4];
 #pragma unroll
-            for (int j = 0; j < kEnergyV; ++j)
-#pragma unroll
-                for (int u = 0; u < 4; ++u) x[j][u] = p[(v + j * n_waves) * nt + off[u]];
+            for (int j = 0; j < kEnergyV; ++j) load(v + j * n_waves, x[j]);
 #pragma unroll
             for (int j = 0; j < kEnergyV; ++j)
 #pragma unroll
@@ -62,8 +82,7 @@ __global__ void __launch_bounds__(kEnergyThreads) energy_partial_kernel(const T*
         }
         for (; v < n_vox; v += n_waves) {
             T x[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) x[u] = p[v * nt + off[u]];
+            load(v, x);
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const double d = tin[u] ? (double)x[u] : 0.0;
@@ -71,7 +90,7 @@ __global__ void __launch_bounds__(kEnergyThreads) energy_partial_kernel(const T*
             }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) red[wid][lane + kWave * u] = acc[u];
+        for (int u = 0; u < 4; ++u) red[wid][slot(u)] = acc[u];
         __syncthreads();
         for (int k = threadIdx.x; k < 4 * kWave; k += blockDim.x) {
             const int64_t t = tc + k;
@@ -336,9 +355,15 @@ int sw_pressure_energy(int dtype, const void* p, int64_t n_vox, int64_t nt, doub
                (long long)(e.nb * nt * (int64_t)sizeof(double)));
     hipStream_t s = (hipStream_t)stream;
     double* part = (double*)workspace;
-    SW_DISPATCH(dtype, T,
-                hipLaunchKernelGGL(energy_partial_kernel<T>, dim3((unsigned)e.nb), dim3(kEnergyThreads), 0, s,
-                                   (const T*)p, n_vox, nt, part));
+    const bool vec = (uintptr_t)p % 16 == 0 && nt % 4 == 0;
+    SW_DISPATCH(dtype, T, {
+        if (vec)
+            hipLaunchKernelGGL((energy_partial_kernel<T, true>), dim3((unsigned)e.nb), dim3(kEnergyThreads), 0, s,
+                               (const T*)p, n_vox, nt, part);
+        else
+            hipLaunchKernelGGL((energy_partial_kernel<T, false>), dim3((unsigned)e.nb), dim3(kEnergyThreads), 0, s,
+                               (const T*)p, n_vox, nt, part);
+    });
     SW_LAUNCH_CHECK();
     const int64_t sum_blocks = nt < 65536 ? nt : 65536;
     hipLaunchKernelGGL(energy_sum_kernel, dim3((unsigned)sum_blocks), dim3(256), 0, s, part, e.nb, nt, env);
